@@ -1,0 +1,439 @@
+// Matched-filter kernels for gfx950 (MI355X).
+//
+// Forward operator (MatchedFilter_cuda_kernel.cu:65-115):
+//   out[v] = (1/A) * sum_a sum_k s[a,k] * exp(-j*2*pi*(f0 + k*dfs)*tau_av)
+// Adjoint (MatchedFilter_cuda_kernel.cu:171-220):
+//   g[a,k] = (1/norm) * sum_v gout[v] * exp(+j*2*pi*(f0 + k*dfs)*tau_av)
+//
+// Tuned forward: one voxel per lane, the wave walks the antennas in lock step,
+// so the echo row s[a,:] and the antenna positions are wave-uniform and are
+// read through the scalar data cache straight into SGPRs (no VGPR or LDS
+// traffic for the 8 bytes a term needs).  Per (voxel, antenna) pair:
+//   * distance, tau and the carrier cycles f0*tau are formed in FP64 and
+//     reduced modulo one cycle; the three phasors (carrier, per-sample step
+//     w = exp(-j*2*pi*dfs*tau) and the 32-sample step W = w^32) come from FP32
+//     sincospi of reduced arguments;
+//   * sum_k s_k w^k is evaluated by a two-level Horner scheme: 32-term inner
+//     Horner in w, outer Horner in W over the 32-sample blocks.  A Horner step
+//     acc = acc*w + s is 4 FP32 FMAs, i.e. the whole complex multiply-add per
+//     term, and the power of w never exceeds 31 (re-seeded every 32 samples),
+//     which bounds the |w|^k magnitude drift and the angle error growth;
+//   * the pair's contribution base*S is added with Kahan compensation.
+// Small volumes split the antennas over the waves of a block; the partial
+// sums are combined in FP64 through LDS in a fixed order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "abi_util.hpp"
+#include "rf_common.hpp"
+
+namespace geraf {
+namespace {
+
+constexpr int kHornerBlock = 32;
+
+// ---------------------------------------------------------------- tuned forward
+
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void mf_forward_horner_kernel(
+    const float* __restrict__ real, const float* __restrict__ imag,
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    float* __restrict__ out_real, float* __restrict__ out_imag,
+    const float* __restrict__ voxel, int A, int K, int N, int groups, PhaseConsts pc) {
+  __shared__ double partial[WAVES][kWave][2];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  // The wave index is uniform across the wave; saying so lets the antenna
+  // loop bounds, positions and echo rows live in SGPRs (scalar loads).
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int group = wave % groups;  // antenna group of this wave
+  const int tile = wave / groups;   // voxel tile (64 voxels) of this wave
+  const int tiles_per_block = WAVES / groups;
+
+  const int v = (blockIdx.x * tiles_per_block + tile) * kWave + lane;
+  const bool valid = v < N;
+  const int vc = valid ? v : (N - 1);
+
+  const double px = voxel[vc];
+  const double py = voxel[(size_t)N + vc];
+  const double pz = voxel[2 * (size_t)N + vc];
+
+  const int ants_per_group = (A + groups - 1) / groups;
+  const int a_begin = group * ants_per_group;
+  const int a_end = min(A, a_begin + ants_per_group);
+
+  const int nb = K / kHornerBlock;     // full 32-sample blocks
+  const int rem = K - nb * kHornerBlock;  // samples in the partial top block
+
+  KahanC acc;
+  acc.init();
+
+  for (int a = a_begin; a < a_end; ++a) {
+    const double d = dist3_f64(tx_pos[3 * a + 0], tx_pos[3 * a + 1], tx_pos[3 * a + 2], px, py,
+                               pz) +
+                     dist3_f64(rx_pos[3 * a + 0], rx_pos[3 * a + 1], rx_pos[3 * a + 2], px, py,
+                               pz);
+    const double tau = (d + pc.bias) * pc.inv_c;
+    const double cyc1 = pc.dfs * tau;
+    const float2 base = phasor_cycles((float)centered_frac(pc.f0 * tau), -1.0f);
+    const float2 w = phasor_cycles((float)centered_frac(cyc1), -1.0f);
+    const float2 W = phasor_cycles((float)centered_frac(kHornerBlock * cyc1), -1.0f);
+
+    const float* __restrict__ er = real + (size_t)a * K;
+    const float* __restrict__ ei = imag + (size_t)a * K;
+
+    float Sr = 0.0f, Si = 0.0f;
+    if (rem) {  // partial top block, k = K-1 .. 32*nb
+      const int k0 = nb * kHornerBlock;
+      float ir = er[K - 1], ii = ei[K - 1];
+      for (int k = K - 2; k >= k0; --k) {
+        const float u1 = fmaf(ir, w.x, er[k]);
+        const float u2 = fmaf(ii, w.x, ei[k]);
+        const float nr = fmaf(-ii, w.y, u1);
+        ii = fmaf(ir, w.y, u2);
+        ir = nr;
+      }
+      Sr = ir;
+      Si = ii;
+    }
+    for (int b = nb - 1; b >= 0; --b) {
+      const float* __restrict__ xr = er + b * kHornerBlock;
+      const float* __restrict__ xi = ei + b * kHornerBlock;
+      float ir = xr[kHornerBlock - 1], ii = xi[kHornerBlock - 1];
+#pragma unroll
+      for (int j = kHornerBlock - 2; j >= 0; --j) {
+        const float u1 = fmaf(ir, w.x, xr[j]);
+        const float u2 = fmaf(ii, w.x, xi[j]);
+        const float nr = fmaf(-ii, w.y, u1);
+        ii = fmaf(ir, w.y, u2);
+        ir = nr;
+      }
+      // S = S*W + inner
+      const float u1 = fmaf(Sr, W.x, ir);
+      const float u2 = fmaf(Si, W.x, ii);
+      const float nr = fmaf(-Si, W.y, u1);
+      Si = fmaf(Sr, W.y, u2);
+      Sr = nr;
+    }
+    acc.add(cmul(base, make_float2(Sr, Si)));
+  }
+
+  const double inv_a = 1.0 / (double)A;
+  if (groups == 1) {
+    if (valid) {
+      out_real[v] = (float)(acc.re() * inv_a);
+      out_imag[v] = (float)(acc.im() * inv_a);
+    }
+    return;
+  }
+  partial[wave][lane][0] = acc.re();
+  partial[wave][lane][1] = acc.im();
+  __syncthreads();
+  if (group == 0 && valid) {
+    double sr = 0.0, si = 0.0;
+    for (int g = 0; g < groups; ++g) {
+      sr += partial[wave + g][lane][0];
+      si += partial[wave + g][lane][1];
+    }
+    out_real[v] = (float)(sr * inv_a);
+    out_imag[v] = (float)(si * inv_a);
+  }
+}
+
+// ---------------------------------------------------------------- plain FP32 form
+
+// Reference formula: FP32 geometry, FP32 phase and a sincosf per term
+// (MatchedFilter_cuda_kernel.cu:87-114).
+__global__ __launch_bounds__(256) void mf_forward_plain_f32_kernel(
+    const float* __restrict__ real, const float* __restrict__ imag,
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    float* __restrict__ out_real, float* __restrict__ out_imag,
+    const float* __restrict__ voxel, int A, int K, int N, float f0, float slope, float fs,
+    float c, float bias) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  const float px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
+  const float two_pi = 6.28318530717958647692f;
+  float acc_r = 0.0f, acc_i = 0.0f;
+  for (int a = 0; a < A; ++a) {
+    const float dtx = tx_pos[3 * a] - px, dty = tx_pos[3 * a + 1] - py,
+                dtz = tx_pos[3 * a + 2] - pz;
+    const float drx = rx_pos[3 * a] - px, dry = rx_pos[3 * a + 1] - py,
+                drz = rx_pos[3 * a + 2] - pz;
+    const float dist = sqrtf(dtx * dtx + dty * dty + dtz * dtz) +
+                       sqrtf(drx * drx + dry * dry + drz * drz) + bias;
+    const float tau = dist / c;
+    const float base_phase = two_pi * f0 * tau;
+    for (int k = 0; k < K; ++k) {
+      const float t = (float)k / fs;
+      const float phase = base_phase + two_pi * slope * t * tau;
+      const float cs = cosf(phase), ns = -sinf(phase);
+      const float sr = real[(size_t)a * K + k], si = imag[(size_t)a * K + k];
+      acc_r += cs * sr - ns * si;
+      acc_i += cs * si + ns * sr;
+    }
+  }
+  out_real[v] = acc_r / (float)A;
+  out_imag[v] = acc_i / (float)A;
+}
+
+// ---------------------------------------------------------------- plain FP64 form
+
+__global__ __launch_bounds__(256) void mf_forward_f64_kernel(
+    const double* __restrict__ real, const double* __restrict__ imag,
+    const double* __restrict__ tx_pos, const double* __restrict__ rx_pos,
+    double* __restrict__ out_real, double* __restrict__ out_imag,
+    const double* __restrict__ voxel, int A, int K, int N, PhaseConsts pc) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
+  double acc_r = 0.0, acc_i = 0.0;
+  for (int a = 0; a < A; ++a) {
+    const double d = dist3_f64(tx_pos[3 * a], tx_pos[3 * a + 1], tx_pos[3 * a + 2], px, py, pz) +
+                     dist3_f64(rx_pos[3 * a], rx_pos[3 * a + 1], rx_pos[3 * a + 2], px, py, pz);
+    const double tau = (d + pc.bias) * pc.inv_c;
+    const double fr0 = centered_frac(pc.f0 * tau);
+    const double cyc1 = pc.dfs * tau;
+    for (int k = 0; k < K; ++k) {
+      double s, c;
+      sincospi(2.0 * centered_frac(fr0 + (double)k * cyc1), &s, &c);
+      const double sr = real[(size_t)a * K + k], si = imag[(size_t)a * K + k];
+      // s * exp(-j*phi) = (sr + j si)(c - j s)
+      acc_r += c * sr + s * si;
+      acc_i += c * si - s * sr;
+    }
+  }
+  out_real[v] = acc_r / (double)A;
+  out_imag[v] = acc_i / (double)A;
+}
+
+// ---------------------------------------------------------------- adjoint
+
+// Block = (antenna a, 32-sample chunk).  Each lane walks a strided subset of
+// the voxels, rotating its seeded phasor through the 32 samples of the chunk
+// and accumulating into 32 private complex registers; the block then reduces
+// them (wave shuffles, then LDS across waves) in a fixed order.
+constexpr int kAdjChunk = 32;
+constexpr int kAdjWaves = 4;
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+
+__global__ __launch_bounds__(64 * kAdjWaves) void mf_backward_kernel(
+    const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    float* __restrict__ grad_real, float* __restrict__ grad_imag,
+    const float* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc) {
+  __shared__ float red[kAdjWaves][kAdjChunk][2];
+  const int a = blockIdx.x;
+  const int k0 = blockIdx.y * kAdjChunk;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x >> 6;
+
+  const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+  const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+
+  float pr[kAdjChunk], pi[kAdjChunk];
+#pragma unroll
+  for (int j = 0; j < kAdjChunk; ++j) pr[j] = pi[j] = 0.0f;
+
+  for (int v = threadIdx.x; v < N; v += blockDim.x) {
+    const float gr = gout_real[v], gi = gout_imag[v];
+    const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
+    const double d = dist3_f64(tx, ty, tz, px, py, pz) + dist3_f64(rx, ry, rz, px, py, pz);
+    const double tau = (d + pc.bias) * pc.inv_c;
+    const double cyc1 = pc.dfs * tau;
+    const float2 seed =
+        phasor_cycles((float)centered_frac(pc.f0 * tau + (double)k0 * cyc1), 1.0f);
+    const float2 w = phasor_cycles((float)centered_frac(cyc1), 1.0f);
+    float2 z = cmul(make_float2(gr, gi), seed);
+#pragma unroll
+    for (int j = 0; j < kAdjChunk; ++j) {
+      pr[j] += z.x;
+      pi[j] += z.y;
+      z = cmul(z, w);
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < kAdjChunk; ++j) {
+    const float sr = wave_sum(pr[j]);
+    const float si = wave_sum(pi[j]);
+    if (lane == 0) {
+      red[wave][j][0] = sr;
+      red[wave][j][1] = si;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kAdjChunk) {
+    const int j = threadIdx.x;
+    const int k = k0 + j;
+    if (k < K) {
+      double sr = 0.0, si = 0.0;
+      for (int w = 0; w < kAdjWaves; ++w) {
+        sr += red[w][j][0];
+        si += red[w][j][1];
+      }
+      grad_real[(size_t)a * K + k] = (float)(sr * inv_norm);
+      grad_imag[(size_t)a * K + k] = (float)(si * inv_norm);
+    }
+  }
+}
+
+// Plain FP64 adjoint: one thread per (a, k), FP64 phase and sincos per voxel.
+__global__ __launch_bounds__(256) void mf_backward_f64_kernel(
+    const double* __restrict__ gout_real, const double* __restrict__ gout_imag,
+    const double* __restrict__ tx_pos, const double* __restrict__ rx_pos,
+    double* __restrict__ grad_real, double* __restrict__ grad_imag,
+    const double* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)A * K) return;
+  const int a = (int)(idx / K);
+  const int k = (int)(idx % K);
+  const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+  const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+  double acc_r = 0.0, acc_i = 0.0;
+  for (int v = 0; v < N; ++v) {
+    const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
+    const double d = dist3_f64(tx, ty, tz, px, py, pz) + dist3_f64(rx, ry, rz, px, py, pz);
+    const double tau = (d + pc.bias) * pc.inv_c;
+    double s, c;
+    sincospi(2.0 * centered_frac(centered_frac(pc.f0 * tau) + (double)k * (pc.dfs * tau)), &s,
+             &c);
+    const double gr = gout_real[v], gi = gout_imag[v];
+    acc_r += c * gr - s * gi;
+    acc_i += s * gr + c * gi;
+  }
+  grad_real[idx] = acc_r * inv_norm;
+  grad_imag[idx] = acc_i * inv_norm;
+}
+
+int pick_groups(int A, int N, int waves) {
+  // Enough waves to give every SIMD of the 256-CU part a few of them.
+  const long long want = 4LL * 1024;
+  const long long tiles = (N + kWave - 1) / kWave;
+  int g = 1;
+  while (g < waves && g * 2 <= A && tiles * g < want) g *= 2;
+  return g;
+}
+
+}  // namespace
+}  // namespace geraf
+
+using namespace geraf;
+
+extern "C" int geraf_mf_forward_f32(const float* real, const float* imag, const float* tx_pos,
+                                    const float* rx_pos, float* out_real, float* out_imag,
+                                    const float* voxel, int64_t A, int64_t K, int64_t N,
+                                    const geraf_fmcw_params* params, geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_mf_sizes(A, K, N);
+  check_params(params);
+  if (N == 0) return GERAF_RF_OK;
+  require_ptrs({real, imag, tx_pos, rx_pos}, A * K > 0 || A > 0);
+  require_ptrs({out_real, out_imag, voxel}, true);
+  const PhaseConsts pc = make_phase_consts(*params);
+  const int n = (int)N;
+  const int tiles = (n + kWave - 1) / kWave;
+  if (tiles < 4 * 1024 && A > 1) {
+    constexpr int W = 16;
+    const int g = pick_groups((int)A, n, W);
+    const int tiles_per_block = W / g;
+    const int blocks = (tiles + tiles_per_block - 1) / tiles_per_block;
+    hipLaunchKernelGGL(mf_forward_horner_kernel<W>, dim3(blocks), dim3(64 * W), 0,
+                       (hipStream_t)stream, real, imag, tx_pos, rx_pos, out_real, out_imag,
+                       voxel, (int)A, (int)K, n, g, pc);
+  } else {
+    constexpr int W = 4;
+    const int blocks = (tiles + W - 1) / W;
+    hipLaunchKernelGGL(mf_forward_horner_kernel<W>, dim3(blocks), dim3(64 * W), 0,
+                       (hipStream_t)stream, real, imag, tx_pos, rx_pos, out_real, out_imag,
+                       voxel, (int)A, (int)K, n, 1, pc);
+  }
+  return check_launch("mf_forward_horner_kernel");
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_mf_forward_plain_f32(const float* real, const float* imag,
+                                          const float* tx_pos, const float* rx_pos,
+                                          float* out_real, float* out_imag, const float* voxel,
+                                          int64_t A, int64_t K, int64_t N,
+                                          const geraf_fmcw_params* params,
+                                          geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_mf_sizes(A, K, N);
+  check_params(params);
+  if (N == 0) return GERAF_RF_OK;
+  require_ptrs({real, imag, tx_pos, rx_pos}, A > 0);
+  require_ptrs({out_real, out_imag, voxel}, true);
+  const int blocks = (int)((N + 255) / 256);
+  hipLaunchKernelGGL(mf_forward_plain_f32_kernel, dim3(blocks), dim3(256), 0,
+                     (hipStream_t)stream, real, imag, tx_pos, rx_pos, out_real, out_imag, voxel,
+                     (int)A, (int)K, (int)N, (float)params->f0, (float)params->slope,
+                     (float)params->fs, (float)params->c, (float)params->path_bias);
+  return check_launch("mf_forward_plain_f32_kernel");
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_mf_forward_f64(const double* real, const double* imag,
+                                    const double* tx_pos, const double* rx_pos,
+                                    double* out_real, double* out_imag, const double* voxel,
+                                    int64_t A, int64_t K, int64_t N,
+                                    const geraf_fmcw_params* params, geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_mf_sizes(A, K, N);
+  check_params(params);
+  if (N == 0) return GERAF_RF_OK;
+  require_ptrs({real, imag, tx_pos, rx_pos}, A > 0);
+  require_ptrs({out_real, out_imag, voxel}, true);
+  const int blocks = (int)((N + 255) / 256);
+  hipLaunchKernelGGL(mf_forward_f64_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     real, imag, tx_pos, rx_pos, out_real, out_imag, voxel, (int)A, (int)K,
+                     (int)N, make_phase_consts(*params));
+  return check_launch("mf_forward_f64_kernel");
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_mf_backward_f32(const float* grad_out_real, const float* grad_out_imag,
+                                     const float* tx_pos, const float* rx_pos,
+                                     float* grad_real, float* grad_imag, const float* voxel,
+                                     int64_t A, int64_t K, int64_t N, int64_t norm_count,
+                                     const geraf_fmcw_params* params, geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_mf_sizes(A, K, N);
+  check_params(params);
+  if (A == 0 || K == 0) return GERAF_RF_OK;
+  require_ptrs({tx_pos, rx_pos, grad_real, grad_imag}, true);
+  require_ptrs({grad_out_real, grad_out_imag, voxel}, N > 0);
+  const dim3 grid((unsigned)A, (unsigned)((K + kAdjChunk - 1) / kAdjChunk));
+  hipLaunchKernelGGL(mf_backward_kernel, grid, dim3(64 * kAdjWaves), 0, (hipStream_t)stream,
+                     grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag, voxel,
+                     (int)A, (int)K, (int)N, 1.0 / (double)norm_count,
+                     make_phase_consts(*params));
+  return check_launch("mf_backward_kernel");
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_mf_backward_f64(const double* grad_out_real, const double* grad_out_imag,
+                                     const double* tx_pos, const double* rx_pos,
+                                     double* grad_real, double* grad_imag, const double* voxel,
+                                     int64_t A, int64_t K, int64_t N, int64_t norm_count,
+                                     const geraf_fmcw_params* params, geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_mf_sizes(A, K, N);
+  check_params(params);
+  if (A == 0 || K == 0) return GERAF_RF_OK;
+  require_ptrs({tx_pos, rx_pos, grad_real, grad_imag}, true);
+  require_ptrs({grad_out_real, grad_out_imag, voxel}, N > 0);
+  const long long total = A * K;
+  hipLaunchKernelGGL(mf_backward_f64_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                     0, (hipStream_t)stream, grad_out_real, grad_out_imag, tx_pos, rx_pos,
+                     grad_real, grad_imag, voxel, (int)A, (int)K, (int)N,
+                     1.0 / (double)norm_count, make_phase_consts(*params));
+  return check_launch("mf_backward_f64_kernel");
+  GERAF_TRY_END
+}
