@@ -1,0 +1,457 @@
+// Fused lensless RF renderer for gfx950: per-ray NeuS-style compositing and
+// FMCW signal synthesis in one kernel, with a hand-written adjoint.
+//
+// Per ray r (monostatic antenna at its origin o, unit direction d) and sample s:
+//   c      = d . g_s                                 (g = raw SDF gradient)
+//   ic     = -(relu(0.5 - 0.5 c)(1 - r_a) + relu(-c) r_a)       rf_rendering.py:145-155
+//   prev   = sdf - ic*dist/2,  next = sdf + ic*dist/2
+//   alpha  = clip((sig(prev*inv_s) - sig(next*inv_s) + 1e-5)
+//                 / (sig(prev*inv_s) + 1e-5), 0, 1)             rf_rendering.py:157-169
+//   T      = prod_{i<s} (1 - alpha_i + 1e-7)                    rf_rendering.py:46-52
+//   Tc     = T + offset_s        (calibration new_cdf - prev_cdf, rf_rendering.py:750)
+//   sigma  = max(refl*power*Tc, 1e-6) * alpha * Tc              sdf_network.py:475-481
+//   n      = g / max(|g|, 1e-12)                                 F.normalize
+//   amp    = sigma * spec(o, o, n, o + z d) / d_path^2,  d_path = 2|z d|
+//   out[r,k] = (1/n_valid) sum_s amp_s exp(+j 2 pi (f0 + k dfs) tau_s)
+//                                         (SignalTracer_cuda_kernel.cu:351-450, one antenna)
+//
+// Layout: one wave per ray, four rays per block; samples are streamed in
+// 64-sample tiles (one sample per lane, coalesced loads).  The transmittance
+// is an exclusive prefix product over the tile (wave shuffles) times the carry
+// of the earlier tiles.
+// Forward signal: each tile's per-sample (amplitude, reduced carrier phase,
+// per-sample phase step) goes to LDS; lane (chunk j, group g) then takes the
+// tile's samples s = g (mod G), seeds exp(j*2*pi*(f0 + 32 j dfs) tau_s) from
+// the FP64-reduced phase and rotates it through its 32 frequencies into 32
+// private complex accumulators; the G groups are summed by shuffles at the end.
+// Backward: a forward pass keeps only the transmittance carry at each tile
+// start; a reverse pass over the tiles recomputes everything, evaluates
+// h_s = Re sum_k g[r,k] exp(-j phi_sk) by two-level Horner (gradient row read
+// through the scalar cache), and back-propagates through the transmittance
+// with a reverse affine scan W_s = gT_s + t_s W_{s+1}, so that
+// dL/dalpha_i = (direct) - T_i W_{i+1} needs no division by (1 - alpha + 1e-7).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "abi_util.hpp"
+#include "rf_common.hpp"
+
+namespace geraf {
+namespace {
+
+constexpr int kRaysPerBlock = 4;
+constexpr int kTile = 64;
+constexpr int kMaxTiles = 64;  // S <= 4096
+constexpr int kSigChunk = 32;
+
+struct LenslessArgs {
+  const float* sdf;           // [R,S]
+  const float* grad;          // [R,S,3]
+  const float* refl;          // [R,S] or null (= 1)
+  const float* dists;         // [R,S]
+  const float* z;             // [R,S]
+  const float* origin;        // [R,3]
+  const float* dirs;          // [R,3]
+  const float* offset;        // [R,S] or null (= 0)
+  const int32_t* n_valid;     // [R] or null (= S)
+  const float* inv_s;         // [1]
+  const float* power;         // [1]
+  int R, S, K;
+  float anneal;
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+struct SampleState {
+  // inputs
+  float sdf, dist, refl, off, zz;
+  V3 g;
+  // compositing
+  float c, ic, prev, next, pc, nc, alpha, t;
+  bool qmask;
+};
+
+__device__ __forceinline__ void load_and_composite(const LenslessArgs& a, size_t idx, V3 dir,
+                                                   float inv_s, SampleState& x) {
+  x.sdf = a.sdf[idx];
+  x.dist = a.dists[idx];
+  x.refl = a.refl ? a.refl[idx] : 1.0f;
+  x.off = a.offset ? a.offset[idx] : 0.0f;
+  x.zz = a.z[idx];
+  x.g = load3(a.grad + 3 * idx);
+  x.c = dot(dir, x.g);
+  x.ic = -(fmaxf(-x.c * 0.5f + 0.5f, 0.0f) * (1.0f - a.anneal) + fmaxf(-x.c, 0.0f) * a.anneal);
+  const float half = x.ic * x.dist * 0.5f;
+  x.next = x.sdf + half;
+  x.prev = x.sdf - half;
+  x.pc = sigmoidf(x.prev * inv_s);
+  x.nc = sigmoidf(x.next * inv_s);
+  const float q = (x.pc - x.nc + 1.0e-5f) / (x.pc + 1.0e-5f);
+  x.qmask = (q >= 0.0f) && (q <= 1.0f);
+  x.alpha = fminf(fmaxf(q, 0.0f), 1.0f);
+  x.t = 1.0f - x.alpha + 1.0e-7f;
+}
+
+// Inclusive prefix product over the wave.
+__device__ __forceinline__ float wave_incl_prod(float v, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const float y = __shfl_up(v, off, kWave);
+    if (lane >= off) v *= y;
+  }
+  return v;
+}
+
+struct Emission {
+  V3 n;          // normalised normal
+  float gl;      // |g|
+  SpecGeom geo;
+  double dpath;  // round-trip distance (no bias)
+  bool live;     // valid and not culled
+};
+
+__device__ __forceinline__ Emission emission(V3 o, V3 dir, const SampleState& x, bool valid) {
+  Emission e;
+  e.gl = sqrtf(dot(x.g, x.g));
+  e.n = (1.0f / fmaxf(e.gl, 1.0e-12f)) * x.g;
+  // point = o + z*d with separately rounded product and sum (as a tensor op would)
+  const V3 zd = v3(__fmul_rn(x.zz, dir.x), __fmul_rn(x.zz, dir.y), __fmul_rn(x.zz, dir.z));
+  const V3 p = v3(__fadd_rn(o.x, zd.x), __fadd_rn(o.y, zd.y), __fadd_rn(o.z, zd.z));
+  e.geo = spec_geom(o, o, e.n, p);
+  e.dpath = 2.0 * dist3_f64(o, p);
+  e.live = valid && !e.geo.culled();
+  return e;
+}
+
+// ------------------------------------------------------------------ forward
+
+__global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
+    LenslessArgs a, float* __restrict__ out_real, float* __restrict__ out_imag,
+    float* __restrict__ sigma_out, PhaseConsts pc, int groups, int chunks) {
+  __shared__ float s_amp[kRaysPerBlock][kTile];
+  __shared__ float s_fr0[kRaysPerBlock][kTile];
+  __shared__ double s_cyc1[kRaysPerBlock][kTile];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = blockIdx.x * kRaysPerBlock + wave;
+  const bool ray_ok = r < a.R;
+  const int rr = ray_ok ? r : a.R - 1;
+  const int nv = a.n_valid ? min(max(a.n_valid[rr], 0), a.S) : a.S;
+  const V3 o = load3(a.origin + 3 * (size_t)rr);
+  const V3 dir = load3(a.dirs + 3 * (size_t)rr);
+  const float inv_s = a.inv_s[0];
+  const float power = a.power[0];
+
+  const int grp = lane % groups;
+  const int chunk = lane / groups;
+  const bool chunk_ok = chunk < chunks;
+  const int k0 = chunk * kSigChunk;
+
+  float acc_r[kSigChunk], acc_i[kSigChunk];
+#pragma unroll
+  for (int j = 0; j < kSigChunk; ++j) acc_r[j] = acc_i[j] = 0.0f;
+
+  float carry = 1.0f;
+  for (int base = 0; base < nv; base += kTile) {
+    const int s = base + lane;
+    const bool valid = s < nv;
+    const size_t idx = (size_t)rr * a.S + (valid ? s : nv - 1);
+    SampleState x;
+    load_and_composite(a, idx, dir, inv_s, x);
+    const float tt = valid ? x.t : 1.0f;
+    const float incl = wave_incl_prod(tt, lane);
+    float excl = __shfl_up(incl, 1, kWave);
+    if (lane == 0) excl = 1.0f;
+    const float T = carry * excl;
+    carry *= __shfl(incl, kWave - 1, kWave);
+    const float Tc = T + x.off;
+    const float sigma = fmaxf(x.refl * power * Tc, 1.0e-6f) * (x.alpha * Tc);
+    const Emission e = emission(o, dir, x, valid);
+    float amp = 0.0f, fr0 = 0.0f;
+    double cyc1 = 0.0;
+    if (e.live) {
+      amp = sigma * e.geo.spec / (float)(e.dpath * e.dpath);
+      const double tau = (e.dpath + pc.bias) * pc.inv_c;
+      fr0 = (float)centered_frac(pc.f0 * tau);
+      cyc1 = pc.dfs * tau;
+    }
+    if (sigma_out && valid && ray_ok) sigma_out[idx] = sigma;
+    s_amp[wave][lane] = amp;
+    s_fr0[wave][lane] = fr0;
+    s_cyc1[wave][lane] = cyc1;
+    wave_sync();
+    if (chunk_ok) {
+      const int cnt = min(kTile, nv - base);
+      for (int i = grp; i < cnt; i += groups) {
+        const float am = s_amp[wave][i];
+        if (am == 0.0f) continue;
+        const double c1 = s_cyc1[wave][i];
+        const float2 seed = phasor_cycles(
+            (float)centered_frac((double)s_fr0[wave][i] + (double)k0 * c1), 1.0f);
+        const float2 w = phasor_cycles((float)centered_frac(c1), 1.0f);
+        float2 zc = make_float2(am * seed.x, am * seed.y);
+#pragma unroll
+        for (int j = 0; j < kSigChunk; ++j) {
+          acc_r[j] += zc.x;
+          acc_i[j] += zc.y;
+          zc = cmul(zc, w);
+        }
+      }
+    }
+    wave_sync();
+  }
+  // sum the sample groups (lanes of one chunk differ in the low bits)
+  for (int off = 1; off < groups; off <<= 1) {
+#pragma unroll
+    for (int j = 0; j < kSigChunk; ++j) {
+      acc_r[j] += __shfl_xor(acc_r[j], off, kWave);
+      acc_i[j] += __shfl_xor(acc_i[j], off, kWave);
+    }
+  }
+  if (ray_ok && chunk_ok && grp == 0) {
+    const float inv_n = nv > 0 ? 1.0f / (float)nv : 0.0f;
+#pragma unroll
+    for (int j = 0; j < kSigChunk; ++j) {
+      const int k = k0 + j;
+      if (k < a.K) {
+        out_real[(size_t)rr * a.K + k] = acc_r[j] * inv_n;
+        out_imag[(size_t)rr * a.K + k] = acc_i[j] * inv_n;
+      }
+    }
+  }
+  if (sigma_out && ray_ok)
+    for (int s = nv + lane; s < a.S; s += kWave) sigma_out[(size_t)rr * a.S + s] = 0.0f;
+}
+
+// ------------------------------------------------------------------ backward
+
+__global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
+    LenslessArgs a, const float* __restrict__ g_real, const float* __restrict__ g_imag,
+    float* __restrict__ grad_sdf, float* __restrict__ grad_g, float* __restrict__ grad_refl,
+    float* __restrict__ grad_inv_s, float* __restrict__ grad_power, PhaseConsts pc) {
+  __shared__ float s_carry[kRaysPerBlock][kMaxTiles];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = blockIdx.x * kRaysPerBlock + wave;
+  const bool ray_ok = r < a.R;
+  const int rr = ray_ok ? r : a.R - 1;
+  const int nv = a.n_valid ? min(max(a.n_valid[rr], 0), a.S) : a.S;
+  const V3 o = load3(a.origin + 3 * (size_t)rr);
+  const V3 dir = load3(a.dirs + 3 * (size_t)rr);
+  const float inv_s = a.inv_s[0];
+  const float power = a.power[0];
+  const float inv_n = nv > 0 ? 1.0f / (float)nv : 0.0f;
+  const int ntiles = (nv + kTile - 1) / kTile;
+
+  // pass 1: transmittance at the start of every tile
+  float carry = 1.0f;
+  for (int tile = 0; tile < ntiles; ++tile) {
+    if (lane == 0) s_carry[wave][tile] = carry;
+    const int s = tile * kTile + lane;
+    const bool valid = s < nv;
+    SampleState x;
+    load_and_composite(a, (size_t)rr * a.S + (valid ? s : nv - 1), dir, inv_s, x);
+    const float incl = wave_incl_prod(valid ? x.t : 1.0f, lane);
+    carry *= __shfl(incl, kWave - 1, kWave);
+  }
+  wave_sync();
+
+  // pass 2: reverse over tiles
+  float w_carry = 0.0f;  // W at the first sample of the next tile
+  float gis = 0.0f, gpow = 0.0f;
+  const float* gr_row = g_real + (size_t)rr * a.K;
+  const float* gi_row = g_imag + (size_t)rr * a.K;
+  for (int tile = ntiles - 1; tile >= 0; --tile) {
+    const int s = tile * kTile + lane;
+    const bool valid = s < nv;
+    const size_t idx = (size_t)rr * a.S + (valid ? s : nv - 1);
+    SampleState x;
+    load_and_composite(a, idx, dir, inv_s, x);
+    const float tt = valid ? x.t : 1.0f;
+    const float incl = wave_incl_prod(tt, lane);
+    float excl = __shfl_up(incl, 1, kWave);
+    if (lane == 0) excl = 1.0f;
+    const float T = s_carry[wave][tile] * excl;
+    const float Tc = T + x.off;
+    const float rp = x.refl * power;
+    const float m = rp * Tc;
+    const bool unclamped = m >= 1.0e-6f;
+    const float aTc = x.alpha * Tc;
+    const float sigma = fmaxf(m, 1.0e-6f) * aTc;
+    const Emission e = emission(o, dir, x, valid);
+
+    float h = 0.0f, att = 0.0f;
+    if (e.live) {
+      att = (float)(1.0 / (e.dpath * e.dpath));
+      const PathPhasors ph = path_phasors((e.dpath + pc.bias) * pc.inv_c, pc, -1.0f);
+      const float2 S = horner2(gr_row, gi_row, a.K, ph.w, ph.W);
+      h = cmul(ph.base, S).x * inv_n;
+    }
+    const float gsig = h * e.geo.spec * att;
+    const float gspec = h * sigma * att;
+    const V3 gnhat = e.live ? gspec * dspec_dnormal(e.geo, e.n) : v3(0.0f, 0.0f, 0.0f);
+    float grefl, gTc, galpha;
+    if (unclamped) {
+      grefl = gsig * power * Tc * aTc;
+      gpow += gsig * x.refl * Tc * aTc;
+      gTc = gsig * 2.0f * m * x.alpha;
+      galpha = gsig * m * Tc;
+    } else {
+      grefl = 0.0f;
+      gTc = gsig * 1.0e-6f * x.alpha;
+      galpha = gsig * 1.0e-6f * Tc;
+    }
+    if (!valid) gTc = 0.0f;
+    // reverse affine scan: W_l = gT_l + t_l * W_{l+1}
+    float mm = tt, bb = gTc;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const float m2 = __shfl_down(mm, off, kWave);
+      const float b2 = __shfl_down(bb, off, kWave);
+      if (lane + off < kWave) {
+        bb = fmaf(mm, b2, bb);
+        mm *= m2;
+      }
+    }
+    const float W = fmaf(mm, w_carry, bb);
+    float w_next = __shfl_down(W, 1, kWave);
+    if (lane == kWave - 1) w_next = w_carry;
+    w_carry = __shfl(W, 0, kWave);
+
+    galpha -= T * w_next;
+    const float gq = (valid && x.qmask) ? galpha : 0.0f;
+    const float den = x.pc + 1.0e-5f;
+    const float gpc = gq * x.nc / (den * den);
+    const float gnc = -gq / den;
+    const float sp = x.pc * (1.0f - x.pc);
+    const float sn = x.nc * (1.0f - x.nc);
+    const float gprev = gpc * inv_s * sp;
+    const float gnext = gnc * inv_s * sn;
+    gis += gpc * x.prev * sp + gnc * x.next * sn;
+    const float gsdf = gprev + gnext;
+    const float gic = (gnext - gprev) * x.dist * 0.5f;
+    const float gc = gic * ((x.c < 1.0f ? 0.5f * (1.0f - a.anneal) : 0.0f) +
+                            (x.c < 0.0f ? a.anneal : 0.0f));
+    V3 gg;
+    if (e.gl > 1.0e-12f)
+      gg = (1.0f / e.gl) * (gnhat - dot(e.n, gnhat) * e.n);
+    else
+      gg = 1.0e12f * gnhat;
+    gg = gg + gc * dir;
+    if (valid && ray_ok) {
+      grad_sdf[idx] = gsdf;
+      grad_g[3 * idx + 0] = gg.x;
+      grad_g[3 * idx + 1] = gg.y;
+      grad_g[3 * idx + 2] = gg.z;
+      if (grad_refl) grad_refl[idx] = grefl;
+    }
+  }
+  if (ray_ok) {
+    for (int s = nv + lane; s < a.S; s += kWave) {
+      const size_t idx = (size_t)rr * a.S + s;
+      grad_sdf[idx] = 0.0f;
+      grad_g[3 * idx] = grad_g[3 * idx + 1] = grad_g[3 * idx + 2] = 0.0f;
+      if (grad_refl) grad_refl[idx] = 0.0f;
+    }
+  }
+  gis = wave_sum(gis);
+  gpow = wave_sum(gpow);
+  if (ray_ok && lane == 0) {
+    grad_inv_s[rr] = gis;
+    grad_power[rr] = gpow;
+  }
+}
+
+LenslessArgs make_args(const float* sdf, const float* grad, const float* refl,
+                       const float* dists, const float* z, const float* origin,
+                       const float* dirs, const float* offset, const int32_t* n_valid,
+                       const float* inv_s, const float* power, int64_t R, int64_t S, int64_t K,
+                       float anneal) {
+  if (R < 0 || S < 0 || K < 0) throw ArgError("sizes must be non-negative");
+  if (R > INT32_MAX / 2 || R * S > (INT64_C(1) << 36)) throw ArgError("too many samples");
+  if (S > kTile * kMaxTiles) throw ArgError("at most 4096 samples per ray");
+  if (!(anneal >= 0.0f && anneal <= 1.0f)) throw ArgError("cos_anneal_ratio must be in [0,1]");
+  require_ptrs({origin, dirs, inv_s, power}, R > 0);
+  require_ptrs({sdf, grad, dists, z}, R > 0 && S > 0);
+  LenslessArgs a;
+  a.sdf = sdf;
+  a.grad = grad;
+  a.refl = refl;
+  a.dists = dists;
+  a.z = z;
+  a.origin = origin;
+  a.dirs = dirs;
+  a.offset = offset;
+  a.n_valid = n_valid;
+  a.inv_s = inv_s;
+  a.power = power;
+  a.R = (int)R;
+  a.S = (int)S;
+  a.K = (int)K;
+  a.anneal = anneal;
+  return a;
+}
+
+}  // namespace
+}  // namespace geraf
+
+using namespace geraf;
+
+extern "C" int geraf_lensless_forward_f32(
+    const float* sdf, const float* grad, const float* refl, const float* dists, const float* z,
+    const float* origin, const float* dirs, const float* trans_offset, const int32_t* n_valid,
+    const float* inv_s, const float* power, float* out_real, float* out_imag, float* sigma_out,
+    int64_t R, int64_t S, int64_t K, float cos_anneal_ratio, const geraf_fmcw_params* params,
+    geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_params(params);
+  const LenslessArgs a = make_args(sdf, grad, refl, dists, z, origin, dirs, trans_offset,
+                                   n_valid, inv_s, power, R, S, K, cos_anneal_ratio);
+  if (K > 2048) throw ArgError("at most 2048 frequencies per ray");
+  if (R == 0 || K == 0) return GERAF_RF_OK;
+  require_ptrs({out_real, out_imag}, true);
+  if (S == 0) throw ArgError("need at least one sample per ray");
+  const int chunks = (int)((K + kSigChunk - 1) / kSigChunk);
+  int groups = 1;
+  while (groups * 2 * chunks <= kWave) groups *= 2;
+  const unsigned blocks = (unsigned)((R + kRaysPerBlock - 1) / kRaysPerBlock);
+  hipLaunchKernelGGL(lensless_forward_kernel, dim3(blocks), dim3(64 * kRaysPerBlock), 0,
+                     (hipStream_t)stream, a, out_real, out_imag, sigma_out,
+                     make_phase_consts(*params), groups, chunks);
+  return check_launch("lensless_forward_kernel");
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_lensless_backward_f32(
+    const float* grad_out_real, const float* grad_out_imag, const float* sdf, const float* grad,
+    const float* refl, const float* dists, const float* z, const float* origin,
+    const float* dirs, const float* trans_offset, const int32_t* n_valid, const float* inv_s,
+    const float* power, float* grad_sdf, float* grad_grad, float* grad_refl,
+    float* grad_inv_s_per_ray, float* grad_power_per_ray, int64_t R, int64_t S, int64_t K,
+    float cos_anneal_ratio, const geraf_fmcw_params* params, geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_params(params);
+  const LenslessArgs a = make_args(sdf, grad, refl, dists, z, origin, dirs, trans_offset,
+                                   n_valid, inv_s, power, R, S, K, cos_anneal_ratio);
+  if (R == 0) return GERAF_RF_OK;
+  require_ptrs({grad_inv_s_per_ray, grad_power_per_ray}, true);
+  require_ptrs({grad_sdf, grad_grad}, S > 0);
+  require_ptrs({grad_out_real, grad_out_imag}, K > 0);
+  if (S == 0) throw ArgError("need at least one sample per ray");
+  const unsigned blocks = (unsigned)((R + kRaysPerBlock - 1) / kRaysPerBlock);
+  hipLaunchKernelGGL(lensless_backward_kernel, dim3(blocks), dim3(64 * kRaysPerBlock), 0,
+                     (hipStream_t)stream, a, grad_out_real, grad_out_imag, grad_sdf, grad_grad,
+                     grad_refl, grad_inv_s_per_ray, grad_power_per_ray,
+                     make_phase_consts(*params));
+  return check_launch("lensless_backward_kernel");
+  GERAF_TRY_END
+}

@@ -217,12 +217,6 @@ __global__ __launch_bounds__(256) void mf_forward_f64_kernel(
 constexpr int kAdjChunk = 32;
 constexpr int kAdjWaves = 4;
 
-__device__ __forceinline__ float wave_sum(float x) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
-  return x;
-}
-
 __global__ __launch_bounds__(64 * kAdjWaves) void mf_backward_kernel(
     const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,

@@ -77,3 +77,132 @@ struct KahanC {
 };
 
 }  // namespace geraf
+
+namespace geraf {
+
+// ------------------------------------------------------------------ 3-vectors
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 load3(const float* p) { return v3(p[0], p[1], p[2]); }
+
+// v / (|v| + 1e-7), the normalisation of SignalTracer_cuda_kernel.cu:40-47.
+__device__ __forceinline__ V3 normalize_eps(V3 v) {
+  const float inv = 1.0f / (sqrtf(dot(v, v)) + 1.0e-7f);
+  return inv * v;
+}
+
+// Specular geometry of one (tx, rx, point, normal) tuple, as
+// ComputeSpecularGeometry (SignalTracer_cuda_kernel.cu:172-197): the normal is
+// used as given (not re-normalised).
+struct SpecGeom {
+  V3 inc;            // normalised incident direction (point - tx)
+  V3 ret;            // normalised return direction (rx - point)
+  V3 refl;           // normalised reflected direction
+  float refl_norm;   // |inc - 2 (inc.n) n|
+  float idn;         // inc . n
+  float spec;        // ret . refl
+  __device__ __forceinline__ bool culled() const { return spec < 1.0e-6f || idn > 0.0f; }
+};
+
+__device__ __forceinline__ SpecGeom spec_geom(V3 tx, V3 rx, V3 n, V3 p) {
+  SpecGeom g;
+  g.inc = normalize_eps(p - tx);
+  g.idn = dot(g.inc, n);
+  const V3 rr = g.inc - (2.0f * g.idn) * n;
+  g.refl_norm = sqrtf(dot(rr, rr));
+  g.refl = (1.0f / (g.refl_norm + 1.0e-7f)) * rr;
+  g.ret = normalize_eps(rx - p);
+  g.spec = dot(g.ret, g.refl);
+  return g;
+}
+
+// d spec / d n with the return and incident directions held fixed: the product
+// of the three Jacobians of SignalTracer_cuda_kernel.cu:91-170,
+//   J = -2 n inc^T - 2 (inc.n) I,  NJ = (I - refl refl^T) J / (|refl_raw| + eps),
+//   grad = NJ^T ret,
+// folded to grad = -2 [ (q.n) inc + (inc.n) q ] / (|refl_raw| + eps) with
+// q = ret - spec * refl.
+__device__ __forceinline__ V3 dspec_dnormal(const SpecGeom& g, V3 n) {
+  const V3 q = g.ret - g.spec * g.refl;
+  const float s = -2.0f / (g.refl_norm + 1.0e-7f);
+  return s * (dot(q, n) * g.inc + g.idn * q);
+}
+
+__device__ __forceinline__ double dist3_f64(V3 a, V3 b) {
+  return dist3_f64((double)a.x, (double)a.y, (double)a.z, (double)b.x, (double)b.y, (double)b.z);
+}
+
+// Sum over the 64 lanes of a wave (butterfly, every lane gets the total).
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+
+// Two-level Horner evaluation of sum_k s_k w^k for wave-uniform coefficient
+// rows (read through the scalar cache): 32-term inner Horner in w, outer
+// Horner in W = w^32.  Returns the complex sum.
+__device__ __forceinline__ float2 horner2(const float* __restrict__ er,
+                                          const float* __restrict__ ei, int K, float2 w,
+                                          float2 W) {
+  constexpr int B = 32;
+  const int nb = K / B;
+  const int rem = K - nb * B;
+  float Sr = 0.0f, Si = 0.0f;
+  if (rem) {
+    const int k0 = nb * B;
+    float ir = er[K - 1], ii = ei[K - 1];
+    for (int k = K - 2; k >= k0; --k) {
+      const float u1 = fmaf(ir, w.x, er[k]);
+      const float u2 = fmaf(ii, w.x, ei[k]);
+      const float nr = fmaf(-ii, w.y, u1);
+      ii = fmaf(ir, w.y, u2);
+      ir = nr;
+    }
+    Sr = ir;
+    Si = ii;
+  }
+  for (int b = nb - 1; b >= 0; --b) {
+    const float* __restrict__ xr = er + b * B;
+    const float* __restrict__ xi = ei + b * B;
+    float ir = xr[B - 1], ii = xi[B - 1];
+#pragma unroll
+    for (int j = B - 2; j >= 0; --j) {
+      const float u1 = fmaf(ir, w.x, xr[j]);
+      const float u2 = fmaf(ii, w.x, xi[j]);
+      const float nr = fmaf(-ii, w.y, u1);
+      ii = fmaf(ir, w.y, u2);
+      ir = nr;
+    }
+    const float u1 = fmaf(Sr, W.x, ir);
+    const float u2 = fmaf(Si, W.x, ii);
+    const float nr = fmaf(-Si, W.y, u1);
+    Si = fmaf(Sr, W.y, u2);
+    Sr = nr;
+  }
+  return make_float2(Sr, Si);
+}
+
+// Phasors of one path for the k-sum: carrier exp(sign*j*2*pi*f0*tau), step
+// w = exp(sign*j*2*pi*dfs*tau) and W = w^32, from FP64-reduced cycles.
+struct PathPhasors {
+  float2 base, w, W;
+};
+__device__ __forceinline__ PathPhasors path_phasors(double tau, const PhaseConsts& pc,
+                                                    float sign) {
+  const double cyc1 = pc.dfs * tau;
+  PathPhasors r;
+  r.base = phasor_cycles((float)centered_frac(pc.f0 * tau), sign);
+  r.w = phasor_cycles((float)centered_frac(cyc1), sign);
+  r.W = phasor_cycles((float)centered_frac(32.0 * cyc1), sign);
+  return r;
+}
+
+}  // namespace geraf

@@ -1,15 +1,9 @@
-"""Matched-filter ops; lazy exports as in the reference (RFMatchedFilter/__init__.py:6-18)."""
-from importlib import import_module
+"""Matched-filter ops (same exports as the reference's RFMatchedFilter/__init__.py:6-18).
 
-_EXPORTS = {
-    "matched_filter_cuda": ".matched_filter_cuda",
-    "faster_matched_filter_cuda": ".faster_matched_filter_cuda",
-}
+Imported eagerly: the ``_C`` shim loads libgeraf_rf.so only on first call, and
+eager binding keeps ``matched_filter_cuda`` the function even after its
+submodule of the same name has been imported."""
+from .faster_matched_filter_cuda import faster_matched_filter_cuda
+from .matched_filter_cuda import matched_filter_cuda
 
-__all__ = list(_EXPORTS)
-
-
-def __getattr__(name):
-    if name not in _EXPORTS:
-        raise AttributeError(name)
-    return getattr(import_module(_EXPORTS[name], __name__), name)
+__all__ = ["matched_filter_cuda", "faster_matched_filter_cuda"]

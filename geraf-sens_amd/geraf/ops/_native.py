@@ -53,6 +53,16 @@ _SIGNATURES = {
     "geraf_mf_forward_f64": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_backward_f32": [_P] * 7 + [_I64] * 4 + [_PARAMS, _P],
     "geraf_mf_backward_f64": [_P] * 7 + [_I64] * 4 + [_PARAMS, _P],
+    "geraf_st_forward_f32": [_P] * 8 + [_I64] * 3 + [_PARAMS, _P],
+    "geraf_st_forward_f64": [_P] * 8 + [_I64] * 3 + [_PARAMS, _P],
+    "geraf_st_backward_f32": [_P] * 10 + [_I64] * 3 + [_PARAMS, _P],
+    "geraf_st_backward_f64": [_P] * 10 + [_I64] * 3 + [_PARAMS, _P],
+    "geraf_bilinear_forward_f32": [_P] * 7 + [_I64] * 4 + [_P],
+    "geraf_bilinear_forward_f64": [_P] * 7 + [_I64] * 4 + [_P],
+    "geraf_bilinear_backward_f32": [_P] * 6 + [_I64] * 4 + [_P],
+    "geraf_bilinear_backward_f64": [_P] * 6 + [_I64] * 4 + [_P],
+    "geraf_lensless_forward_f32": [_P] * 14 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
+    "geraf_lensless_backward_f32": [_P] * 18 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
 }
 
 _lock = threading.Lock()

@@ -101,6 +101,109 @@ int geraf_mf_backward_f64(const double* grad_out_real, const double* grad_out_im
                           int64_t N, int64_t norm_count, const geraf_fmcw_params* params,
                           geraf_stream_t stream);
 
+/* ---------------------------------------------------------------- signal tracer */
+
+/* [A,P] signal tracer forward:
+ *   out[a,k] = (1/P) sum_{p: inbounds, spec >= 1e-6, inc.n <= 0}
+ *              sigma[a,p] * spec_ap / d_ap^2 * exp(+j*phi_{a,k}(p))
+ * with the specular geometry of SignalTracer_cuda_kernel.cu:172-197 (normals
+ * used as given), d = |tx-p| + |rx-p| (no bias in the amplitude).
+ * normals/points [P,3]; sigmas [A,P]; tx/rx [A,3]; inbounds [P] (bool, 1 byte);
+ * out_real/out_imag [A,K]. params->f0 is the tracer's fc_start.
+ * Replaces _C.SignalTracer_forward (RFSignalTracer/csrc/vision.cpp;
+ * SignalTracer_cuda_kernel.cu:351-450). */
+int geraf_st_forward_f32(const float* normals, const float* sigmas, const float* points,
+                         const float* tx_pos, const float* rx_pos, const uint8_t* inbounds,
+                         float* out_real, float* out_imag, int64_t A, int64_t P, int64_t K,
+                         const geraf_fmcw_params* params, geraf_stream_t stream);
+int geraf_st_forward_f64(const double* normals, const double* sigmas, const double* points,
+                         const double* tx_pos, const double* rx_pos, const uint8_t* inbounds,
+                         double* out_real, double* out_imag, int64_t A, int64_t P, int64_t K,
+                         const geraf_fmcw_params* params, geraf_stream_t stream);
+
+/* [A,P] signal tracer backward: gradients into normals [P,3] and sigmas [A,P]
+ * from grad_out_real/imag [A,K]. Every output element is written; points with
+ * inbounds == false get zeros (the reference leaves them uninitialised).
+ * Replaces _C.SignalTracer_backward_normal_sigma
+ * (SignalTracer_cuda_kernel.cu:452-587). */
+int geraf_st_backward_f32(const float* grad_out_real, const float* grad_out_imag,
+                          const float* normals, const float* sigmas, const float* points,
+                          const float* tx_pos, const float* rx_pos, const uint8_t* inbounds,
+                          float* grad_normals, float* grad_sigmas, int64_t A, int64_t P,
+                          int64_t K, const geraf_fmcw_params* params, geraf_stream_t stream);
+int geraf_st_backward_f64(const double* grad_out_real, const double* grad_out_imag,
+                          const double* normals, const double* sigmas, const double* points,
+                          const double* tx_pos, const double* rx_pos, const uint8_t* inbounds,
+                          double* grad_normals, double* grad_sigmas, int64_t A, int64_t P,
+                          int64_t K, const geraf_fmcw_params* params, geraf_stream_t stream);
+
+/* Linear resampling of per-ray normals [R,S,3] and sigmas [A,R,S] from source
+ * depths z_vals [R,S] to target depths tgt_z_vals [R,T] (binary search for the
+ * bracket; targets outside [z_lo, z_hi] give 0 and inbounds = false).
+ * Requires S >= 2. Replaces _C.BilinearRaysample_forward
+ * (SignalTracer_cuda_kernel.cu:199-279). */
+int geraf_bilinear_forward_f32(const float* normals, const float* sigmas, const float* z_vals,
+                               const float* tgt_z_vals, float* out_normals, float* out_sigmas,
+                               uint8_t* out_inbounds, int64_t A, int64_t R, int64_t S,
+                               int64_t T, geraf_stream_t stream);
+int geraf_bilinear_forward_f64(const double* normals, const double* sigmas,
+                               const double* z_vals, const double* tgt_z_vals,
+                               double* out_normals, double* out_sigmas, uint8_t* out_inbounds,
+                               int64_t A, int64_t R, int64_t S, int64_t T,
+                               geraf_stream_t stream);
+
+/* Transpose of the resampler, written without atomics (deterministic); every
+ * element of grad_normals [R,S,3] and grad_sigmas [A,R,S] is written.
+ * Replaces _C.BilinearRaysample_backward (SignalTracer_cuda_kernel.cu:281-349). */
+int geraf_bilinear_backward_f32(const float* grad_out_normals, const float* grad_out_sigmas,
+                                const float* z_vals, const float* tgt_z_vals,
+                                float* grad_normals, float* grad_sigmas, int64_t A, int64_t R,
+                                int64_t S, int64_t T, geraf_stream_t stream);
+int geraf_bilinear_backward_f64(const double* grad_out_normals, const double* grad_out_sigmas,
+                                const double* z_vals, const double* tgt_z_vals,
+                                double* grad_normals, double* grad_sigmas, int64_t A, int64_t R,
+                                int64_t S, int64_t T, geraf_stream_t stream);
+
+/* ---------------------------------------------------------------- fused lensless renderer */
+
+/* Per-ray lensless renderer: NeuS-style compositing of SDF samples and the
+ * monostatic FMCW signal of each ray, in one kernel. For ray r (antenna at
+ * origin[r], unit direction dirs[r]) and sample s < n_valid[r]:
+ *   alpha, T     as rf_rendering.py:46-52,145-169 (raw gradients in iter_cos)
+ *   Tc = T + trans_offset; sigma = max(refl*power*Tc, 1e-6)*alpha*Tc
+ *                  (rf_rendering.py:749-759, sdf_network.py:475-481)
+ *   signal       SignalTracer_cuda_kernel.cu:351-450 with tx = rx = origin,
+ *                normal = grad/max(|grad|,1e-12), point = origin + z*dir
+ *   out[r,k] = (1/n_valid[r]) sum_s amp_s exp(+j*phi_{s,k})
+ * sdf, dists, z, refl, trans_offset: [R,S]; grad: [R,S,3]; origin, dirs: [R,3];
+ * n_valid: [R] int32; inv_s, power: device scalars [1]; out_real/out_imag
+ * [R,K]; sigma_out [R,S] (optional). refl, trans_offset, n_valid, sigma_out
+ * may be NULL (1, 0, S, not written). Limits: S <= 4096, K <= 2048.
+ * No counterpart in the reference: it fuses the torch glue of
+ * rf_rendering.py:740-780 with the tracer op for the per-ray case. */
+int geraf_lensless_forward_f32(const float* sdf, const float* grad, const float* refl,
+                               const float* dists, const float* z, const float* origin,
+                               const float* dirs, const float* trans_offset,
+                               const int32_t* n_valid, const float* inv_s, const float* power,
+                               float* out_real, float* out_imag, float* sigma_out, int64_t R,
+                               int64_t S, int64_t K, float cos_anneal_ratio,
+                               const geraf_fmcw_params* params, geraf_stream_t stream);
+
+/* Adjoint of geraf_lensless_forward_f32: gradients into sdf, grad (the raw
+ * SDF gradients, through iter_cos and the normalised normal), refl, and per-ray
+ * partial sums of the inv_s and power gradients ([R] each; the caller sums
+ * them). Recomputes the compositing instead of reading stored intermediates.
+ * grad_refl may be NULL. Samples past n_valid get zeros. */
+int geraf_lensless_backward_f32(const float* grad_out_real, const float* grad_out_imag,
+                                const float* sdf, const float* grad, const float* refl,
+                                const float* dists, const float* z, const float* origin,
+                                const float* dirs, const float* trans_offset,
+                                const int32_t* n_valid, const float* inv_s, const float* power,
+                                float* grad_sdf, float* grad_grad, float* grad_refl,
+                                float* grad_inv_s_per_ray, float* grad_power_per_ray, int64_t R,
+                                int64_t S, int64_t K, float cos_anneal_ratio,
+                                const geraf_fmcw_params* params, geraf_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,89 @@
+"""FP64 restatement of the lensless compositing glue and the per-ray signal.
+
+TEST INFRASTRUCTURE ONLY (the checker of the fused lensless kernels).
+
+The composition follows the reference's torch glue line by line, in float64:
+  iter_cos            rf_rendering.py:145-155 (_compute_iter_cos)
+  alpha               rf_rendering.py:157-169 (_compute_alpha_from_sdf), EPS = 1e-5
+  transmittance       rf_rendering.py:46-52   (_compute_transmission), TRANS_EPS = 1e-7
+  calibration         rf_rendering.py:750     (T - prev_cdf + new_cdf = T + offset)
+  sigma               sdf_network.py:475-481  (clamp(refl * exp(light_power) * T, 1e-6) * alpha * T)
+  normals             F.normalize(gradients)  (rf_rendering.py:638,1054)
+  signal              SignalTracer_cuda_kernel.cu:172-197,351-450 with tx = rx = ray origin
+Gradients of this function by torch autograd are the reference composition's
+gradients; tests pin them with central finite differences.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+import torch.nn.functional as F
+
+EPS = 1e-5
+TRANS_EPS = 1e-7
+
+
+def _normalize_eps(v: torch.Tensor) -> torch.Tensor:
+    # SignalTracer_cuda_kernel.cu:40-47: v / (|v| + 1e-7)
+    return v / (torch.linalg.norm(v, dim=-1, keepdim=True) + 1e-7)
+
+
+def points_f32(o32: torch.Tensor, z32: torch.Tensor, d32: torch.Tensor) -> torch.Tensor:
+    """Sample positions o + z*d formed in float32 (separately rounded product
+    and sum), as a float32 tensor op produces them."""
+    return (o32[:, None, :].float() + z32[:, :, None].float() * d32[:, None, :].float()).double()
+
+
+def lensless_forward(sdf, grad, dists, pts, o, d, inv_s, power, K, fs, As_sci, f0, c, bias,
+                     refl=None, offset=None, n_valid=None, anneal=1.0, return_parts=False):
+    """All tensors float64 on the CPU; pts [R,S,3] from points_f32.  Returns the
+    complex signal [R,K] (complex128)."""
+    R, S = sdf.shape
+    true_cos = (d[:, None, :] * grad).sum(-1)
+    iter_cos = -(F.relu(-true_cos * 0.5 + 0.5) * (1.0 - anneal) + F.relu(-true_cos) * anneal)
+    next_sdf = sdf + iter_cos * dists * 0.5
+    prev_sdf = sdf - iter_cos * dists * 0.5
+    prev_cdf = torch.sigmoid(prev_sdf * inv_s)
+    next_cdf = torch.sigmoid(next_sdf * inv_s)
+    alpha = ((prev_cdf - next_cdf + EPS) / (prev_cdf + EPS)).clip(0.0, 1.0)
+    if n_valid is not None:
+        valid = torch.arange(S)[None, :] < n_valid[:, None]
+    else:
+        valid = torch.ones((R, S), dtype=torch.bool)
+    ones = torch.ones((R, 1), dtype=sdf.dtype)
+    trans = torch.cumprod(torch.cat([ones, 1.0 - alpha + TRANS_EPS], dim=-1), dim=-1)[..., :-1]
+    tc = trans + (offset if offset is not None else 0.0)
+    rp = (refl if refl is not None else torch.ones_like(sdf)) * power
+    sigma = torch.clamp(rp * tc, min=1e-6) * (alpha * tc)
+    n = F.normalize(grad, dim=-1)
+    # monostatic specular geometry (the reference tracer with tx = rx = origin)
+    oo = o[:, None, :].expand_as(pts)
+    inc = _normalize_eps(pts - oo)
+    idn = (inc * n).sum(-1)
+    refl_raw = inc - 2.0 * idn[..., None] * n
+    refl_dir = _normalize_eps(refl_raw)
+    ret = _normalize_eps(oo - pts)
+    spec = (ret * refl_dir).sum(-1)
+    live = valid & (spec >= 1e-6) & (idn <= 0.0)
+    dpath = 2.0 * torch.linalg.norm(pts - oo, dim=-1)
+    amp = torch.where(live, sigma * spec / dpath ** 2, torch.zeros_like(sigma))
+    tau = (dpath + bias) / c
+    k = torch.arange(K, dtype=torch.float64)
+    phase = 2.0 * math.pi * (f0 + As_sci * (k / fs))[None, None, :] * tau[..., None]
+    sig = (amp[..., None] * torch.polar(torch.ones_like(phase), phase)).sum(1)
+    nv = valid.sum(-1).clamp(min=1).to(torch.float64)
+    out = sig / nv[:, None]
+    if return_parts:
+        return out, dict(alpha=alpha, trans=trans, sigma=torch.where(valid, sigma, 0 * sigma),
+                         spec=spec, live=live, true_cos=true_cos, amp=amp)
+    return out
+
+
+def masks(sdf, grad, dists, pts, o, d, inv_s, power, refl=None, offset=None, n_valid=None,
+          anneal=1.0):
+    """The piecewise branches a finite difference must not cross."""
+    _, parts = lensless_forward(sdf, grad, dists, pts, o, d, inv_s, power, 1, 1.0, 0.0, 1.0, 1.0,
+                                0.0, refl=refl, offset=offset, n_valid=n_valid, anneal=anneal,
+                                return_parts=True)
+    return parts

@@ -1,0 +1,213 @@
+"""Matched-filter parity on the GPU: the gfx950 kernels (through the C ABI and
+the drop-in ops) against the CPU oracle and against each other."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import rf as orc
+from rf_workloads import mf_scenes
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances (volume rel-L2 = ||x - x64|| / ||x64||).
+TOL_TUNED_VS_F64 = 1.0e-5     # tuned FP32+FP64-reduction kernel vs FP64 oracle
+TOL_PLAIN32_VS_REF32 = 2.0e-4  # plain FP32 GPU form vs FP32 reference formula on CPU
+TOL_F64 = 1.0e-10              # FP64 GPU form vs FP64 oracle
+
+
+def rel_l2(x, ref):
+    return float(np.linalg.norm(np.asarray(x) - ref) / np.linalg.norm(ref))
+
+
+def run_mf(scene_or_arrays, dev, form="f32"):
+    from geraf.ops import _native
+
+    if isinstance(scene_or_arrays, mf_scenes.MFScene):
+        sc = scene_or_arrays
+        echo, tx, rx, vox, prm = sc.echo, sc.tx, sc.rx, sc.voxels, sc.params()
+    else:
+        echo, tx, rx, vox, prm = scene_or_arrays
+    dt = torch.float64 if form == "f64" else torch.float32
+    re = torch.as_tensor(np.ascontiguousarray(echo.real), device=dev, dtype=dt)
+    im = torch.as_tensor(np.ascontiguousarray(echo.imag), device=dev, dtype=dt)
+    t = torch.as_tensor(tx, device=dev, dtype=dt).contiguous()
+    r = torch.as_tensor(rx, device=dev, dtype=dt).contiguous()
+    v = torch.as_tensor(np.ascontiguousarray(np.asarray(vox).T), device=dev, dtype=dt)
+    A, K = re.shape
+    N = v.shape[1]
+    o_r = torch.empty(N, device=dev, dtype=dt)
+    o_i = torch.empty(N, device=dev, dtype=dt)
+    name = {"f32": "geraf_mf_forward_f32", "plain32": "geraf_mf_forward_plain_f32",
+            "f64": "geraf_mf_forward_f64"}[form]
+    params = _native.make_params(prm["f0"], prm["slope"], prm["fs"], prm["c"], prm["bias"])
+    _native.call(name, *map(_native.ptr, (re, im, t, r, o_r, o_i, v)), A, K, N, params,
+                 _native.stream_of(re))
+    torch.cuda.synchronize()
+    return o_r.double().cpu().numpy() + 1j * o_i.double().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    sc = mf_scenes.config1(0)
+    ref = orc.mf_forward(sc.echo.real, sc.echo.imag, sc.tx, sc.rx, sc.voxels, **sc.params())
+    return sc, ref
+
+
+def test_cfg1_tuned_vs_fp64_oracle(cuda, cfg1):
+    sc, ref = cfg1
+    out = run_mf(sc, cuda)
+    err = rel_l2(out, ref)
+    print(f"cfg1 tuned rel-L2 vs FP64 oracle: {err:.3e}")
+    assert err <= TOL_TUNED_VS_F64
+    idx = mf_scenes.nearest_voxels(sc, sc.scatterers)
+    peak = np.max(np.abs(np.abs(out[idx]) / np.abs(ref[idx]) - 1.0))
+    phase = np.max(np.abs(np.angle(out[idx] / ref[idx])))
+    print(f"cfg1 peak-magnitude err {peak:.3e}, phase err {phase:.3e} rad")
+    assert peak < 1e-4 and phase < 1e-4
+
+
+def test_cfg1_plain_forms(cuda, cfg1):
+    sc, ref = cfg1
+    ref32 = orc.mf_forward(sc.echo.real, sc.echo.imag, sc.tx, sc.rx, sc.voxels, form="ref32",
+                           **sc.params())
+    out32 = run_mf(sc, cuda, "plain32")
+    assert rel_l2(out32, ref32) <= TOL_PLAIN32_VS_REF32
+    out64 = run_mf(sc, cuda, "f64")
+    assert rel_l2(out64, ref) <= TOL_F64
+
+
+@pytest.mark.parametrize("A,K,N", [(1, 1, 1), (3, 37, 100), (7, 64, 65), (2, 32, 4097),
+                                   (33, 96, 130), (5, 200, 1)])
+def test_shapes_vs_oracle(cuda, A, K, N):
+    rng = np.random.default_rng(A * 1000 + K * 10 + N)
+    echo = (rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K))).astype(np.complex64)
+    tx = rng.uniform(-0.05, 0.05, (A, 3)).astype(np.float32)
+    tx[:, 2] = 0.0
+    rx = (tx + np.float32([0.002, 0, 0])).astype(np.float32)
+    vox = rng.uniform(-0.1, 0.1, (N, 3)).astype(np.float32)
+    vox[:, 2] += 0.3
+    prm = dict(f0=mf_scenes.MF_F0, slope=float(np.float32(701500.0)), fs=1.25e6,
+               c=mf_scenes.SPEED_OF_LIGHT, bias=mf_scenes.PATH_BIAS)
+    ref = orc.mf_forward(echo.real, echo.imag, tx, rx, vox, **prm)
+    out = run_mf((echo, tx, rx, vox, prm), cuda)
+    assert rel_l2(out, ref) <= 3e-6 * np.sqrt(K) + 1e-6
+
+
+def test_empty_volume(cuda):
+    from geraf.ops.RFMatchedFilter import matched_filter_cuda
+
+    re = torch.randn(4, 8, device=cuda)
+    im = torch.randn(4, 8, device=cuda)
+    pos = torch.zeros(4, 3, device=cuda)
+    vox = torch.zeros(0, 3, device=cuda)
+    o_r, o_i = matched_filter_cuda(re, im, pos, pos, vox, 4, 8, 1.25e6, 701500.0, 0)
+    assert o_r.shape == (0,) and o_i.shape == (0,)
+
+
+def _adjoint_case(dev, A=6, K=64, N=300, seed=3):
+    rng = np.random.default_rng(seed)
+    tx = rng.uniform(-0.05, 0.05, (A, 3)).astype(np.float32)
+    rx = (tx + np.float32([0.001, 0.0005, 0])).astype(np.float32)
+    vox = rng.uniform(-0.1, 0.1, (N, 3)).astype(np.float32)
+    vox[:, 2] += 0.3
+    g = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    return tx, rx, vox, g
+
+
+def test_backward_vs_oracle_and_adjoint_identity(cuda):
+    from geraf.ops.RFMatchedFilter import matched_filter_cuda
+
+    A, K, N = 6, 64, 300
+    tx, rx, vox, g = _adjoint_case(cuda, A, K, N)
+    fs, As = 1.25e6, 701500.0
+    prm = dict(f0=mf_scenes.MF_F0, slope=As, fs=fs, c=mf_scenes.SPEED_OF_LIGHT,
+               bias=mf_scenes.PATH_BIAS)
+    rng = np.random.default_rng(9)
+    s = (rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K))).astype(np.complex64)
+    re = torch.tensor(s.real, device=cuda, requires_grad=True)
+    im = torch.tensor(s.imag, device=cuda, requires_grad=True)
+    t = torch.tensor(tx, device=cuda)
+    r = torch.tensor(rx, device=cuda)
+    v = torch.tensor(vox, device=cuda)
+    o_r, o_i = matched_filter_cuda(re, im, t, r, v, A, K, fs, As, N)
+    (o_r * torch.tensor(g.real, device=cuda) + o_i * torch.tensor(g.imag, device=cuda)).sum() \
+        .backward()
+    grad = re.grad.double().cpu().numpy() + 1j * im.grad.double().cpu().numpy()
+    # d/ds of Re<out, g> is the adjoint applied to g (real-linear pairing).
+    ref = orc.mf_backward(g, tx, rx, vox, K, norm=A, **prm)
+    assert rel_l2(grad, ref) <= 1e-5
+    # <MF s, g>_R == <s, MF^T g>_R in the oracle itself, and for the GPU pair.
+    fwd = orc.mf_forward(s.real, s.imag, tx, rx, vox, **prm)
+    lhs = np.sum(fwd.real * g.real + fwd.imag * g.imag)
+    rhs = np.sum(s.real * ref.real + s.imag * ref.imag)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    out = o_r.detach().double().cpu().numpy() + 1j * o_i.detach().double().cpu().numpy()
+    lhs_gpu = np.sum(out.real * g.real + out.imag * g.imag)
+    rhs_gpu = np.sum(s.real * grad.real + s.imag * grad.imag)
+    assert abs(lhs_gpu - rhs_gpu) <= 1e-5 * abs(lhs_gpu)
+
+
+def test_faster_matched_filter_equivalence(cuda):
+    from geraf.ops.RFMatchedFilter import faster_matched_filter_cuda, matched_filter_cuda
+
+    A, K, N, sel = 8, 64, 257, 3
+    tx, rx, vox, g = _adjoint_case(cuda, A, K, N, seed=5)
+    rng = np.random.default_rng(11)
+    s = (rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K))).astype(np.complex64)
+    fs, As = 1.25e6, 701500.0
+    re = torch.tensor(s.real, device=cuda, requires_grad=True)
+    im = torch.tensor(s.imag, device=cuda, requires_grad=True)
+    t, r, v = (torch.tensor(x, device=cuda) for x in (tx, rx, vox))
+    o_r, o_i = matched_filter_cuda(re, im, t, r, v, A, K, fs, As, N)
+    gr = torch.tensor(g.real, device=cuda)
+    gi = torch.tensor(g.imag, device=cuda)
+    (o_r * gr + o_i * gi).sum().backward()
+    re2 = torch.tensor(s.real[:sel], device=cuda, requires_grad=True)
+    im2 = torch.tensor(s.imag[:sel], device=cuda, requires_grad=True)
+    f_r, f_i = faster_matched_filter_cuda(re2, im2, torch.tensor(s.real[sel:], device=cuda),
+                                          torch.tensor(s.imag[sel:], device=cuda), t[:sel],
+                                          r[:sel], t[sel:], r[sel:], v, A, sel, K, fs, As, N)
+    (f_r * gr + f_i * gi).sum().backward()
+    torch.testing.assert_close(f_r, o_r, rtol=0, atol=0)
+    torch.testing.assert_close(f_i, o_i, rtol=0, atol=0)
+    torch.testing.assert_close(re2.grad, re.grad[:sel], rtol=1e-5, atol=1e-7)
+    torch.testing.assert_close(im2.grad, im.grad[:sel], rtol=1e-5, atol=1e-7)
+
+
+def test_float64_and_half_dispatch(cuda):
+    from geraf.ops.RFMatchedFilter import matched_filter_cuda
+
+    A, K, N = 4, 64, 100
+    tx, rx, vox, _ = _adjoint_case(cuda, A, K, N, seed=7)
+    rng = np.random.default_rng(1)
+    s = (rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K)))
+    fs, As = 1.25e6, 701500.0
+    prm = dict(f0=mf_scenes.MF_F0, slope=As, fs=fs, c=mf_scenes.SPEED_OF_LIGHT,
+               bias=mf_scenes.PATH_BIAS)
+    ref = orc.mf_forward(s.real, s.imag, tx, rx, vox, **prm)
+    args = [torch.tensor(x, device=cuda, dtype=torch.float64) for x in (s.real, s.imag, tx, rx, vox)]
+    o_r, o_i = matched_filter_cuda(*args, A, K, fs, As, N)
+    assert o_r.dtype == torch.float64
+    assert rel_l2(o_r.cpu().numpy() + 1j * o_i.cpu().numpy(), ref) <= 1e-10
+    h = [a.half() for a in args]
+    o_r, o_i = matched_filter_cuda(*h, A, K, fs, As, N)
+    assert o_r.dtype == torch.float16
+    ref_h = orc.mf_forward(*(a.double().cpu().numpy() for a in h[:2]), *(a.float().cpu().numpy()
+                           for a in h[2:]), **prm)
+    assert rel_l2(o_r.double().cpu().numpy() + 1j * o_i.double().cpu().numpy(), ref_h) < 2e-3
+
+
+def test_cfg2_subvolume_vs_fp64(cuda):
+    sc = mf_scenes.config2(0, device=str(cuda))
+    rng = np.random.default_rng(0)
+    sel = np.unique(np.concatenate([mf_scenes.nearest_voxels(sc, sc.scatterers[::25]),
+                                    rng.integers(0, sc.N, 200)]))
+    arrays = (sc.echo, sc.tx, sc.rx, sc.voxels[sel], sc.params())
+    out = run_mf(arrays, cuda)
+    out64 = run_mf(arrays, cuda, "f64")
+    err = rel_l2(out, out64)
+    print(f"cfg2 subset ({len(sel)} voxels) tuned rel-L2 vs GPU FP64: {err:.3e}")
+    assert err <= TOL_TUNED_VS_F64
+    few = sel[:8]
+    ref = orc.mf_forward(sc.echo.real, sc.echo.imag, sc.tx, sc.rx, sc.voxels[few], **sc.params())
+    assert rel_l2(out64[:8], ref) <= 1e-9

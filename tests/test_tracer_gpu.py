@@ -1,0 +1,194 @@
+"""Signal-tracer parity on the GPU: [A,P] op, bilinear resampler and the fused
+lensless tracer against the CPU oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lensless as L
+from oracle import rf as orc
+from rf_workloads import tracer_scenes as ts
+
+pytestmark = pytest.mark.gpu
+
+TOL_FWD = 1e-5    # FP32 kernels vs FP64 oracle, rel-L2
+TOL_GRAD = 1e-4   # FP32 adjoints vs FP64 oracle gradients, rel-L2
+TOL_F64 = 1e-10
+
+FC, AS, FS = ts.FC_START, ts.AS_SCI, ts.f32(1.25e6)
+PRM = dict(f0=FC, slope=AS, fs=FS, c=ts.SPEED_OF_LIGHT, bias=ts.PATH_BIAS)
+
+
+def rel(x, ref):
+    x, ref = np.asarray(x), np.asarray(ref)
+    return float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+
+
+def ap_case(A=6, P=1500, seed=0, frac_out=0.2):
+    rng = np.random.default_rng(seed)
+    tx = rng.uniform(-0.05, 0.05, (A, 3)).astype(np.float32)
+    tx[:, 2] = 0
+    rx = (tx + np.float32([0.002, 0.0, 0.0])).astype(np.float32)
+    pts = rng.uniform(-0.1, 0.1, (P, 3)).astype(np.float32)
+    pts[:, 2] += 0.3
+    # normals pointing back toward the array, jittered: many live pairs
+    n = -(pts - np.array([0, 0, 0])) / np.linalg.norm(pts, axis=1, keepdims=True)
+    n = (n + 0.3 * rng.standard_normal((P, 3))).astype(np.float32)
+    sig = rng.uniform(0.1, 1.0, (A, P)).astype(np.float32)
+    inb = rng.uniform(0, 1, P) > frac_out
+    return tx, rx, pts, n, sig, inb
+
+
+def test_signal_tracer_forward_backward_vs_oracle(cuda):
+    from geraf.ops.RFSignalTracer import signal_tracer_cuda
+
+    A, P, K = 6, 1500, 64
+    tx, rx, pts, n, sig, inb = ap_case(A, P)
+    T = lambda x: torch.tensor(x, device=cuda)  # noqa: E731
+    nt = T(n).requires_grad_(True)
+    st = T(sig).requires_grad_(True)
+    o_r, o_i = signal_tracer_cuda(nt, st, T(pts), T(tx), T(rx), T(inb), K, FS, FC, AS, A, P)
+    out = o_r.detach().double().cpu().numpy() + 1j * o_i.detach().double().cpu().numpy()
+    ref = orc.st_forward(n, sig, pts, tx, rx, inb, K, **PRM)
+    err = rel(out, ref)
+    print(f"[A,P] tracer fwd rel-L2 {err:.2e}")
+    assert err <= TOL_FWD
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K))
+    (o_r * T(g.real.astype(np.float32)) + o_i * T(g.imag.astype(np.float32))).sum().backward()
+    gn_ref, gs_ref = orc.st_backward(g.astype(np.complex64), n, sig, pts, tx, rx, inb, **PRM)
+    assert rel(nt.grad.cpu().numpy(), gn_ref) <= TOL_FWD * 10
+    assert rel(st.grad.cpu().numpy(), gs_ref) <= TOL_FWD * 10
+    # out-of-bounds points get exact zeros
+    assert torch.all(nt.grad[~T(inb)] == 0) and torch.all(st.grad[:, ~T(inb)] == 0)
+
+
+def test_signal_tracer_f64(cuda):
+    from geraf.ops.RFSignalTracer import signal_tracer_cuda
+
+    A, P, K = 3, 400, 40
+    tx, rx, pts, n, sig, inb = ap_case(A, P, seed=4)
+    T = lambda x: torch.tensor(x, device=cuda, dtype=torch.float64)  # noqa: E731
+    nt = T(n).requires_grad_(True)
+    st = T(sig).requires_grad_(True)
+    o_r, o_i = signal_tracer_cuda(nt, st, T(pts), T(tx), T(rx),
+                                  torch.tensor(inb, device=cuda), K, FS, FC, AS, A, P)
+    ref = orc.st_forward(n, sig, pts, tx, rx, inb, K, **PRM)
+    assert rel(o_r.detach().cpu().numpy() + 1j * o_i.detach().cpu().numpy(), ref) <= TOL_F64
+    g = np.ones((A, K)) + 0.5j * np.ones((A, K))
+    (o_r * T(g.real) + o_i * T(g.imag)).sum().backward()
+    gn_ref, gs_ref = orc.st_backward(g, n, sig, pts, tx, rx, inb, **PRM)
+    assert rel(nt.grad.cpu().numpy(), gn_ref) <= 1e-9
+    assert rel(st.grad.cpu().numpy(), gs_ref) <= 1e-9
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_bilinear_vs_oracle(cuda, dtype):
+    from geraf.ops.RFSignalTracer import bilinear_raysample_cuda
+
+    rng = np.random.default_rng(2)
+    A, R, S, Tn = 2, 37, 32, 64
+    z = np.sort(rng.uniform(0.2, 0.5, (R, S)), axis=1)
+    tz = np.sort(rng.uniform(0.15, 0.55, (R, Tn)), axis=1)
+    tz[0, :S] = z[0]  # exact hits on source depths
+    nrm = rng.standard_normal((R, S, 3))
+    sig = rng.uniform(0, 1, (A, R, S))
+    T = lambda x: torch.tensor(x, device=cuda, dtype=dtype)  # noqa: E731
+    nt = T(nrm).requires_grad_(True)
+    st = T(sig).requires_grad_(True)
+    on, os_, inb = bilinear_raysample_cuda(nt, st, T(z), T(tz), A, R, S, Tn)
+    # the oracle sees the same (possibly float32-rounded) inputs
+    cv = lambda t: t.detach().double().cpu().numpy()  # noqa: E731
+    rn, rs, rinb = orc.bilinear_forward(cv(nt), cv(st), cv(T(z)), cv(T(tz)))
+    tol = 1e-6 if dtype == torch.float32 else 1e-13
+    assert np.array_equal(inb.cpu().numpy(), rinb)
+    assert np.max(np.abs(cv(on) - rn)) <= tol
+    assert np.max(np.abs(cv(os_) - rs)) <= tol
+    gn = rng.standard_normal((R, Tn, 3))
+    gs = rng.standard_normal((A, R, Tn))
+    (on * T(gn)).sum().add((os_ * T(gs)).sum()).backward()
+    rgn, rgs = orc.bilinear_backward(gn, gs, cv(T(z)), cv(T(tz)), S)
+    assert np.max(np.abs(cv(nt.grad) - rgn)) <= tol * 10
+    assert np.max(np.abs(cv(st.grad) - rgs)) <= tol * 10
+    # deterministic: same result twice, bit for bit
+    nt.grad = None
+    st.grad = None
+    on2, os2, _ = bilinear_raysample_cuda(nt, st, T(z), T(tz), A, R, S, Tn)
+    (on2 * T(gn)).sum().add((os2 * T(gs)).sum()).backward()
+    g1 = nt.grad.clone()
+    nt.grad = None
+    on3, os3, _ = bilinear_raysample_cuda(nt, st, T(z), T(tz), A, R, S, Tn)
+    (on3 * T(gn)).sum().add((os3 * T(gs)).sum()).backward()
+    assert torch.equal(g1, nt.grad)
+
+
+def _lensless_run(b, dev, with_grads=True, seed=0):
+    from geraf.ops.RFSignalTracer.lensless_tracer import lensless_signal_tracer
+
+    bb = b.to(dev)
+    leaves = {k: getattr(bb, k).clone().requires_grad_(True) for k in
+              ("sdf", "grad", "refl", "inv_s", "power") if getattr(bb, k) is not None}
+    o_r, o_i = lensless_signal_tracer(leaves["sdf"], leaves["grad"], bb.dists, bb.z, bb.o, bb.d,
+                                      leaves["inv_s"], leaves["power"],
+                                      reflectance=leaves.get("refl"), trans_offset=bb.offset,
+                                      n_valid=bb.n_valid, cos_anneal_ratio=b.anneal, **b.radar)
+    out = o_r.detach().double().cpu().numpy() + 1j * o_i.detach().double().cpu().numpy()
+    grads = None
+    if with_grads:
+        rng = np.random.default_rng(seed)
+        g = rng.standard_normal(out.shape) + 1j * rng.standard_normal(out.shape)
+        gt_r = torch.tensor(g.real, dtype=torch.float32, device=dev)
+        gt_i = torch.tensor(g.imag, dtype=torch.float32, device=dev)
+        (o_r * gt_r + o_i * gt_i).sum().backward()
+        grads = {k: v.grad.double().cpu() for k, v in leaves.items()}
+        grads["_g"] = g
+    return out, grads
+
+
+def _lensless_oracle(b, g=None):
+    f64 = lambda t: None if t is None else t.double()  # noqa: E731
+    leaves = {k: getattr(b, k).double().clone().requires_grad_(True) for k in
+              ("sdf", "grad", "refl", "inv_s", "power") if getattr(b, k) is not None}
+    pts = L.points_f32(b.o, b.z, b.d)
+    r = b.radar
+    out = L.lensless_forward(leaves["sdf"], leaves["grad"], f64(b.dists), pts, f64(b.o),
+                             f64(b.d), leaves["inv_s"], leaves["power"], r["numADCSample"],
+                             r["adcSampleRate"], r["As_sci"], r["fc_start"], ts.SPEED_OF_LIGHT,
+                             ts.PATH_BIAS, refl=leaves.get("refl"), offset=f64(b.offset),
+                             n_valid=None if b.n_valid is None else b.n_valid.long(),
+                             anneal=b.anneal)
+    grads = None
+    if g is not None:
+        gt = torch.tensor(g)
+        (out.real * gt.real + out.imag * gt.imag).sum().backward()
+        grads = {k: v.grad for k, v in leaves.items()}
+    return out.detach().numpy(), grads
+
+
+@pytest.mark.parametrize("case", ["plain", "offset_uneven", "anneal_half_K96"])
+def test_lensless_vs_oracle(cuda, case):
+    if case == "plain":
+        b = ts.rays(256, 128, 256, seed=3)
+    elif case == "offset_uneven":
+        b = ts.rays(200, 96, 64, seed=5, with_offset=True, uneven=(16, 96))
+    else:
+        b = ts.rays(130, 70, 96, seed=7, anneal=0.5)
+    out, grads = _lensless_run(b, cuda)
+    ref, rgrads = _lensless_oracle(b, grads["_g"])
+    err = rel(out, ref)
+    print(f"lensless {case}: fwd rel-L2 {err:.2e}")
+    assert err <= TOL_FWD
+    for k, v in rgrads.items():
+        e = rel(grads[k].numpy().ravel(), v.numpy().ravel())
+        print(f"  grad {k}: rel-L2 {e:.2e}")
+        assert e <= TOL_GRAD, k
+
+
+def test_lensless_config3_subset(cuda):
+    b = ts.config3(0)
+    out, _ = _lensless_run(b, cuda, with_grads=False)
+    sub = ts.RayBatch(*(None if x is None else (x[:384] if torch.is_tensor(x) and x.dim() > 0
+                                                   else x) for x in
+                        (b.sdf, b.grad, b.dists, b.z, b.o, b.d, b.refl, b.offset, b.n_valid,
+                         b.inv_s, b.power)), b.radar, b.anneal)
+    ref, _ = _lensless_oracle(sub)
+    assert rel(out[:384], ref) <= TOL_FWD
