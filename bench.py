@@ -37,6 +37,125 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool):
+    """Fused lensless tracer, forward + backward, on the 4,096-ray x 128-sample,
+    K=256 workload.  Returns a dict for the JSON line."""
+    import numpy as np
+    import torch
+
+    from geraf.ops import _native
+    from geraf.ops.RFSignalTracer import _C as stC
+    from rf_workloads import tracer_scenes as ts
+
+    b0 = ts.config3(0)
+    b = b0.to(dev)
+    R, S, K = b.R, b.S, b.radar["numADCSample"]
+    prm = stC.tracer_params(b.radar["fc_start"], b.radar["As_sci"], b.radar["adcSampleRate"])
+    o_r = torch.empty((R, K), device=dev)
+    o_i = torch.empty_like(o_r)
+    rng = np.random.default_rng(0)
+    g_r = torch.tensor(rng.standard_normal((R, K)), dtype=torch.float32, device=dev)
+    g_i = torch.tensor(rng.standard_normal((R, K)), dtype=torch.float32, device=dev)
+    gs = torch.empty((R, S), device=dev)
+    gg = torch.empty((R, S, 3), device=dev)
+    grf = torch.empty((R, S), device=dev)
+    gis = torch.empty(R, device=dev)
+    gpw = torch.empty(R, device=dev)
+    inv_s = b.inv_s.reshape(1).contiguous()
+    power = b.power.reshape(1).contiguous()
+    P = _native.ptr
+    stream = _native.stream_of(o_r)
+    ins = [P(b.sdf), P(b.grad), P(b.refl), P(b.dists), P(b.z), P(b.o), P(b.d), P(None), P(None),
+           P(inv_s), P(power)]
+
+    def fwd():
+        _native.call("geraf_lensless_forward_f32", *ins, P(o_r), P(o_i), P(None), R, S, K,
+                     b.anneal, prm, stream)
+
+    def bwd():
+        _native.call("geraf_lensless_backward_f32", P(g_r), P(g_i), *ins, P(gs), P(gg), P(grf),
+                     P(gis), P(gpw), R, S, K, b.anneal, prm, stream)
+
+    for _ in range(warmup):
+        fwd()
+        bwd()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for e in ev:
+        e[0].record()
+        fwd()
+        e[1].record()
+        bwd()
+        e[2].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    f_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    b_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    samples = R * S
+    rate = samples * steps / wall
+    # algorithmic traffic and work (see DESIGN.md)
+    bytes_fwd = samples * 28 + R * (24 + 8 * K) + 8
+    bytes_bwd = samples * (28 + 20) + R * (24 + 8 * K + 8) + 8
+    flops = 16.0 * K * samples
+    gbs = (bytes_fwd + bytes_bwd) / ((f_ms + b_ms) / 1e3) / 1e9
+    tfs = flops / ((f_ms + b_ms) / 1e3) / 1e12
+    log(f"tracer: fwd {f_ms:.3f} ms, bwd {b_ms:.3f} ms, {rate:.3e} samples/s, "
+        f"{gbs:.0f} GB/s, {tfs:.1f} TFLOP/s")
+
+    # accuracy against the FP64 oracle on a ray subset
+    errs = {}
+    try:
+        from oracle import lensless as L
+        n = 256
+        sub = {k: getattr(b0, k)[:n].double().clone().requires_grad_(True)
+               for k in ("sdf", "grad", "refl")}
+        inv = b0.inv_s.double().clone().requires_grad_(True)
+        pw = b0.power.double().clone().requires_grad_(True)
+        r = b0.radar
+        ref = L.lensless_forward(sub["sdf"], sub["grad"], b0.dists[:n].double(),
+                                 L.points_f32(b0.o[:n], b0.z[:n], b0.d[:n]),
+                                 b0.o[:n].double(), b0.d[:n].double(), inv, pw, K,
+                                 r["adcSampleRate"], r["As_sci"], r["fc_start"],
+                                 ts.SPEED_OF_LIGHT, ts.PATH_BIAS, refl=sub["refl"],
+                                 anneal=b0.anneal)
+        gt = torch.complex(g_r[:n].double().cpu(), g_i[:n].double().cpu())
+        (ref.real * gt.real + ref.imag * gt.imag).sum().backward()
+        fwd()
+        bwd()
+        torch.cuda.synchronize()
+        out = torch.complex(o_r[:n].double(), o_i[:n].double()).cpu()
+        errs["fwd_rel_l2"] = float((out - ref.detach()).norm() / ref.detach().norm())
+        for name, mine, theirs in (("sdf", gs[:n], sub["sdf"].grad),
+                                   ("gradients", gg[:n], sub["grad"].grad),
+                                   ("reflectance", grf[:n], sub["refl"].grad)):
+            m = mine.double().cpu()
+            errs[f"grad_{name}_rel_l2"] = float((m - theirs).norm() / theirs.norm())
+        log(f"tracer errors vs FP64 oracle (256 rays): {errs}")
+    except Exception as e:  # a checker failure is reported, not fatal to the timing
+        log(f"tracer oracle check skipped: {e}")
+
+    cpu = None
+    if not no_cpu:
+        from oracle import cpu_baseline
+        crate, threads, desc = cpu_baseline.time_tracer_sample(b0, cpu_seconds,
+                                                               threads=len(os.sched_getaffinity(0)))
+        cpu = {"value": crate, "unit": "samples/s", "cores": threads, "kind": "port",
+               "sample": "torch-CPU FP32 restatement of the reference composition: " + desc}
+        log(f"tracer cpu baseline {crate:.3e} samples/s")
+    return {
+        "metric": "tracer fwd+bwd samples/s",
+        "value": rate, "unit": "samples/s",
+        "workload": "fused lensless tracer, 4096 rays x 128 samples, K=256",
+        "fwd_ms": f_ms, "bwd_ms": b_ms,
+        "roofline": {"hbm_gbs": gbs, "hbm_frac": gbs / 8000.0, "valu_tflops": tfs,
+                     "valu_frac": tfs / VALU_FP32_PEAK_TFLOPS,
+                     "bytes_per_step": bytes_fwd + bytes_bwd, "flops_per_step": flops},
+        "errors_vs_fp64": errs,
+        "cpu_baseline": cpu,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +163,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tracer", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -158,6 +278,11 @@ def main() -> int:
                          f"(per-term cos/sin) on {desc}"}
         log(f"cpu baseline {rate:.3e} terms/s on {threads} threads")
 
+    tracer = None
+    if not args.no_tracer:
+        tracer = bench_tracer(dev, max(args.steps, 5), max(args.warmup, 1), args.cpu_seconds / 2,
+                              args.no_cpu)
+
     line = {
         "metric": "MF voxel*antenna*freq terms/s",
         "value": value,
@@ -186,6 +311,7 @@ def main() -> int:
         "plain_forms_terms_per_s": {"fp32_per_term_sincos": p32_rate, "fp64": f64_rate,
                                     "tuned_speedup_vs_fp64": value / f64_rate,
                                     "tuned_speedup_vs_fp32": value / p32_rate},
+        "tracer": tracer,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -33,111 +33,157 @@ namespace {
 
 constexpr int kHornerBlock = 32;
 
+// Voxels per lane and waves per block of the large-volume launch.
+#ifndef GERAF_MF_VPL
+#define GERAF_MF_VPL 2
+#endif
+#ifndef GERAF_MF_WAVES
+#define GERAF_MF_WAVES 4
+#endif
+
 // ---------------------------------------------------------------- tuned forward
 
-template <int WAVES>
+// VPL voxels per lane (tiles of 64 voxels, VPL apart): the VPL Horner chains
+// share every echo value read into SGPRs and give the scheduler independent
+// FMAs to interleave.
+template <int WAVES, int VPL>
 __global__ __launch_bounds__(64 * WAVES) void mf_forward_horner_kernel(
     const float* __restrict__ real, const float* __restrict__ imag,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
     float* __restrict__ out_real, float* __restrict__ out_imag,
     const float* __restrict__ voxel, int A, int K, int N, int groups, PhaseConsts pc) {
-  __shared__ double partial[WAVES][kWave][2];
+  __shared__ double partial[WAVES][VPL][kWave][2];
 
   const int lane = threadIdx.x & (kWave - 1);
   // The wave index is uniform across the wave; saying so lets the antenna
   // loop bounds, positions and echo rows live in SGPRs (scalar loads).
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int group = wave % groups;  // antenna group of this wave
-  const int tile = wave / groups;   // voxel tile (64 voxels) of this wave
+  const int tile = wave / groups;   // voxel tile group of this wave
   const int tiles_per_block = WAVES / groups;
 
-  const int v = (blockIdx.x * tiles_per_block + tile) * kWave + lane;
-  const bool valid = v < N;
-  const int vc = valid ? v : (N - 1);
-
-  const double px = voxel[vc];
-  const double py = voxel[(size_t)N + vc];
-  const double pz = voxel[2 * (size_t)N + vc];
+  int v[VPL];
+  bool valid[VPL];
+  double px[VPL], py[VPL], pz[VPL];
+#pragma unroll
+  for (int i = 0; i < VPL; ++i) {
+    v[i] = ((blockIdx.x * tiles_per_block + tile) * VPL + i) * kWave + lane;
+    valid[i] = v[i] < N;
+    const int vc = valid[i] ? v[i] : (N - 1);
+    px[i] = voxel[vc];
+    py[i] = voxel[(size_t)N + vc];
+    pz[i] = voxel[2 * (size_t)N + vc];
+  }
 
   const int ants_per_group = (A + groups - 1) / groups;
   const int a_begin = group * ants_per_group;
   const int a_end = min(A, a_begin + ants_per_group);
 
-  const int nb = K / kHornerBlock;     // full 32-sample blocks
+  const int nb = K / kHornerBlock;        // full 32-sample blocks
   const int rem = K - nb * kHornerBlock;  // samples in the partial top block
 
-  KahanC acc;
-  acc.init();
+  KahanC acc[VPL];
+#pragma unroll
+  for (int i = 0; i < VPL; ++i) acc[i].init();
 
   for (int a = a_begin; a < a_end; ++a) {
-    const double d = dist3_f64(tx_pos[3 * a + 0], tx_pos[3 * a + 1], tx_pos[3 * a + 2], px, py,
-                               pz) +
-                     dist3_f64(rx_pos[3 * a + 0], rx_pos[3 * a + 1], rx_pos[3 * a + 2], px, py,
-                               pz);
-    const double tau = (d + pc.bias) * pc.inv_c;
-    const double cyc1 = pc.dfs * tau;
-    const float2 base = phasor_cycles((float)centered_frac(pc.f0 * tau), -1.0f);
-    const float2 w = phasor_cycles((float)centered_frac(cyc1), -1.0f);
-    const float2 W = phasor_cycles((float)centered_frac(kHornerBlock * cyc1), -1.0f);
+    const double tx = tx_pos[3 * a + 0], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+    const double rx = rx_pos[3 * a + 0], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+    PathPhasors ph[VPL];
+#pragma unroll
+    for (int i = 0; i < VPL; ++i) {
+      const double d = dist3_f64(tx, ty, tz, px[i], py[i], pz[i]) +
+                       dist3_f64(rx, ry, rz, px[i], py[i], pz[i]);
+      ph[i] = path_phasors((d + pc.bias) * pc.inv_c, pc, -1.0f);
+    }
 
     const float* __restrict__ er = real + (size_t)a * K;
     const float* __restrict__ ei = imag + (size_t)a * K;
 
-    float Sr = 0.0f, Si = 0.0f;
+    float Sr[VPL], Si[VPL];
+#pragma unroll
+    for (int i = 0; i < VPL; ++i) Sr[i] = Si[i] = 0.0f;
     if (rem) {  // partial top block, k = K-1 .. 32*nb
       const int k0 = nb * kHornerBlock;
-      float ir = er[K - 1], ii = ei[K - 1];
-      for (int k = K - 2; k >= k0; --k) {
-        const float u1 = fmaf(ir, w.x, er[k]);
-        const float u2 = fmaf(ii, w.x, ei[k]);
-        const float nr = fmaf(-ii, w.y, u1);
-        ii = fmaf(ir, w.y, u2);
-        ir = nr;
+#pragma unroll
+      for (int i = 0; i < VPL; ++i) {
+        Sr[i] = er[K - 1];
+        Si[i] = ei[K - 1];
       }
-      Sr = ir;
-      Si = ii;
+      for (int k = K - 2; k >= k0; --k) {
+        const float xr = er[k], xi = ei[k];
+#pragma unroll
+        for (int i = 0; i < VPL; ++i) {
+          const float u1 = fmaf(Sr[i], ph[i].w.x, xr);
+          const float u2 = fmaf(Si[i], ph[i].w.x, xi);
+          const float nr = fmaf(-Si[i], ph[i].w.y, u1);
+          Si[i] = fmaf(Sr[i], ph[i].w.y, u2);
+          Sr[i] = nr;
+        }
+      }
     }
     for (int b = nb - 1; b >= 0; --b) {
       const float* __restrict__ xr = er + b * kHornerBlock;
       const float* __restrict__ xi = ei + b * kHornerBlock;
-      float ir = xr[kHornerBlock - 1], ii = xi[kHornerBlock - 1];
+      float ir[VPL], ii[VPL];
+#pragma unroll
+      for (int i = 0; i < VPL; ++i) {
+        ir[i] = xr[kHornerBlock - 1];
+        ii[i] = xi[kHornerBlock - 1];
+      }
 #pragma unroll
       for (int j = kHornerBlock - 2; j >= 0; --j) {
-        const float u1 = fmaf(ir, w.x, xr[j]);
-        const float u2 = fmaf(ii, w.x, xi[j]);
-        const float nr = fmaf(-ii, w.y, u1);
-        ii = fmaf(ir, w.y, u2);
-        ir = nr;
+#pragma unroll
+        for (int i = 0; i < VPL; ++i) {
+          const float u1 = fmaf(ir[i], ph[i].w.x, xr[j]);
+          const float u2 = fmaf(ii[i], ph[i].w.x, xi[j]);
+          const float nr = fmaf(-ii[i], ph[i].w.y, u1);
+          ii[i] = fmaf(ir[i], ph[i].w.y, u2);
+          ir[i] = nr;
+        }
       }
       // S = S*W + inner
-      const float u1 = fmaf(Sr, W.x, ir);
-      const float u2 = fmaf(Si, W.x, ii);
-      const float nr = fmaf(-Si, W.y, u1);
-      Si = fmaf(Sr, W.y, u2);
-      Sr = nr;
+#pragma unroll
+      for (int i = 0; i < VPL; ++i) {
+        const float u1 = fmaf(Sr[i], ph[i].W.x, ir[i]);
+        const float u2 = fmaf(Si[i], ph[i].W.x, ii[i]);
+        const float nr = fmaf(-Si[i], ph[i].W.y, u1);
+        Si[i] = fmaf(Sr[i], ph[i].W.y, u2);
+        Sr[i] = nr;
+      }
     }
-    acc.add(cmul(base, make_float2(Sr, Si)));
+#pragma unroll
+    for (int i = 0; i < VPL; ++i) acc[i].add(cmul(ph[i].base, make_float2(Sr[i], Si[i])));
   }
 
   const double inv_a = 1.0 / (double)A;
   if (groups == 1) {
-    if (valid) {
-      out_real[v] = (float)(acc.re() * inv_a);
-      out_imag[v] = (float)(acc.im() * inv_a);
-    }
+#pragma unroll
+    for (int i = 0; i < VPL; ++i)
+      if (valid[i]) {
+        out_real[v[i]] = (float)(acc[i].re() * inv_a);
+        out_imag[v[i]] = (float)(acc[i].im() * inv_a);
+      }
     return;
   }
-  partial[wave][lane][0] = acc.re();
-  partial[wave][lane][1] = acc.im();
+#pragma unroll
+  for (int i = 0; i < VPL; ++i) {
+    partial[wave][i][lane][0] = acc[i].re();
+    partial[wave][i][lane][1] = acc[i].im();
+  }
   __syncthreads();
-  if (group == 0 && valid) {
-    double sr = 0.0, si = 0.0;
-    for (int g = 0; g < groups; ++g) {
-      sr += partial[wave + g][lane][0];
-      si += partial[wave + g][lane][1];
+  if (group == 0) {
+#pragma unroll
+    for (int i = 0; i < VPL; ++i) {
+      if (!valid[i]) continue;
+      double sr = 0.0, si = 0.0;
+      for (int g = 0; g < groups; ++g) {
+        sr += partial[wave + g][i][lane][0];
+        si += partial[wave + g][i][lane][1];
+      }
+      out_real[v[i]] = (float)(sr * inv_a);
+      out_imag[v[i]] = (float)(si * inv_a);
     }
-    out_real[v] = (float)(sr * inv_a);
-    out_imag[v] = (float)(si * inv_a);
   }
 }
 
@@ -334,17 +380,19 @@ extern "C" int geraf_mf_forward_f32(const float* real, const float* imag, const 
   const int n = (int)N;
   const int tiles = (n + kWave - 1) / kWave;
   if (tiles < 4 * 1024 && A > 1) {
+    // small volume: split the antennas over the waves of a block
     constexpr int W = 16;
     const int g = pick_groups((int)A, n, W);
     const int tiles_per_block = W / g;
     const int blocks = (tiles + tiles_per_block - 1) / tiles_per_block;
-    hipLaunchKernelGGL(mf_forward_horner_kernel<W>, dim3(blocks), dim3(64 * W), 0,
+    hipLaunchKernelGGL((mf_forward_horner_kernel<W, 1>), dim3(blocks), dim3(64 * W), 0,
                        (hipStream_t)stream, real, imag, tx_pos, rx_pos, out_real, out_imag,
                        voxel, (int)A, (int)K, n, g, pc);
   } else {
-    constexpr int W = 4;
-    const int blocks = (tiles + W - 1) / W;
-    hipLaunchKernelGGL(mf_forward_horner_kernel<W>, dim3(blocks), dim3(64 * W), 0,
+    constexpr int W = GERAF_MF_WAVES;
+    constexpr int V = GERAF_MF_VPL;
+    const int blocks = (tiles + W * V - 1) / (W * V);
+    hipLaunchKernelGGL((mf_forward_horner_kernel<W, V>), dim3(blocks), dim3(64 * W), 0,
                        (hipStream_t)stream, real, imag, tx_pos, rx_pos, out_real, out_imag,
                        voxel, (int)A, (int)K, n, 1, pc);
   }

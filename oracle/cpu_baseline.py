@@ -62,3 +62,44 @@ def time_mf_sample(scene, target_seconds: float = 15.0, threads: int = 0, seed: 
     desc = (f"{n} random voxels of {scene.name} x {scene.A} antennas x {scene.K} samples "
             f"({terms:.3e} terms, {dt:.1f} s)")
     return terms / dt, nthreads, desc
+
+
+def time_tracer_sample(batch, target_seconds: float = 10.0, threads: int = 0):
+    """Time the reference composition (torch-CPU FP32 glue + per-ray signal with
+    a per-term cos/sin, forward and autograd backward) on a ray subset sized for
+    about `target_seconds`; returns (samples_per_s, threads, description)."""
+    from oracle import lensless as L
+
+    if threads:
+        torch.set_num_threads(threads)
+    nthreads = torch.get_num_threads()
+    r = batch.radar
+
+    def run(n):
+        leaves = {k: getattr(batch, k)[:n].clone().float().requires_grad_(True)
+                  for k in ("sdf", "grad", "refl")}
+        inv_s = batch.inv_s.clone().float().requires_grad_(True)
+        power = batch.power.clone().float().requires_grad_(True)
+        pts = (batch.o[:n, None, :] + batch.z[:n, :, None] * batch.d[:n, None, :]).float()
+        out = L.lensless_forward(leaves["sdf"], leaves["grad"], batch.dists[:n], pts,
+                                 batch.o[:n], batch.d[:n], inv_s, power, r["numADCSample"],
+                                 r["adcSampleRate"], r["As_sci"], r["fc_start"], 2.9979, 0.15,
+                                 refl=leaves["refl"], anneal=batch.anneal)
+        (out.real.sum() + out.imag.sum()).backward()
+
+    n0 = 16
+    while True:  # grow the pilot until it takes a measurable time
+        t0 = time.perf_counter()
+        run(n0)
+        dt = time.perf_counter() - t0
+        if dt > 0.5 or n0 >= batch.R:
+            break
+        n0 = min(batch.R, 4 * n0)
+    n = int(min(batch.R, max(n0, n0 * target_seconds / dt)))
+    t0 = time.perf_counter()
+    run(n)
+    dt = time.perf_counter() - t0
+    samples = n * batch.S
+    desc = (f"{n} rays x {batch.S} samples, K={r['numADCSample']} (fwd+bwd, torch autograd), "
+            f"{dt:.1f} s")
+    return samples / dt, nthreads, desc

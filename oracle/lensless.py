@@ -69,10 +69,10 @@ def lensless_forward(sdf, grad, dists, pts, o, d, inv_s, power, K, fs, As_sci, f
     dpath = 2.0 * torch.linalg.norm(pts - oo, dim=-1)
     amp = torch.where(live, sigma * spec / dpath ** 2, torch.zeros_like(sigma))
     tau = (dpath + bias) / c
-    k = torch.arange(K, dtype=torch.float64)
+    k = torch.arange(K, dtype=sdf.dtype)
     phase = 2.0 * math.pi * (f0 + As_sci * (k / fs))[None, None, :] * tau[..., None]
     sig = (amp[..., None] * torch.polar(torch.ones_like(phase), phase)).sum(1)
-    nv = valid.sum(-1).clamp(min=1).to(torch.float64)
+    nv = valid.sum(-1).clamp(min=1).to(sdf.dtype)
     out = sig / nv[:, None]
     if return_parts:
         return out, dict(alpha=alpha, trans=trans, sigma=torch.where(valid, sigma, 0 * sigma),
