@@ -154,6 +154,62 @@ def MatchedFilter_backward_realimag(grad_output_real, grad_output_imag, real, im
                      A, K, N, A, params, _native.stream_of(grad_real))
 
 
+def MatchedFilter_fft_forward(real, imag, tx_pos, rx_pos, output_real, output_imag, voxel,
+                              total_ant_size: int, num_range_bins: int, fft_spacing: float,
+                              adc_sample_rate: float, chirp_slope_scaled: float,
+                              num_voxels: int) -> None:
+    """In-place range-bin forward; reference: MatchedFilter_cuda.cu:212-250 (same
+    checks as MatchedFilter_forward with num_range_bins as the sample count)."""
+    _check_common(real, imag, tx_pos, rx_pos, voxel)
+    for t, n in ((output_real, "output_real"), (output_imag, "output_imag")):
+        _check(t.is_cuda, f"`{n}` must be a CUDA tensor.")
+        _check(t.is_contiguous(), f"`{n}` must be contiguous.")
+        _check(t.dim() == 1, f"`{n}` must have shape [N].")
+    A, R, N = int(total_ant_size), int(num_range_bins), int(num_voxels)
+    _check(tuple(real.shape) == (A, R), "`real` antenna dimension mismatch.")
+    _check(tuple(imag.shape) == (A, R), "`imag` antenna dimension mismatch.")
+    _check(tx_pos.size(0) == A and rx_pos.size(0) == A, "`tx_pos` antenna dimension mismatch.")
+    _check(tuple(voxel.shape) == (3, N), "`voxel` must be laid out as [3, N].")
+    _check(output_real.size(0) == N and output_imag.size(0) == N,
+           "`output_real` voxel dimension mismatch.")
+    dtype = _check_dtypes(real, imag, tx_pos, rx_pos, output_real, output_imag, voxel)
+    _check(dtype != torch.float16, "range-bin matched filter supports float32/float64")
+    params = _params(adc_sample_rate, chirp_slope_scaled)
+    name = "geraf_mf_fft_forward_" + ("f32" if dtype == torch.float32 else "f64")
+    with torch.cuda.device(real.device):
+        _native.call(name, *map(_native.ptr, (real, imag, tx_pos, rx_pos, output_real,
+                                             output_imag, voxel)),
+                     A, R, N, _F32(fft_spacing), params, _native.stream_of(real))
+
+
+def MatchedFilter_fft_backward_realimag(grad_output_real, grad_output_imag, real, imag, tx_pos,
+                                        rx_pos, grad_real, grad_imag, voxel,
+                                        total_ant_size: int, num_range_bins: int,
+                                        fft_spacing: float, adc_sample_rate: float,
+                                        chirp_slope_scaled: float, num_voxels: int) -> None:
+    """In-place range-bin adjoint, divided by total_ant_size; reference:
+    MatchedFilter_cuda.cu:298-343."""
+    _check_common(real, imag, tx_pos, rx_pos, voxel)
+    A, R, N = int(total_ant_size), int(num_range_bins), int(num_voxels)
+    for t, n in ((grad_output_real, "grad_output_real"), (grad_output_imag, "grad_output_imag"),
+                 (grad_real, "grad_real"), (grad_imag, "grad_imag")):
+        _check(t.is_cuda, f"`{n}` must be a CUDA tensor.")
+        _check(t.is_contiguous(), f"`{n}` must be contiguous.")
+    _check(tuple(grad_output_real.shape) == (N,), "`grad_output_real` must have shape [N].")
+    _check(tuple(grad_output_imag.shape) == (N,), "`grad_output_imag` must have shape [N].")
+    _check(tuple(grad_real.shape) == (A, R), "`grad_real` must have shape [A, S].")
+    _check(tuple(grad_imag.shape) == (A, R), "`grad_imag` must have shape [A, S].")
+    dtype = _check_dtypes(grad_output_real, grad_output_imag, tx_pos, rx_pos, grad_real,
+                          grad_imag, voxel)
+    _check(dtype != torch.float16, "range-bin matched filter supports float32/float64")
+    params = _params(adc_sample_rate, chirp_slope_scaled)
+    name = "geraf_mf_fft_backward_" + ("f32" if dtype == torch.float32 else "f64")
+    with torch.cuda.device(grad_real.device):
+        _native.call(name, *map(_native.ptr, (grad_output_real, grad_output_imag, tx_pos, rx_pos,
+                                             grad_real, grad_imag, voxel)),
+                     A, R, N, A, _F32(fft_spacing), params, _native.stream_of(grad_real))
+
+
 def get_compiler_version() -> str:
     return _native.version()
 

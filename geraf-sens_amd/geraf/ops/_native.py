@@ -53,6 +53,10 @@ _SIGNATURES = {
     "geraf_mf_forward_f64": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_backward_f32": [_P] * 7 + [_I64] * 4 + [_PARAMS, _P],
     "geraf_mf_backward_f64": [_P] * 7 + [_I64] * 4 + [_PARAMS, _P],
+    "geraf_mf_fft_forward_f32": [_P] * 7 + [_I64] * 3 + [ctypes.c_double, _PARAMS, _P],
+    "geraf_mf_fft_forward_f64": [_P] * 7 + [_I64] * 3 + [ctypes.c_double, _PARAMS, _P],
+    "geraf_mf_fft_backward_f32": [_P] * 7 + [_I64] * 4 + [ctypes.c_double, _PARAMS, _P],
+    "geraf_mf_fft_backward_f64": [_P] * 7 + [_I64] * 4 + [ctypes.c_double, _PARAMS, _P],
     "geraf_st_forward_f32": [_P] * 8 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_st_forward_f64": [_P] * 8 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_st_backward_f32": [_P] * 10 + [_I64] * 3 + [_PARAMS, _P],

@@ -101,6 +101,42 @@ int geraf_mf_backward_f64(const double* grad_out_real, const double* grad_out_im
                           int64_t N, int64_t norm_count, const geraf_fmcw_params* params,
                           geraf_stream_t stream);
 
+/* Range-bin ("FFT") matched filter on range-compressed echoes real/imag
+ * [A, num_range_bins]: with d = |tx-v| + |rx-v| + path_bias, pairs with d < 0
+ * or d > fft_spacing*num_range_bins are skipped; bin = floor(d/fft_spacing/2)
+ * (skipped if >= num_range_bins);
+ *   out[v] = (1/A) sum_a s[a,bin] * exp(-j*2*pi*f0*d/c)
+ * (params->slope and params->fs are unused). Distance, bin and phase in FP64.
+ * Replaces _C.MatchedFilter_fft_forward (vision.cpp:85-88;
+ * MatchedFilter_cuda_kernel.cu:117-169). */
+int geraf_mf_fft_forward_f32(const float* real, const float* imag, const float* tx_pos,
+                             const float* rx_pos, float* out_real, float* out_imag,
+                             const float* voxel, int64_t A, int64_t num_range_bins, int64_t N,
+                             double fft_spacing, const geraf_fmcw_params* params,
+                             geraf_stream_t stream);
+int geraf_mf_fft_forward_f64(const double* real, const double* imag, const double* tx_pos,
+                             const double* rx_pos, double* out_real, double* out_imag,
+                             const double* voxel, int64_t A, int64_t num_range_bins, int64_t N,
+                             double fft_spacing, const geraf_fmcw_params* params,
+                             geraf_stream_t stream);
+
+/* Adjoint of the range-bin matched filter into grad_real/grad_imag
+ * [A, num_range_bins], divided by norm_count; num_range_bins <= 8192.
+ * Replaces _C.MatchedFilter_fft_backward_realimag (vision.cpp:93-96;
+ * MatchedFilter_cuda_kernel.cu:222-279). */
+int geraf_mf_fft_backward_f32(const float* grad_out_real, const float* grad_out_imag,
+                              const float* tx_pos, const float* rx_pos, float* grad_real,
+                              float* grad_imag, const float* voxel, int64_t A,
+                              int64_t num_range_bins, int64_t N, int64_t norm_count,
+                              double fft_spacing, const geraf_fmcw_params* params,
+                              geraf_stream_t stream);
+int geraf_mf_fft_backward_f64(const double* grad_out_real, const double* grad_out_imag,
+                              const double* tx_pos, const double* rx_pos, double* grad_real,
+                              double* grad_imag, const double* voxel, int64_t A,
+                              int64_t num_range_bins, int64_t N, int64_t norm_count,
+                              double fft_spacing, const geraf_fmcw_params* params,
+                              geraf_stream_t stream);
+
 /* ---------------------------------------------------------------- signal tracer */
 
 /* [A,P] signal tracer forward:

@@ -35,7 +35,8 @@ def _load():
         lib = ctypes.CDLL(LIB_PATH)
         for fn in ("orc_mf_forward_f64", "orc_mf_forward_ref32", "orc_mf_backward_f64",
                    "orc_st_forward_f64", "orc_st_backward_f64", "orc_bilinear_forward_f64",
-                   "orc_bilinear_backward_f64"):
+                   "orc_bilinear_backward_f64", "orc_mf_fft_forward_f64",
+                   "orc_mf_fft_backward_f64"):
             getattr(lib, fn).restype = None
         _lib = lib
     return _lib
@@ -158,3 +159,41 @@ def bilinear_backward(g_n, g_s, z, tz, S):
                                   ctypes.c_int64(A), ctypes.c_int64(R), ctypes.c_int64(S),
                                   ctypes.c_int64(T))
     return gn, gs
+
+
+def _fft_max_d(spacing, bins):
+    return float(np.float32(np.float32(spacing) * np.float32(bins)))
+
+
+def mf_fft_forward(real, imag, tx, rx, voxel_n3, spacing, f0, c, bias, slope=0.0, fs=1.0):
+    """Range-bin matched filter: echoes [A,R] -> complex [N] (FP64)."""
+    lib = _load()
+    re, im, t, r = map(_d, (real, imag, tx, rx))
+    A, R = re.shape
+    vox = _d(np.asarray(voxel_n3).T)
+    N = vox.shape[1]
+    o_r = np.zeros(N)
+    o_i = np.zeros(N)
+    lib.orc_mf_fft_forward_f64(_p(re), _p(im), _p(t), _p(r), _p(vox), _p(o_r), _p(o_i),
+                               ctypes.c_int64(A), ctypes.c_int64(R), ctypes.c_int64(N),
+                               ctypes.c_double(spacing), ctypes.c_double(_fft_max_d(spacing, R)),
+                               ctypes.byref(_params(f0, slope, fs, c, bias)))
+    return o_r + 1j * o_i
+
+
+def mf_fft_backward(g_out, tx, rx, voxel_n3, R, spacing, norm, f0, c, bias, slope=0.0, fs=1.0):
+    """Adjoint of mf_fft_forward: complex [N] -> complex [A,R] (FP64)."""
+    lib = _load()
+    g = np.asarray(g_out)
+    gr, gi, t, r = _d(g.real), _d(g.imag), _d(tx), _d(rx)
+    A = t.shape[0]
+    vox = _d(np.asarray(voxel_n3).T)
+    N = vox.shape[1]
+    o_r = np.zeros((A, R))
+    o_i = np.zeros((A, R))
+    lib.orc_mf_fft_backward_f64(_p(gr), _p(gi), _p(t), _p(r), _p(vox), _p(o_r), _p(o_i),
+                                ctypes.c_int64(A), ctypes.c_int64(R), ctypes.c_int64(N),
+                                ctypes.c_int64(norm), ctypes.c_double(spacing),
+                                ctypes.c_double(_fft_max_d(spacing, R)),
+                                ctypes.byref(_params(f0, slope, fs, c, bias)))
+    return o_r + 1j * o_i

@@ -324,3 +324,60 @@ void orc_bilinear_backward_f64(const double* g_n, const double* g_s, const doubl
     }
   }
 }
+
+/* ------------------------------------------------------------------ range-bin matched filter */
+
+/* MatchedFilter_cuda_kernel.cu:117-169 / :222-279.  d includes the bias; pairs
+ * with d < 0 or d > max_d (the reference's FP32 fft_spacing*num_range_bins) are
+ * skipped; bin = floor(d/spacing/2). */
+static int64_t fft_bin(double d, double spacing, double max_d, int64_t R) {
+  if (d < 0.0 || d > max_d) return -1;
+  const double b = floor(d / spacing / 2.0);
+  return (b < 0.0 || b >= (double)R) ? -1 : (int64_t)b;
+}
+
+void orc_mf_fft_forward_f64(const double* re, const double* im, const double* tx,
+                            const double* rx, const double* voxel, double* out_re,
+                            double* out_im, int64_t A, int64_t R, int64_t N, double spacing,
+                            double max_d, const orc_params* p) {
+#pragma omp parallel for
+  for (int64_t v = 0; v < N; ++v) {
+    const double x = voxel[v], y = voxel[N + v], z = voxel[2 * N + v];
+    double ar = 0.0, ai = 0.0;
+    for (int64_t a = 0; a < A; ++a) {
+      const double d = orc_dist(tx + 3 * a, x, y, z) + orc_dist(rx + 3 * a, x, y, z) + p->bias;
+      const int64_t b = fft_bin(d, spacing, max_d, R);
+      if (b < 0) continue;
+      const double phase = TWO_PI * p->f0 * (d / p->c);
+      const double c = cos(phase), ns = -sin(phase);
+      ar += c * re[a * R + b] - ns * im[a * R + b];
+      ai += c * im[a * R + b] + ns * re[a * R + b];
+    }
+    out_re[v] = ar / (double)A;
+    out_im[v] = ai / (double)A;
+  }
+}
+
+void orc_mf_fft_backward_f64(const double* gr, const double* gi, const double* tx,
+                             const double* rx, const double* voxel, double* g_re, double* g_im,
+                             int64_t A, int64_t R, int64_t N, int64_t norm, double spacing,
+                             double max_d, const orc_params* p) {
+#pragma omp parallel for collapse(2)
+  for (int64_t a = 0; a < A; ++a) {
+    for (int64_t b = 0; b < R; ++b) {
+      double ar = 0.0, ai = 0.0;
+      for (int64_t v = 0; v < N; ++v) {
+        const double x = voxel[v], y = voxel[N + v], z = voxel[2 * N + v];
+        const double d =
+            orc_dist(tx + 3 * a, x, y, z) + orc_dist(rx + 3 * a, x, y, z) + p->bias;
+        if (fft_bin(d, spacing, max_d, R) != b) continue;
+        const double phase = TWO_PI * p->f0 * (d / p->c);
+        const double c = cos(phase), ns = -sin(phase);
+        ar += c * gr[v] + ns * gi[v];
+        ai += (-ns) * gr[v] + c * gi[v];
+      }
+      g_re[a * R + b] = ar / (double)norm;
+      g_im[a * R + b] = ai / (double)norm;
+    }
+  }
+}

@@ -211,3 +211,34 @@ def test_cfg2_subvolume_vs_fp64(cuda):
     few = sel[:8]
     ref = orc.mf_forward(sc.echo.real, sc.echo.imag, sc.tx, sc.rx, sc.voxels[few], **sc.params())
     assert rel_l2(out64[:8], ref) <= 1e-9
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_range_bin_mf_vs_oracle(cuda, dtype):
+    from geraf.ops.RFMatchedFilter import _C
+
+    rng = np.random.default_rng(21)
+    A, R, N = 12, 128, 3000
+    tx = rng.uniform(-0.03, 0.03, (A, 3)).astype(np.float32)
+    tx[:, 2] = 0
+    rx = (tx + np.float32([0.001, 0, 0])).astype(np.float32)
+    vox = (rng.uniform(-0.08, 0.08, (N, 3)) + [0, 0, 0.3]).astype(np.float32)
+    s = (rng.standard_normal((A, R)) + 1j * rng.standard_normal((A, R))).astype(np.complex64)
+    g = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    spacing = float(np.float32(0.008))  # cutoff 1.02 m > the ~0.75 m paths
+    T = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda, dtype=dtype)  # noqa: E731
+    o_r = torch.empty(N, device=cuda, dtype=dtype)
+    o_i = torch.empty_like(o_r)
+    _C.MatchedFilter_fft_forward(T(s.real), T(s.imag), T(tx), T(rx), o_r, o_i, T(vox.T), A, R,
+                                 spacing, 1.25e6, 701500.0, N)
+    prm = dict(f0=mf_scenes.MF_F0, c=mf_scenes.SPEED_OF_LIGHT, bias=mf_scenes.PATH_BIAS)
+    ref = orc.mf_fft_forward(s.real, s.imag, tx, rx, vox, spacing, **prm)
+    tol = 1e-6 if dtype == torch.float32 else 1e-12
+    assert rel_l2(o_r.double().cpu().numpy() + 1j * o_i.double().cpu().numpy(), ref) <= tol
+    g_r = torch.empty((A, R), device=cuda, dtype=dtype)
+    g_i = torch.empty_like(g_r)
+    _C.MatchedFilter_fft_backward_realimag(T(g.real), T(g.imag), T(s.real), T(s.imag), T(tx),
+                                           T(rx), g_r, g_i, T(vox.T), A, R, spacing, 1.25e6,
+                                           701500.0, N)
+    refg = orc.mf_fft_backward(g, tx, rx, vox, R, spacing, A, **prm)
+    assert rel_l2(g_r.double().cpu().numpy() + 1j * g_i.double().cpu().numpy(), refg) <= tol

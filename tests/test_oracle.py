@@ -303,3 +303,32 @@ def test_lensless_oracle_gradients_vs_finite_differences(seed, uneven, offset):
             assert abs(fd - ad) <= 1e-4 * scale, (name, i, fd, ad)
             checked += 1
     assert checked >= 30
+
+
+def test_range_bin_mf_known_answer_and_adjoint():
+    rng = np.random.default_rng(12)
+    A, R, N = 6, 64, 40
+    tx = rng.uniform(-0.02, 0.02, (A, 3))
+    tx[:, 2] = 0
+    rx = tx + [0.001, 0, 0]
+    v = rng.uniform(-0.05, 0.05, (N, 3)) + [0, 0, 0.3]
+    spacing = 0.02  # bins of 4 cm of round-trip path; cutoff spacing*R = 1.28 m
+    prm = dict(f0=F0, c=C, bias=BIAS)
+    # echo holding exp(+j*2*pi*f0*d/c) in voxel 0's bin focuses voxel 0 to exactly 1
+    s = np.zeros((A, R), complex)
+    for a in range(A):
+        d = np.linalg.norm(tx[a] - v[0]) + np.linalg.norm(rx[a] - v[0]) + BIAS
+        s[a, int(np.floor(d / spacing / 2))] = np.exp(2j * np.pi * F0 * d / C)
+    out = orc.mf_fft_forward(s.real, s.imag, tx, rx, v, spacing, **prm)
+    assert abs(out[0] - 1.0) < 1e-12
+    # adjoint identity
+    s = rng.standard_normal((A, R)) + 1j * rng.standard_normal((A, R))
+    g = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    fwd = orc.mf_fft_forward(s.real, s.imag, tx, rx, v, spacing, **prm)
+    adj = orc.mf_fft_backward(g, tx, rx, v, R, spacing, A, **prm)
+    lhs = np.sum(fwd.real * g.real + fwd.imag * g.imag)
+    rhs = np.sum(s.real * adj.real + s.imag * adj.imag)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    # pairs beyond spacing * R are skipped
+    far = orc.mf_fft_forward(s.real, s.imag, tx, rx, v + [0, 0, 5.0], spacing, **prm)
+    assert np.all(far == 0)
