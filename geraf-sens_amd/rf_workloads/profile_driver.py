@@ -1,0 +1,74 @@
+"""One launch of each hot kernel on its benchmark workload, for rocprofv3
+counter passes, plus a 1 GiB device copy whose FETCH_SIZE / WRITE_SIZE
+readings calibrate those counters (known traffic: 1 GiB read, 1 GiB written).
+
+    rocprofv3 --pmc <counters> -d <dir> -o pmc -- python3 geraf-sens_amd/rf_workloads/profile_driver.py
+
+Kernels, in dispatch order: the copy; mf_forward_horner_kernel on the 64x64
+MIMO / K=512 / 128^3 workload; lensless_forward_kernel and
+lensless_backward_kernel on 4,096 rays x 128 samples, K=256.
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+sys.path[:0] = [PKG]
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+from geraf.ops import _native  # noqa: E402
+from geraf.ops.RFSignalTracer import _C as stC  # noqa: E402
+from rf_workloads import mf_scenes, tracer_scenes as ts  # noqa: E402
+
+
+def main() -> None:
+    dev = torch.device("cuda:0")
+    print("profile_driver: 1 GiB copy", flush=True)
+    a = torch.empty(1 << 28, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+    print("profile_driver: matched filter 64x64 MIMO, K=512, 128^3", flush=True)
+    sc = mf_scenes.config2(0, device=str(dev))
+    A, K, N = sc.A, sc.K, sc.N
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), device=dev)  # noqa: E731
+    re, im, tx, rx, vox = t(sc.echo.real), t(sc.echo.imag), t(sc.tx), t(sc.rx), t(sc.voxels.T)
+    o_r = torch.empty(N, device=dev)
+    o_i = torch.empty(N, device=dev)
+    prm = _native.make_params(sc.f0, sc.As_sci, sc.fs, sc.c, sc.bias)
+    _native.call("geraf_mf_forward_f32", *map(_native.ptr, (re, im, tx, rx, o_r, o_i, vox)),
+                 A, K, N, prm, _native.stream_of(re))
+    torch.cuda.synchronize()
+
+    print("profile_driver: lensless tracer 4096 x 128, K=256", flush=True)
+    bb = ts.config3(0).to(dev)
+    R, S, Kt = bb.R, bb.S, bb.radar["numADCSample"]
+    tp = stC.tracer_params(bb.radar["fc_start"], bb.radar["As_sci"], bb.radar["adcSampleRate"])
+    P = _native.ptr
+    inv_s = bb.inv_s.reshape(1).contiguous()
+    power = bb.power.reshape(1).contiguous()
+    ins = [P(bb.sdf), P(bb.grad), P(bb.refl), P(bb.dists), P(bb.z), P(bb.o), P(bb.d), P(None),
+           P(None), P(inv_s), P(power)]
+    lr = torch.empty((R, Kt), device=dev)
+    li = torch.empty_like(lr)
+    st = _native.stream_of(lr)
+    _native.call("geraf_lensless_forward_f32", *ins, P(lr), P(li), P(None), R, S, Kt, bb.anneal,
+                 tp, st)
+    g_s = torch.empty((R, S), device=dev)
+    g_g = torch.empty((R, S, 3), device=dev)
+    g_r = torch.empty((R, S), device=dev)
+    g_i = torch.empty(R, device=dev)
+    g_p = torch.empty(R, device=dev)
+    _native.call("geraf_lensless_backward_f32", P(lr), P(li), *ins, P(g_s), P(g_g), P(g_r),
+                 P(g_i), P(g_p), R, S, Kt, bb.anneal, tp, st)
+    torch.cuda.synchronize()
+    print("profile_driver: done", flush=True)
+
+
+if __name__ == "__main__":
+    main()
