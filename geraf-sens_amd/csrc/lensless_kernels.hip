@@ -99,6 +99,12 @@ __device__ __forceinline__ void load_and_composite(const LenslessArgs& a, size_t
   x.t = 1.0f - x.alpha + 1.0e-7f;
 }
 
+// Number of set bits of `mask` below `lane`.
+__device__ __forceinline__ int lane_prefix(unsigned long long mask, int lane) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
 // Inclusive prefix product over the wave.
 __device__ __forceinline__ float wave_incl_prod(float v, int lane) {
 #pragma unroll
@@ -184,15 +190,21 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
       cyc1 = pc.dfs * tau;
     }
     if (sigma_out && valid && ray_ok) sigma_out[idx] = sigma;
-    s_amp[wave][lane] = amp;
-    s_fr0[wave][lane] = fr0;
-    s_cyc1[wave][lane] = cyc1;
+    // compact the tile's emitting samples to the front of the LDS slots, so
+    // the frequency loop below only visits them
+    const bool emit = e.live && amp != 0.0f;
+    const unsigned long long mask = __ballot(emit);
+    const int slot = lane_prefix(mask, lane);
+    if (emit) {
+      s_amp[wave][slot] = amp;
+      s_fr0[wave][slot] = fr0;
+      s_cyc1[wave][slot] = cyc1;
+    }
+    const int cnt = __popcll(mask);
     wave_sync();
     if (chunk_ok) {
-      const int cnt = min(kTile, nv - base);
       for (int i = grp; i < cnt; i += groups) {
         const float am = s_amp[wave][i];
-        if (am == 0.0f) continue;
         const double c1 = s_cyc1[wave][i];
         const float2 seed = phasor_cycles(
             (float)centered_frac((double)s_fr0[wave][i] + (double)k0 * c1), 1.0f);

@@ -40,6 +40,10 @@ constexpr int kHornerBlock = 32;
 #ifndef GERAF_MF_WAVES
 #define GERAF_MF_WAVES 4
 #endif
+// Distance evaluation of the tuned forward (FP64 accurate either way).
+#ifndef GERAF_MF_DIST
+#define GERAF_MF_DIST dist3_f64_fast
+#endif
 
 // ---------------------------------------------------------------- tuned forward
 
@@ -92,13 +96,16 @@ __global__ __launch_bounds__(64 * WAVES) void mf_forward_horner_kernel(
     PathPhasors ph[VPL];
 #pragma unroll
     for (int i = 0; i < VPL; ++i) {
-      const double d = dist3_f64(tx, ty, tz, px[i], py[i], pz[i]) +
-                       dist3_f64(rx, ry, rz, px[i], py[i], pz[i]);
+      const double d = GERAF_MF_DIST(tx, ty, tz, px[i], py[i], pz[i]) +
+                       GERAF_MF_DIST(rx, ry, rz, px[i], py[i], pz[i]);
       ph[i] = path_phasors((d + pc.bias) * pc.inv_c, pc, -1.0f);
     }
 
-    const float* __restrict__ er = real + (size_t)a * K;
-    const float* __restrict__ ei = imag + (size_t)a * K;
+    // a*K < 2^31 (checked on the host); readfirstlane keeps the row offset,
+    // and with it every echo address below, in SGPRs.
+    const int row = __builtin_amdgcn_readfirstlane(a * K);
+    const float* __restrict__ er = real + row;
+    const float* __restrict__ ei = imag + row;
 
     float Sr[VPL], Si[VPL];
 #pragma unroll
@@ -376,6 +383,7 @@ extern "C" int geraf_mf_forward_f32(const float* real, const float* imag, const 
   if (N == 0) return GERAF_RF_OK;
   require_ptrs({real, imag, tx_pos, rx_pos}, A * K > 0 || A > 0);
   require_ptrs({out_real, out_imag, voxel}, true);
+  if (A * K >= (INT64_C(1) << 31)) throw ArgError("A*K must be < 2^31");
   const PhaseConsts pc = make_phase_consts(*params);
   const int n = (int)N;
   const int tiles = (n + kWave - 1) / kWave;

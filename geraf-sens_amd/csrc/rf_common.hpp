@@ -42,14 +42,53 @@ __device__ __forceinline__ double dist3_f64(double ax, double ay, double az, dou
   return sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
 }
 
+// sqrt(D) for D >= 0 to ~1e-14 relative: an FP32 reciprocal-square-root seed
+// (about 1 ulp, 6e-8) refined by one FP64 Newton step (error ~1.5 * 6e-8^2).
+// Far cheaper than a correctly rounded FP64 sqrt; returns 0 for D == 0.
+__device__ __forceinline__ double sqrt_f64_newton(double D) {
+  const double y = (double)__builtin_amdgcn_rsqf((float)D);
+  const double h = 0.5 * D * (y * y);
+  const double r = D * fma(y, 0.5 - h, y);  // D * y * (1.5 - h)
+  return D > 0.0 ? r : 0.0;
+}
+
+__device__ __forceinline__ double dist3_f64_fast(double ax, double ay, double az, double bx,
+                                                 double by, double bz) {
+  const double dx = ax - bx, dy = ay - by, dz = az - bz;
+  return sqrt_f64_newton(fma(dx, dx, fma(dy, dy, dz * dz)));
+}
+
 // Fractional part of x in [-0.5, 0.5].
 __device__ __forceinline__ double centered_frac(double x) { return x - rint(x); }
 
-// (cos(2*pi*x), sign*sin(2*pi*x)) for |x| <= 0.5 (x in cycles).
+// (cos(2*pi*x), sign*sin(2*pi*x)) for finite |x| <= 0.5 (x in cycles, already
+// range-reduced in FP64 by the caller).  Quadrant reduction q = rint(4x),
+// r = x - q/4 (exact), theta = 2*pi*r in [-pi/4, pi/4]; Taylor polynomials of
+// degree 9 (sin) and 10 (cos), truncation < 3e-9; then the quadrant rotation.
+// About 1 ulp, with none of the general sincospif's range reduction and
+// special-case handling.
 __device__ __forceinline__ float2 phasor_cycles(float x, float sign) {
-  float s, c;
-  sincospif(2.0f * x, &s, &c);
-  return make_float2(c, sign * s);
+  const float qf = rintf(4.0f * x);
+  const float r = fmaf(qf, -0.25f, x);
+  const float t = 6.28318530717958647692f * r;
+  const float t2 = t * t;
+  float ps = fmaf(t2, 2.75573192e-6f, -1.98412698e-4f);
+  ps = fmaf(t2, ps, 8.33333333e-3f);
+  ps = fmaf(t2, ps, -1.66666667e-1f);
+  const float sn = fmaf(t * t2, ps, t);
+  float pc = fmaf(t2, -2.75573192e-7f, 2.48015873e-5f);
+  pc = fmaf(t2, pc, -1.38888889e-3f);
+  pc = fmaf(t2, pc, 4.16666667e-2f);
+  pc = fmaf(t2, pc, -0.5f);
+  const float cs = fmaf(t2, pc, 1.0f);
+  const int q = (int)qf;
+  // rotate (cs, sn) by q quarter turns
+  const bool odd = q & 1;
+  float s2 = odd ? cs : sn;
+  float c2 = odd ? sn : cs;
+  if (q & 2) s2 = -s2;
+  if ((q + 1) & 2) c2 = -c2;
+  return make_float2(c2, sign * s2);
 }
 
 // Complex helpers on float2 = (re, im).
