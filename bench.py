@@ -37,9 +37,22 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool):
-    """Fused lensless tracer, forward + backward, on the 4,096-ray x 128-sample,
-    K=256 workload.  Returns a dict for the JSON line."""
+def emitting_fraction(b) -> float:
+    """Share of valid samples whose monostatic specular term is not culled
+    (inc.n <= 0 and spec >= 1e-6; unit normals, inc = ray direction)."""
+    import torch
+
+    n = torch.nn.functional.normalize(b.grad, dim=-1)
+    idn = (n * b.d[:, None, :]).sum(-1)
+    spec = 2.0 * idn * idn - 1.0
+    return float(((idn <= 0) & (spec >= 1e-6)).float().mean())
+
+
+def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
+                 rays: int = 4096, samples: int = 128, K: int = 256, oracle_check: bool = True):
+    """Fused lensless tracer, forward + backward, on `rays` x `samples`, K
+    frequencies (default: the 4,096 x 128, K=256 workload).  Returns a dict
+    for the JSON line."""
     import numpy as np
     import torch
 
@@ -47,8 +60,9 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool)
     from geraf.ops.RFSignalTracer import _C as stC
     from rf_workloads import tracer_scenes as ts
 
-    b0 = ts.config3(0)
+    b0 = ts.rays(rays, samples, K, seed=0)
     b = b0.to(dev)
+    live = emitting_fraction(b)
     R, S, K = b.R, b.S, b.radar["numADCSample"]
     prm = stC.tracer_params(b.radar["fc_start"], b.radar["As_sci"], b.radar["adcSampleRate"])
     o_r = torch.empty((R, K), device=dev)
@@ -97,15 +111,17 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool)
     # algorithmic traffic and work
     bytes_fwd = samples * 28 + R * (24 + 8 * K) + 8
     bytes_bwd = samples * (28 + 20) + R * (24 + 8 * K + 8) + 8
-    flops = 16.0 * K * samples
+    flops = 16.0 * K * samples * live  # only emitting samples carry signal work
     gbs = (bytes_fwd + bytes_bwd) / ((f_ms + b_ms) / 1e3) / 1e9
     tfs = flops / ((f_ms + b_ms) / 1e3) / 1e12
-    log(f"tracer: fwd {f_ms:.3f} ms, bwd {b_ms:.3f} ms, {rate:.3e} samples/s, "
-        f"{gbs:.0f} GB/s, {tfs:.1f} TFLOP/s")
+    log(f"tracer {R}x{S} K={K}: fwd {f_ms:.3f} ms, bwd {b_ms:.3f} ms, {rate:.3e} samples/s, "
+        f"{gbs:.0f} GB/s, {tfs:.1f} TFLOP/s (emitting fraction {live:.3f})")
 
     # accuracy against the FP64 oracle on a ray subset
     errs = {}
     try:
+        if not oracle_check:
+            raise RuntimeError("not requested for this size")
         from oracle import lensless as L
         n = 256
         sub = {k: getattr(b0, k)[:n].double().clone().requires_grad_(True)
@@ -146,8 +162,8 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool)
     return {
         "metric": "tracer fwd+bwd samples/s",
         "value": rate, "unit": "samples/s",
-        "workload": "fused lensless tracer, 4096 rays x 128 samples, K=256",
-        "fwd_ms": f_ms, "bwd_ms": b_ms,
+        "workload": f"fused lensless tracer, {R} rays x {S} samples, K={K}",
+        "fwd_ms": f_ms, "bwd_ms": b_ms, "emitting_fraction": live,
         "roofline": {"hbm_gbs": gbs, "hbm_frac": gbs / 8000.0, "valu_tflops": tfs,
                      "valu_frac": tfs / VALU_FP32_PEAK_TFLOPS,
                      "bytes_per_step": bytes_fwd + bytes_bwd, "flops_per_step": flops},
@@ -282,6 +298,8 @@ def main() -> int:
     if not args.no_tracer:
         tracer = bench_tracer(dev, max(args.steps, 5), max(args.warmup, 1), args.cpu_seconds / 2,
                               args.no_cpu)
+        tracer["stage2_size"] = bench_tracer(dev, max(args.steps, 3), 1, 0.0, True, rays=32768,
+                                             samples=256, K=512, oracle_check=False)
 
     line = {
         "metric": "MF voxel*antenna*freq terms/s",

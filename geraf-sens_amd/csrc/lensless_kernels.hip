@@ -42,7 +42,7 @@ namespace {
 
 constexpr int kRaysPerBlock = 4;
 constexpr int kTile = 64;
-constexpr int kMaxTiles = 64;  // S <= 4096
+constexpr int kMaxTiles = 32;  // S <= 2048
 constexpr int kSigChunk = 32;
 
 struct LenslessArgs {
@@ -100,7 +100,7 @@ __device__ __forceinline__ void load_and_composite(const LenslessArgs& a, size_t
 }
 
 // Number of set bits of `mask` below `lane`.
-__device__ __forceinline__ int lane_prefix(unsigned long long mask, int lane) {
+__device__ __forceinline__ int lane_prefix(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
     // the frequency loop below only visits them
     const bool emit = e.live && amp != 0.0f;
     const unsigned long long mask = __ballot(emit);
-    const int slot = lane_prefix(mask, lane);
+    const int slot = lane_prefix(mask);
     if (emit) {
       s_amp[wave][slot] = amp;
       s_fr0[wave][slot] = fr0;
@@ -245,14 +245,33 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
 
 // ------------------------------------------------------------------ backward
 
+// Per-wave LDS layout of the backward kernel.
+struct BwdLds {
+  double* btau;  // [2*kTile] tau of queued emitting samples
+  int* bsid;     // [2*kTile] their sample indices
+  float* carry;  // [kMaxTiles] transmittance at each tile start
+  float* h;      // [S] h_s (0 for samples that do not emit)
+};
+__host__ __device__ constexpr size_t bwd_lds_bytes_per_wave(int S) {
+  return 2 * kTile * sizeof(double) + 2 * kTile * sizeof(int) + kMaxTiles * sizeof(float) +
+         (size_t)S * sizeof(float);
+}
+
 __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
     LenslessArgs a, const float* __restrict__ g_real, const float* __restrict__ g_imag,
     float* __restrict__ grad_sdf, float* __restrict__ grad_g, float* __restrict__ grad_refl,
     float* __restrict__ grad_inv_s, float* __restrict__ grad_power, PhaseConsts pc) {
-  __shared__ float s_carry[kRaysPerBlock][kMaxTiles];
-
+  extern __shared__ double lds_raw[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned char* base_ptr =
+      reinterpret_cast<unsigned char*>(lds_raw) + (size_t)wave * bwd_lds_bytes_per_wave(a.S);
+  BwdLds L;
+  L.btau = reinterpret_cast<double*>(base_ptr);
+  L.bsid = reinterpret_cast<int*>(L.btau + 2 * kTile);
+  L.carry = reinterpret_cast<float*>(L.bsid + 2 * kTile);
+  L.h = L.carry + kMaxTiles;
+
   const int r = blockIdx.x * kRaysPerBlock + wave;
   const bool ray_ok = r < a.R;
   const int rr = ray_ok ? r : a.R - 1;
@@ -263,25 +282,66 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
   const float power = a.power[0];
   const float inv_n = nv > 0 ? 1.0f / (float)nv : 0.0f;
   const int ntiles = (nv + kTile - 1) / kTile;
+  const float* gr_row = g_real + (size_t)rr * a.K;
+  const float* gi_row = g_imag + (size_t)rr * a.K;
 
-  // pass 1: transmittance at the start of every tile
+  // h_s = (1/n) Re sum_k g[r,k] exp(-j phi_sk) for the first `cnt` queued
+  // samples, one per lane (the gradient row is wave-uniform: scalar loads).
+  auto horner_batch = [&](int cnt) {
+    if (lane < cnt) {
+      const PathPhasors ph = path_phasors(L.btau[lane], pc, -1.0f);
+      const float2 S = horner2(gr_row, gi_row, a.K, ph.w, ph.W);
+      L.h[L.bsid[lane]] = cmul(ph.base, S).x * inv_n;
+    }
+  };
+
+  // pass A (forward over tiles): transmittance carries, and h_s of every
+  // emitting sample, evaluated in full 64-lane batches of such samples
+  for (int s = lane; s < nv; s += kWave) L.h[s] = 0.0f;
+  wave_sync();
   float carry = 1.0f;
+  int queued = 0;
   for (int tile = 0; tile < ntiles; ++tile) {
-    if (lane == 0) s_carry[wave][tile] = carry;
+    if (lane == 0) L.carry[tile] = carry;
     const int s = tile * kTile + lane;
     const bool valid = s < nv;
     SampleState x;
     load_and_composite(a, (size_t)rr * a.S + (valid ? s : nv - 1), dir, inv_s, x);
     const float incl = wave_incl_prod(valid ? x.t : 1.0f, lane);
     carry *= __shfl(incl, kWave - 1, kWave);
+    const Emission e = emission(o, dir, x, valid);
+    const unsigned long long mask = __ballot(e.live);
+    if (e.live) {
+      const int slot = queued + lane_prefix(mask);
+      L.btau[slot] = (e.dpath + pc.bias) * pc.inv_c;
+      L.bsid[slot] = s;
+    }
+    queued += __popcll(mask);
+    wave_sync();
+    if (queued >= kWave) {
+      horner_batch(kWave);
+      const int rest = queued - kWave;
+      double t_keep = 0.0;
+      int s_keep = 0;
+      if (lane < rest) {
+        t_keep = L.btau[kWave + lane];
+        s_keep = L.bsid[kWave + lane];
+      }
+      wave_sync();
+      if (lane < rest) {
+        L.btau[lane] = t_keep;
+        L.bsid[lane] = s_keep;
+      }
+      queued = rest;
+      wave_sync();
+    }
   }
+  if (queued > 0) horner_batch(queued);
   wave_sync();
 
-  // pass 2: reverse over tiles
+  // pass B: reverse over tiles
   float w_carry = 0.0f;  // W at the first sample of the next tile
   float gis = 0.0f, gpow = 0.0f;
-  const float* gr_row = g_real + (size_t)rr * a.K;
-  const float* gi_row = g_imag + (size_t)rr * a.K;
   for (int tile = ntiles - 1; tile >= 0; --tile) {
     const int s = tile * kTile + lane;
     const bool valid = s < nv;
@@ -292,7 +352,7 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
     const float incl = wave_incl_prod(tt, lane);
     float excl = __shfl_up(incl, 1, kWave);
     if (lane == 0) excl = 1.0f;
-    const float T = s_carry[wave][tile] * excl;
+    const float T = L.carry[tile] * excl;
     const float Tc = T + x.off;
     const float rp = x.refl * power;
     const float m = rp * Tc;
@@ -304,9 +364,7 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
     float h = 0.0f, att = 0.0f;
     if (e.live) {
       att = (float)(1.0 / (e.dpath * e.dpath));
-      const PathPhasors ph = path_phasors((e.dpath + pc.bias) * pc.inv_c, pc, -1.0f);
-      const float2 S = horner2(gr_row, gi_row, a.K, ph.w, ph.W);
-      h = cmul(ph.base, S).x * inv_n;
+      h = L.h[s];
     }
     const float gsig = h * e.geo.spec * att;
     const float gspec = h * sigma * att;
@@ -390,7 +448,8 @@ LenslessArgs make_args(const float* sdf, const float* grad, const float* refl,
                        float anneal) {
   if (R < 0 || S < 0 || K < 0) throw ArgError("sizes must be non-negative");
   if (R > INT32_MAX / 2 || R * S > (INT64_C(1) << 36)) throw ArgError("too many samples");
-  if (S > kTile * kMaxTiles) throw ArgError("at most 4096 samples per ray");
+  // S <= 2048 keeps the backward's LDS (4 waves x (1.8 KB + 4 S B)) under 64 KB
+  if (S > 2048) throw ArgError("at most 2048 samples per ray");
   if (!(anneal >= 0.0f && anneal <= 1.0f)) throw ArgError("cos_anneal_ratio must be in [0,1]");
   require_ptrs({origin, dirs, inv_s, power}, R > 0);
   require_ptrs({sdf, grad, dists, z}, R > 0 && S > 0);
@@ -460,7 +519,8 @@ extern "C" int geraf_lensless_backward_f32(
   require_ptrs({grad_out_real, grad_out_imag}, K > 0);
   if (S == 0) throw ArgError("need at least one sample per ray");
   const unsigned blocks = (unsigned)((R + kRaysPerBlock - 1) / kRaysPerBlock);
-  hipLaunchKernelGGL(lensless_backward_kernel, dim3(blocks), dim3(64 * kRaysPerBlock), 0,
+  const size_t lds = kRaysPerBlock * bwd_lds_bytes_per_wave((int)S);
+  hipLaunchKernelGGL(lensless_backward_kernel, dim3(blocks), dim3(64 * kRaysPerBlock), lds,
                      (hipStream_t)stream, a, grad_out_real, grad_out_imag, grad_sdf, grad_grad,
                      grad_refl, grad_inv_s_per_ray, grad_power_per_ray,
                      make_phase_consts(*params));

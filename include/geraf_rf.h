@@ -214,7 +214,7 @@ int geraf_bilinear_backward_f64(const double* grad_out_normals, const double* gr
  * sdf, dists, z, refl, trans_offset: [R,S]; grad: [R,S,3]; origin, dirs: [R,3];
  * n_valid: [R] int32; inv_s, power: device scalars [1]; out_real/out_imag
  * [R,K]; sigma_out [R,S] (optional). refl, trans_offset, n_valid, sigma_out
- * may be NULL (1, 0, S, not written). Limits: S <= 4096, K <= 2048.
+ * may be NULL (1, 0, S, not written). Limits: S <= 2048, K <= 2048.
  * No counterpart in the reference: it fuses the torch glue of
  * rf_rendering.py:740-780 with the tracer op for the per-ray case. */
 int geraf_lensless_forward_f32(const float* sdf, const float* grad, const float* refl,
