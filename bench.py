@@ -37,6 +37,84 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (profiles/traffic.json, calibrated against a 1 GiB copy),
+    or None if absent."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)[kernel]
+        return t["fetch_bytes"] + t["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def bench_stage_step(dev, target_volume, axes, steps: int = 5, warmup: int = 2):
+    """Stage-1-style training step (8,192 rays x 64 samples, valid counts
+    16..64, K=64), fused form vs the all-FP32 autograd form."""
+    import torch
+
+    from rf_workloads import stage_step as st
+
+    out = {"workload": "stage-1-style step: 8192 rays x 64 samples (valid 16..64), K=64, "
+                       "SDF MLP 8x256 + signal MLP 4x256, L1 vs |MF| of the 128^3 volume"}
+    for form in ("fused", "fp32_autograd"):
+        step = st.StageStep(st.STAGE1, target_volume, axes, dev, form=form)
+        for _ in range(warmup):
+            step.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            loss = step.step()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        samples = int(step.valid.sum())
+        out[form] = {"ms_per_step": ms, "valid_samples_per_step": samples,
+                     "samples_per_s": samples / ms * 1e3, "last_loss": loss}
+        log(f"stage step [{form}]: {ms:.2f} ms/step, {samples} valid samples, loss {loss:.4f}")
+        del step
+        torch.cuda.empty_cache()
+    out["speedup_vs_fp32_autograd"] = out["fp32_autograd"]["ms_per_step"] / \
+        out["fused"]["ms_per_step"]
+    return out
+
+
+def bench_large_target(dev):
+    """The 256^3 target volume of the stage-2 workload (128x128 aperture,
+    K=1024): one 256x256x4 slab (1/64 of the volume) timed; terms/s is
+    independent of the slab size."""
+    import numpy as np
+    import torch
+
+    from geraf.ops import _native
+    from rf_workloads import mf_scenes
+
+    sc = mf_scenes.config2(0, device=str(dev), n_side=256, n_ap=128, K=1024, slab_z=(0, 4))
+    A, K, N = sc.A, sc.K, sc.N
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), device=dev)  # noqa: E731
+    args = [t(sc.echo.real), t(sc.echo.imag), t(sc.tx), t(sc.rx)]
+    o_r = torch.empty(N, device=dev)
+    o_i = torch.empty_like(o_r)
+    vox = t(sc.voxels.T)
+    prm = _native.make_params(sc.f0, sc.As_sci, sc.fs, sc.c, sc.bias)
+    call = lambda: _native.call("geraf_mf_forward_f32",  # noqa: E731
+                                *map(_native.ptr, args + [o_r, o_i, vox]), A, K, N, prm,
+                                _native.stream_of(o_r))
+    call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    call()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    rate = A * K * N / (ms / 1e3)
+    log(f"256^3 target slab ({N} voxels, A={A}, K={K}): {ms:.1f} ms, {rate:.3e} terms/s; "
+        f"full volume {A * K * 256**3 / rate:.1f} s")
+    return {"workload": "MF target 256^3 (128x128 aperture, K=1024), 256x256x4 slab",
+            "terms_per_s": rate, "slab_ms": ms, "full_volume_s": A * K * 256 ** 3 / rate}
+
+
 def emitting_fraction(b) -> float:
     """Share of valid samples whose monostatic specular term is not culled
     (inc.n <= 0 and spec >= 1e-6; unit normals, inc = ray direction)."""
@@ -180,6 +258,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tracer", action="store_true")
+    ap.add_argument("--no-stage", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -294,6 +373,13 @@ def main() -> int:
                          f"(per-term cos/sin) on {desc}"}
         log(f"cpu baseline {rate:.3e} terms/s on {threads} threads")
 
+    stage = large = None
+    if not args.no_stage:
+        axes = tuple(torch.tensor(a) for a in sc.axes)
+        vol = torch.complex(o_r, o_i).reshape(sc.grid_shape)
+        stage = bench_stage_step(dev, vol, axes)
+        large = bench_large_target(dev)
+
     tracer = None
     if not args.no_tracer:
         tracer = bench_tracer(dev, max(args.steps, 5), max(args.warmup, 1), args.cpu_seconds / 2,
@@ -319,7 +405,8 @@ def main() -> int:
                    "A": A, "K": K, "N": N, "terms": terms},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / VALU_FP32_PEAK_TFLOPS,
-                     "traffic": None, "kernel": "mf_forward_horner_kernel<4>",
+                     "traffic": profiled_traffic("mf_forward_horner_kernel<4,2>"),
+                     "kernel": "mf_forward_horner_kernel<4,2>",
                      "mean_launch_ms": 1e3 * mean_launch_s,
                      "flops_per_launch": FLOPS_PER_TERM * terms},
         "cpu_baseline": cpu,
@@ -330,6 +417,8 @@ def main() -> int:
                                     "tuned_speedup_vs_fp64": value / f64_rate,
                                     "tuned_speedup_vs_fp32": value / p32_rate},
         "tracer": tracer,
+        "stage1_step": stage,
+        "stage2_target_volume": large,
     }
     print(json.dumps(line), flush=True)
     return 0

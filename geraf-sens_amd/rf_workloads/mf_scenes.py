@@ -159,7 +159,7 @@ def box_torus_scatterers(rng: np.random.Generator, M: int, center, box=0.16, wal
 
 
 def config2(seed: int = 0, device: str = "cpu", n_side: int = 128, n_ap: int = 64,
-            K: int = 512) -> MFScene:
+            K: int = 512, slab_z=None) -> MFScene:
     """MIMO 64x64 virtual array (tx/rx offset -/+1 mm in x, 1.95 mm pitch),
     K=512, 128^3 voxels over a 20 cm cube at z=30 cm, 5,000 scatterers on a
     16 cm hollow box (5 mm walls) enclosing a 6 cm torus, log-normal(0,0.5)
@@ -171,14 +171,19 @@ def config2(seed: int = 0, device: str = "cpu", n_side: int = 128, n_ap: int = 6
     rx = (p + off).astype(np.float32)
     center = np.array([0.0, 0.0, 0.30])
     vox, axes = voxel_grid(n_side, 0.20, center)
+    if slab_z is not None:  # keep the z-planes [slab_z[0], slab_z[1]) of the grid
+        z0, z1 = slab_z
+        vox = vox.reshape(n_side, n_side, n_side, 3)[:, :, z0:z1].reshape(-1, 3).copy()
+        axes = (axes[0], axes[1], axes[2][z0:z1])
     M = 5000
     scat = box_torus_scatterers(rng, M, center)
     amp = rng.lognormal(0.0, 0.5, M) * np.exp(1j * rng.uniform(0.0, 2 * np.pi, M))
     fs, As = sweep(K)
     clean = synth_echo(tx, rx, scat, amp, K, fs, As, device=device).cpu().numpy()
     echo = add_noise(clean, 10.0, rng).astype(np.complex64)
-    return MFScene(f"mf_cfg2_mimo{n_ap}x{n_ap}_K{K}_{n_side}c", tx, rx, vox,
-                   (n_side, n_side, n_side), axes, scat, amp, K, fs, As, echo=echo)
+    shape = (n_side, n_side, n_side if slab_z is None else slab_z[1] - slab_z[0])
+    return MFScene(f"mf_cfg2_mimo{n_ap}x{n_ap}_K{K}_{n_side}c", tx, rx, vox, shape, axes, scat,
+                   amp, K, fs, As, echo=echo)
 
 
 def nearest_voxels(scene: MFScene, pts: np.ndarray) -> np.ndarray:

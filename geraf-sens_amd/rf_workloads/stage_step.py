@@ -1,0 +1,177 @@
+"""Stage-1/2-style training steps built on the RF ops (the step workloads of
+the benchmark).
+
+One step (GeRaFStage1.loss, rf_rendering.py:721-820, in its per-ray form):
+  1. SDF MLP (8 x 256, softplus(beta=100), skip at 4, 6-frequency PE,
+     geometric init) on the normalised sample points -> sdf, features, and the
+     SDF gradient with create_graph (the normals, differentiated again);
+  2. inv_s from the variance network; reflectance from the 4 x 256 signal MLP
+     on [features, points]; power = exp(light_power);
+  3. the fused lensless tracer renders each ray's FMCW signal (its antenna is
+     the ray origin);
+  4. the matched filter backprojects all rays' signals onto the valid sample
+     points; L1 between |MF| and the target volume sampled there
+     (trilinear), plus the eikonal term (rf_rendering.py:645-650);
+  5. backward (matched-filter adjoint, tracer adjoint, MLP double-backward)
+     and an AdamW step.
+
+``form="fused"`` is the MI355X path (BF16 MLPs on MFMA via hipBLASLt, the
+hand-written tracer kernels); ``form="fp32_autograd"`` is the all-FP32
+autograd form of the same step: FP32 MLPs and the compositing + signal as
+torch FP32 ops (the reference's glue style), with the same matched filter.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from geraf.models.networks.sdf_network import (ReflectivePowerNetwork, SDFNetwork,
+                                                SingleVarianceNetwork)
+from geraf.ops.RFMatchedFilter import matched_filter_cuda
+from geraf.ops.RFSignalTracer.lensless_tracer import lensless_signal_tracer
+
+from . import tracer_scenes as ts
+from .mf_scenes import PATH_BIAS, SPEED_OF_LIGHT
+
+CENTER = (0.0, 0.0, 0.30)
+RADIUS = 0.3
+
+
+def torch_lensless(sdf, grad, dists, z, o, d, inv_s, power, refl, n_valid, K, fs, As_sci, f0,
+                   anneal=1.0):
+    """The per-ray compositing + signal as plain torch ops (FP32, autograd)."""
+    R, S = sdf.shape
+    true_cos = (d[:, None, :] * grad).sum(-1)
+    iter_cos = -(F.relu(-true_cos * 0.5 + 0.5) * (1.0 - anneal) + F.relu(-true_cos) * anneal)
+    nxt = sdf + iter_cos * dists * 0.5
+    prv = sdf - iter_cos * dists * 0.5
+    pc = torch.sigmoid(prv * inv_s)
+    nc = torch.sigmoid(nxt * inv_s)
+    alpha = ((pc - nc + 1e-5) / (pc + 1e-5)).clip(0.0, 1.0)
+    trans = torch.cumprod(torch.cat([torch.ones_like(alpha[:, :1]), 1.0 - alpha + 1e-7], -1),
+                          -1)[:, :-1]
+    sigma = torch.clamp(refl * power * trans, min=1e-6) * (alpha * trans)
+    n = F.normalize(grad, dim=-1)
+    pts = o[:, None, :] + z[..., None] * d[:, None, :]
+    v = pts - o[:, None, :]
+    inc = v / (v.norm(dim=-1, keepdim=True) + 1e-7)
+    idn = (inc * n).sum(-1)
+    rr = inc - 2.0 * idn[..., None] * n
+    rd = rr / (rr.norm(dim=-1, keepdim=True) + 1e-7)
+    spec = (-inc * rd).sum(-1)
+    valid = torch.arange(S, device=sdf.device)[None, :] < n_valid[:, None]
+    live = valid & (spec >= 1e-6) & (idn <= 0)
+    dpath = 2.0 * v.norm(dim=-1)
+    amp = torch.where(live, sigma * spec / dpath ** 2, torch.zeros_like(sigma))
+    tau = (dpath + PATH_BIAS) / SPEED_OF_LIGHT
+    k = torch.arange(K, device=sdf.device, dtype=sdf.dtype)
+    ph = 2.0 * math.pi * (f0 + As_sci * k / fs)[None, None, :] * tau[..., None]
+    out_r = (amp[..., None] * torch.cos(ph)).sum(1) / n_valid[:, None]
+    out_i = (amp[..., None] * torch.sin(ph)).sum(1) / n_valid[:, None]
+    return out_r, out_i
+
+
+@dataclass
+class StageConfig:
+    rays: int
+    samples: int
+    K: int
+    min_valid: int
+    consistency_points: int = 0
+
+
+STAGE1 = StageConfig(rays=8192, samples=64, K=64, min_valid=16)
+STAGE2 = StageConfig(rays=32768, samples=256, K=512, min_valid=256, consistency_points=262144)
+
+
+class StageStep:
+    def __init__(self, cfg: StageConfig, target_volume: torch.Tensor, volume_axes,
+                 device: torch.device, form: str = "fused", seed: int = 0,
+                 mlp_dtype: Optional[torch.dtype] = "auto"):
+        """``mlp_dtype``: compute dtype of the MLP GEMMs; "auto" is BF16 for the
+        fused form and FP32 for the autograd form."""
+        if form not in ("fused", "fp32_autograd"):
+            raise ValueError(f"unknown step form {form!r}")
+        self.cfg, self.form, self.dev = cfg, form, device
+        torch.manual_seed(seed)
+        if mlp_dtype == "auto":
+            mlp_dtype = torch.bfloat16 if form == "fused" else None
+        cd = mlp_dtype
+        self.sdf_net = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6,
+                                  compute_dtype=cd).to(device)
+        self.var_net = SingleVarianceNetwork(0.3).to(device)
+        self.sig_net = ReflectivePowerNetwork(256, light_power=0.0, compute_dtype=cd).to(device)
+        params = list(self.sdf_net.parameters()) + list(self.var_net.parameters()) + \
+            list(self.sig_net.parameters())
+        self.opt = torch.optim.AdamW(params, lr=5e-4)
+        uneven = None if cfg.min_valid >= cfg.samples else (cfg.min_valid, cfg.samples)
+        b = ts.rays(cfg.rays, cfg.samples, cfg.K, seed=seed, uneven=uneven).to(device)
+        if b.n_valid is None:
+            b.n_valid = torch.full((cfg.rays,), cfg.samples, dtype=torch.int32, device=device)
+        self.b = b
+        center = torch.tensor(CENTER, device=device)
+        self.pts = b.o[:, None, :] + b.z[..., None] * b.d[:, None, :]           # [R,S,3] m
+        self.pts_n = ((self.pts - center) / RADIUS).reshape(-1, 3)              # SDF frame
+        self.valid = (torch.arange(cfg.samples, device=device)[None, :] <
+                      b.n_valid[:, None].long())
+        self.tgt = self.pts[self.valid].contiguous()                           # [N,3]
+        # target |MF| at the valid sample points: trilinear in the volume
+        # (grid_sample axis order (z, y, x), align_corners=True, as loading.py:20-25)
+        lo = torch.tensor([float(a[0]) for a in volume_axes], device=device)
+        hi = torch.tensor([float(a[-1]) for a in volume_axes], device=device)
+        g = (self.tgt - lo) / (hi - lo) * 2.0 - 1.0
+        vol = target_volume.abs().float()[None, None]                          # [1,1,X,Y,Z]
+        samp = F.grid_sample(vol, g[:, [2, 1, 0]].view(1, -1, 1, 1, 3), align_corners=True)
+        self.gt = samp.view(-1)
+        self.gt = self.gt / self.gt.max().clamp(min=1e-12)
+        if cfg.consistency_points:
+            gen = torch.Generator(device="cpu").manual_seed(seed + 1)
+            self.cons_pts = (torch.rand(cfg.consistency_points, 3, generator=gen) * 2 - 1) \
+                .to(device) * 0.6
+        self.radar = b.radar
+
+    def loss(self) -> torch.Tensor:
+        cfg, b = self.cfg, self.b
+        R, S = cfg.rays, cfg.samples
+        out, grad = self.sdf_net.sdf_and_gradient(self.pts_n.clone())
+        sdf = out[:, 0].reshape(R, S)
+        feat = out[:, 1:]
+        grad = grad.reshape(R, S, 3)
+        inv_s = self.var_net.inv_s().clip(1e-6, 1e6)
+        refl = self.sig_net.reflectance(self.pts_n, feat).reshape(R, S)
+        power = self.sig_net.power()
+        r = self.radar
+        if self.form == "fused":
+            sim_r, sim_i = lensless_signal_tracer(sdf, grad, b.dists, b.z, b.o, b.d, inv_s, power,
+                                                  reflectance=refl, n_valid=b.n_valid,
+                                                  **r)
+        else:
+            sim_r, sim_i = torch_lensless(sdf, grad, b.dists, b.z, b.o, b.d, inv_s, power, refl,
+                                          b.n_valid.float(), r["numADCSample"],
+                                          r["adcSampleRate"], r["As_sci"], r["fc_start"])
+        mf_r, mf_i = matched_filter_cuda(sim_r, sim_i, b.o, b.o, self.tgt, R, r["numADCSample"],
+                                         r["adcSampleRate"], r["As_sci"], self.tgt.shape[0])
+        pred = torch.hypot(mf_r, mf_i)
+        pred = pred / pred.detach().max().clamp(min=1e-12)
+        loss = (pred - self.gt).abs().mean()
+        eik = ((grad.norm(dim=-1) - 1.0) ** 2)[self.valid].mean()
+        loss = loss + 0.1 * eik
+        if self.cfg.consistency_points:
+            # vision-SDF consistency vs an analytic box SDF (half side 0.16/0.3/2)
+            h = 0.08 / RADIUS
+            q = self.cons_pts.abs() - h
+            box = q.clamp(min=0).norm(dim=-1) + q.max(dim=-1).values.clamp(max=0)
+            loss = loss + (self.sdf_net.sdf(self.cons_pts)[:, 0] - box).abs().mean()
+        return loss
+
+    def step(self) -> float:
+        self.opt.zero_grad(set_to_none=True)
+        loss = self.loss()
+        loss.backward()
+        self.opt.step()
+        return float(loss.detach())

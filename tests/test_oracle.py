@@ -332,3 +332,25 @@ def test_range_bin_mf_known_answer_and_adjoint():
     # pairs beyond spacing * R are skipped
     far = orc.mf_fft_forward(s.real, s.imag, tx, rx, v + [0, 0, 5.0], spacing, **prm)
     assert np.all(far == 0)
+
+
+def test_stage_step_torch_form_matches_oracle():
+    # the all-FP32 autograd form of the training-step benchmark restates the same
+    # composition; in float64 it must equal the oracle
+    from rf_workloads.stage_step import torch_lensless
+
+    b = ts.rays(9, 30, 16, seed=13, uneven=(5, 30))
+    r = b.radar
+    f64 = lambda t: t.double()  # noqa: E731
+    pts = L.points_f32(b.o, b.z, b.d)
+    ref = L.lensless_forward(f64(b.sdf), f64(b.grad), f64(b.dists), pts, f64(b.o), f64(b.d),
+                             f64(b.inv_s), f64(b.power), r["numADCSample"], r["adcSampleRate"],
+                             r["As_sci"], r["fc_start"], ts.SPEED_OF_LIGHT, ts.PATH_BIAS,
+                             refl=f64(b.refl), n_valid=b.n_valid.long())
+    o_r, o_i = torch_lensless(f64(b.sdf), f64(b.grad), f64(b.dists), f64(b.z), f64(b.o),
+                              f64(b.d), f64(b.inv_s), f64(b.power), f64(b.refl),
+                              b.n_valid.double(), r["numADCSample"], r["adcSampleRate"],
+                              r["As_sci"], r["fc_start"])
+    out = (o_r + 1j * o_i).numpy()
+    # points are formed in float64 here and in float32 by the oracle
+    assert np.linalg.norm(out - ref.numpy()) <= 1e-4 * np.linalg.norm(ref.numpy())
