@@ -16,20 +16,29 @@
 //                                         (SignalTracer_cuda_kernel.cu:351-450, one antenna)
 //
 // Layout: one wave per ray, four rays per block; samples are streamed in
-// 64-sample tiles (one sample per lane, coalesced loads).  The transmittance
-// is an exclusive prefix product over the tile (wave shuffles) times the carry
-// of the earlier tiles.
-// Forward signal: each tile's per-sample (amplitude, reduced carrier phase,
-// per-sample phase step) goes to LDS; lane (chunk j, group g) then takes the
-// tile's samples s = g (mod G), seeds exp(j*2*pi*(f0 + 32 j dfs) tau_s) from
-// the FP64-reduced phase and rotates it through its 32 frequencies into 32
-// private complex accumulators; the G groups are summed by shuffles at the end.
-// Backward: a forward pass keeps only the transmittance carry at each tile
-// start; a reverse pass over the tiles recomputes everything, evaluates
-// h_s = Re sum_k g[r,k] exp(-j phi_sk) by two-level Horner (gradient row read
-// through the scalar cache), and back-propagates through the transmittance
-// with a reverse affine scan W_s = gT_s + t_s W_{s+1}, so that
-// dL/dalpha_i = (direct) - T_i W_{i+1} needs no division by (1 - alpha + 1e-7).
+// 64-sample tiles (one sample per lane; per-ray base addresses live in SGPRs,
+// lanes add 32-bit offsets).  The transmittance is an exclusive prefix product
+// over the tile (DPP row shifts and broadcasts) times the carry of the earlier
+// tiles.  Divisions, square roots and the sigmoid's exponential use the
+// hardware approximations (about 1 ulp); the round-trip distance and the
+// carrier phase are formed in FP64.  The antenna is monostatic, so the return
+// direction is exactly the negated incident direction.
+//
+// Forward signal: emitting samples (valid, not culled, amp != 0) are appended
+// to a per-wave LDS queue as (amplitude, reduced carrier phase, per-frequency
+// phase step in FP64 cycles, step phasor).  The queue is drained in batches
+// of 64 (and once at the end): lane (chunk j, group g) takes the entries
+// i = g (mod G), seeds exp(j*2*pi*(f0 + 32 j dfs) tau_i) from the FP64-reduced
+// phase and rotates it through its 32 frequencies into 32 private complex
+// accumulators; the G groups are summed with DPP at the end.
+// Backward: pass A walks the tiles forward, keeps the transmittance carry at
+// each tile start, and queues (tau, sample) of every emitting sample; full
+// 64-lane batches evaluate h_s = Re sum_k g[r,k] exp(-j phi_sk) by two-level
+// Horner (gradient row read through the scalar cache).  Pass B walks the
+// tiles backward, recomputes the compositing and geometry, and propagates
+// through the transmittance with a reverse affine scan
+// W_s = gT_s + t_s W_{s+1}, so dL/dalpha_i = (direct) - T_i W_{i+1} needs no
+// division by (1 - alpha + 1e-7).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -44,6 +53,7 @@ constexpr int kRaysPerBlock = 4;
 constexpr int kTile = 64;
 constexpr int kMaxTiles = 32;  // S <= 2048
 constexpr int kSigChunk = 32;
+constexpr int kQueue = 2 * kTile;
 
 struct LenslessArgs {
   const float* sdf;           // [R,S]
@@ -67,94 +77,181 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+// Element `i` of a row whose base is wave-uniform (SGPR base + 32-bit offset).
+__device__ __forceinline__ float ld_row(const float* row, unsigned i) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(row) + i * 4u);
+}
+__device__ __forceinline__ V3 ld_row3(const float* row, unsigned i) {
+  const float* p = reinterpret_cast<const float*>(reinterpret_cast<const char*>(row) + i * 12u);
+  return v3(p[0], p[1], p[2]);
+}
+__device__ __forceinline__ void st_row(float* row, unsigned i, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(row) + i * 4u) = v;
+}
 
-struct SampleState {
-  // inputs
+// One ray's wave-uniform state.
+struct Ray {
+  size_t row;        // r * S
+  int nv;            // valid samples
+  V3 o, dir;
+  double ox, oy, oz;
+  float inv_s, power;
+};
+
+__device__ __forceinline__ Ray load_ray(const LenslessArgs& a, int r) {
+  Ray y;
+  y.row = (size_t)r * a.S;
+  y.nv = a.n_valid ? min(max(a.n_valid[r], 0), a.S) : a.S;
+  y.o = load3(a.origin + 3 * (size_t)r);
+  y.dir = load3(a.dirs + 3 * (size_t)r);
+  y.ox = y.o.x;
+  y.oy = y.o.y;
+  y.oz = y.o.z;
+  y.inv_s = a.inv_s[0];
+  y.power = a.power[0];
+  return y;
+}
+
+// Per-sample inputs and compositing terms.
+struct Sample {
   float sdf, dist, refl, off, zz;
   V3 g;
-  // compositing
-  float c, ic, prev, next, pc, nc, alpha, t;
+  float c, prev, next, pc, nc, alpha, t;
   bool qmask;
 };
 
-__device__ __forceinline__ void load_and_composite(const LenslessArgs& a, size_t idx, V3 dir,
-                                                   float inv_s, SampleState& x) {
-  x.sdf = a.sdf[idx];
-  x.dist = a.dists[idx];
-  x.refl = a.refl ? a.refl[idx] : 1.0f;
-  x.off = a.offset ? a.offset[idx] : 0.0f;
-  x.zz = a.z[idx];
-  x.g = load3(a.grad + 3 * idx);
-  x.c = dot(dir, x.g);
-  x.ic = -(fmaxf(-x.c * 0.5f + 0.5f, 0.0f) * (1.0f - a.anneal) + fmaxf(-x.c, 0.0f) * a.anneal);
-  const float half = x.ic * x.dist * 0.5f;
+__device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ray& y, int s) {
+  Sample x;
+  const unsigned u = (unsigned)s;
+  x.sdf = ld_row(a.sdf + y.row, u);
+  x.dist = ld_row(a.dists + y.row, u);
+  x.zz = ld_row(a.z + y.row, u);
+  x.refl = a.refl ? ld_row(a.refl + y.row, u) : 1.0f;
+  x.off = a.offset ? ld_row(a.offset + y.row, u) : 0.0f;
+  x.g = ld_row3(a.grad + 3 * y.row, u);
+  x.c = dot(y.dir, x.g);
+  const float ic =
+      -(fmaxf(-x.c * 0.5f + 0.5f, 0.0f) * (1.0f - a.anneal) + fmaxf(-x.c, 0.0f) * a.anneal);
+  const float half = ic * x.dist * 0.5f;
   x.next = x.sdf + half;
   x.prev = x.sdf - half;
-  x.pc = sigmoidf(x.prev * inv_s);
-  x.nc = sigmoidf(x.next * inv_s);
-  const float q = (x.pc - x.nc + 1.0e-5f) / (x.pc + 1.0e-5f);
+  x.pc = sigmoid_fast(x.prev * y.inv_s);
+  x.nc = sigmoid_fast(x.next * y.inv_s);
+  const float q = (x.pc - x.nc + 1.0e-5f) * rcp_fast(x.pc + 1.0e-5f);
   x.qmask = (q >= 0.0f) && (q <= 1.0f);
   x.alpha = fminf(fmaxf(q, 0.0f), 1.0f);
   x.t = 1.0f - x.alpha + 1.0e-7f;
+  return x;
 }
 
-// Number of set bits of `mask` below `lane`.
+// Specular geometry of the monostatic sample (ComputeSpecularGeometry,
+// SignalTracer_cuda_kernel.cu:172-197, with tx = rx = o; ret = -inc exactly).
+struct Geo {
+  V3 n;             // normalised normal
+  float gl;         // |g|
+  V3 inc;           // normalised incident direction
+  V3 refl;          // normalised reflected direction
+  float refl_norm;  // |inc - 2 (inc.n) n|
+  float idn;        // inc . n
+  float spec;       // ret . refl
+  double dpath;     // round-trip distance (no bias)
+  bool live;        // valid and not culled
+};
+
+__device__ __forceinline__ Geo geometry(const Ray& y, const Sample& x, bool valid) {
+  Geo e;
+  e.gl = sqrt_fast(dot(x.g, x.g));
+  e.n = rcp_fast(fmaxf(e.gl, 1.0e-12f)) * x.g;
+  // point = o + z*d with separately rounded product and sum (as a tensor op would)
+  const V3 zd = v3(__fmul_rn(x.zz, y.dir.x), __fmul_rn(x.zz, y.dir.y), __fmul_rn(x.zz, y.dir.z));
+  const V3 p = v3(__fadd_rn(y.o.x, zd.x), __fadd_rn(y.o.y, zd.y), __fadd_rn(y.o.z, zd.z));
+  const V3 v = p - y.o;
+  e.inc = rcp_fast(sqrt_fast(dot(v, v)) + 1.0e-7f) * v;
+  e.idn = dot(e.inc, e.n);
+  const V3 rr = e.inc - (2.0f * e.idn) * e.n;
+  e.refl_norm = sqrt_fast(dot(rr, rr));
+  e.refl = rcp_fast(e.refl_norm + 1.0e-7f) * rr;
+  e.spec = -dot(e.inc, e.refl);
+  const double dx = (double)p.x - y.ox, dy = (double)p.y - y.oy, dz = (double)p.z - y.oz;
+  e.dpath = 2.0 * sqrt_f64_newton(fma(dx, dx, fma(dy, dy, dz * dz)));
+  e.live = valid && !(e.spec < 1.0e-6f || e.idn > 0.0f);
+  return e;
+}
+
+// d spec / d n (see dspec_dnormal) with ret = -inc.
+__device__ __forceinline__ V3 dspec_dn_mono(const Geo& e) {
+  const V3 q = v3(-e.inc.x, -e.inc.y, -e.inc.z) - e.spec * e.refl;
+  const float s = -2.0f * rcp_fast(e.refl_norm + 1.0e-7f);
+  return s * (dot(q, e.n) * e.inc + e.idn * q);
+}
+
+// Number of set bits of `mask` below this lane.
 __device__ __forceinline__ int lane_prefix(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-// Inclusive prefix product over the wave.
-__device__ __forceinline__ float wave_incl_prod(float v, int lane) {
+// Sums each of the N values over the lanes of every aligned block of `groups`
+// lanes (every lane of a block gets the block totals).
+template <int N>
+__device__ __forceinline__ void group_sum(float* v, int groups) {
+  if (groups >= 2)
 #pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const float y = __shfl_up(v, off, kWave);
-    if (lane >= off) v *= y;
-  }
-  return v;
-}
-
-struct Emission {
-  V3 n;          // normalised normal
-  float gl;      // |g|
-  SpecGeom geo;
-  double dpath;  // round-trip distance (no bias)
-  bool live;     // valid and not culled
-};
-
-__device__ __forceinline__ Emission emission(V3 o, V3 dir, const SampleState& x, bool valid) {
-  Emission e;
-  e.gl = sqrtf(dot(x.g, x.g));
-  e.n = (1.0f / fmaxf(e.gl, 1.0e-12f)) * x.g;
-  // point = o + z*d with separately rounded product and sum (as a tensor op would)
-  const V3 zd = v3(__fmul_rn(x.zz, dir.x), __fmul_rn(x.zz, dir.y), __fmul_rn(x.zz, dir.z));
-  const V3 p = v3(__fadd_rn(o.x, zd.x), __fadd_rn(o.y, zd.y), __fadd_rn(o.z, zd.z));
-  e.geo = spec_geom(o, o, e.n, p);
-  e.dpath = 2.0 * dist3_f64(o, p);
-  e.live = valid && !e.geo.culled();
-  return e;
+    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kQuadXor1>(0.0f, v[j]);
+  if (groups >= 4)
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kQuadXor2>(0.0f, v[j]);
+  if (groups >= 8)
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kRowHalfMirror>(0.0f, v[j]);
+  if (groups >= 16)
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kRowMirror>(0.0f, v[j]);
+  if (groups >= 32)
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], 16, kWave);
+  if (groups >= 64)
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], 32, kWave);
 }
 
 // ------------------------------------------------------------------ forward
 
+struct FwdQueue {
+  float amp[kRaysPerBlock][kQueue];
+  float fr0[kRaysPerBlock][kQueue];
+  float2 w[kRaysPerBlock][kQueue];
+  double cyc1[kRaysPerBlock][kQueue];
+};
+
+// Adds entries [0, n) of this wave's queue to the lane's 32 frequencies.
+__device__ __forceinline__ void synth(const FwdQueue& q, int wave, int n, int grp, int groups,
+                                      int k0, float* acc_r, float* acc_i) {
+  for (int i = grp; i < n; i += groups) {
+    const float am = q.amp[wave][i];
+    const float2 w = q.w[wave][i];
+    const double c1 = q.cyc1[wave][i];
+    const float2 seed =
+        phasor_cycles((float)centered_frac(fma((double)k0, c1, (double)q.fr0[wave][i])), 1.0f);
+    float2 zc = make_float2(am * seed.x, am * seed.y);
+#pragma unroll
+    for (int j = 0; j < kSigChunk; ++j) {
+      acc_r[j] += zc.x;
+      acc_i[j] += zc.y;
+      zc = cmul(zc, w);
+    }
+  }
+}
+
 __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
     LenslessArgs a, float* __restrict__ out_real, float* __restrict__ out_imag,
     float* __restrict__ sigma_out, PhaseConsts pc, int groups, int chunks) {
-  __shared__ float s_amp[kRaysPerBlock][kTile];
-  __shared__ float s_fr0[kRaysPerBlock][kTile];
-  __shared__ double s_cyc1[kRaysPerBlock][kTile];
-
+  __shared__ FwdQueue q;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = blockIdx.x * kRaysPerBlock + wave;
-  const bool ray_ok = r < a.R;
-  const int rr = ray_ok ? r : a.R - 1;
-  const int nv = a.n_valid ? min(max(a.n_valid[rr], 0), a.S) : a.S;
-  const V3 o = load3(a.origin + 3 * (size_t)rr);
-  const V3 dir = load3(a.dirs + 3 * (size_t)rr);
-  const float inv_s = a.inv_s[0];
-  const float power = a.power[0];
+  if (r >= a.R) return;  // only wave-level synchronisation below
+  const Ray y = load_ray(a, r);
 
   const int grp = lane % groups;
   const int chunk = lane / groups;
@@ -166,94 +263,88 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
   for (int j = 0; j < kSigChunk; ++j) acc_r[j] = acc_i[j] = 0.0f;
 
   float carry = 1.0f;
-  for (int base = 0; base < nv; base += kTile) {
+  int queued = 0;
+  for (int base = 0; base < y.nv; base += kTile) {
     const int s = base + lane;
-    const bool valid = s < nv;
-    const size_t idx = (size_t)rr * a.S + (valid ? s : nv - 1);
-    SampleState x;
-    load_and_composite(a, idx, dir, inv_s, x);
-    const float tt = valid ? x.t : 1.0f;
-    const float incl = wave_incl_prod(tt, lane);
-    float excl = __shfl_up(incl, 1, kWave);
-    if (lane == 0) excl = 1.0f;
-    const float T = carry * excl;
-    carry *= __shfl(incl, kWave - 1, kWave);
+    const bool valid = s < y.nv;
+    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
+    const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
+    const float T = carry * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
+    carry *= readlane(incl, kWave - 1);
     const float Tc = T + x.off;
-    const float sigma = fmaxf(x.refl * power * Tc, 1.0e-6f) * (x.alpha * Tc);
-    const Emission e = emission(o, dir, x, valid);
-    float amp = 0.0f, fr0 = 0.0f;
-    double cyc1 = 0.0;
-    if (e.live) {
-      amp = sigma * e.geo.spec / (float)(e.dpath * e.dpath);
-      const double tau = (e.dpath + pc.bias) * pc.inv_c;
-      fr0 = (float)centered_frac(pc.f0 * tau);
-      cyc1 = pc.dfs * tau;
-    }
-    if (sigma_out && valid && ray_ok) sigma_out[idx] = sigma;
-    // compact the tile's emitting samples to the front of the LDS slots, so
-    // the frequency loop below only visits them
+    const float sigma = fmaxf(x.refl * y.power * Tc, 1.0e-6f) * (x.alpha * Tc);
+    if (sigma_out && valid) st_row(sigma_out + y.row, (unsigned)s, sigma);
+    const Geo e = geometry(y, x, valid);
+    float amp = 0.0f;
+    if (e.live) amp = sigma * e.spec * rcp_fast((float)(e.dpath * e.dpath));
     const bool emit = e.live && amp != 0.0f;
     const unsigned long long mask = __ballot(emit);
-    const int slot = lane_prefix(mask);
     if (emit) {
-      s_amp[wave][slot] = amp;
-      s_fr0[wave][slot] = fr0;
-      s_cyc1[wave][slot] = cyc1;
+      const int slot = queued + lane_prefix(mask);
+      const double tau = (e.dpath + pc.bias) * pc.inv_c;
+      const double c1 = pc.dfs * tau;
+      q.amp[wave][slot] = amp;
+      q.fr0[wave][slot] = (float)centered_frac(pc.f0 * tau);
+      q.cyc1[wave][slot] = c1;
+      q.w[wave][slot] = phasor_cycles((float)centered_frac(c1), 1.0f);
     }
-    const int cnt = __popcll(mask);
+    queued += __popcll(mask);
     wave_sync();
-    if (chunk_ok) {
-      for (int i = grp; i < cnt; i += groups) {
-        const float am = s_amp[wave][i];
-        const double c1 = s_cyc1[wave][i];
-        const float2 seed = phasor_cycles(
-            (float)centered_frac((double)s_fr0[wave][i] + (double)k0 * c1), 1.0f);
-        const float2 w = phasor_cycles((float)centered_frac(c1), 1.0f);
-        float2 zc = make_float2(am * seed.x, am * seed.y);
-#pragma unroll
-        for (int j = 0; j < kSigChunk; ++j) {
-          acc_r[j] += zc.x;
-          acc_i[j] += zc.y;
-          zc = cmul(zc, w);
-        }
+    if (queued >= kTile) {
+      if (chunk_ok) synth(q, wave, kTile, grp, groups, k0, acc_r, acc_i);
+      const int rest = queued - kTile;
+      float am = 0.0f, f0 = 0.0f;
+      float2 w = make_float2(0.0f, 0.0f);
+      double c1 = 0.0;
+      if (lane < rest) {
+        am = q.amp[wave][kTile + lane];
+        f0 = q.fr0[wave][kTile + lane];
+        w = q.w[wave][kTile + lane];
+        c1 = q.cyc1[wave][kTile + lane];
       }
+      wave_sync();
+      if (lane < rest) {
+        q.amp[wave][lane] = am;
+        q.fr0[wave][lane] = f0;
+        q.w[wave][lane] = w;
+        q.cyc1[wave][lane] = c1;
+      }
+      queued = rest;
+      wave_sync();
     }
-    wave_sync();
   }
-  // sum the sample groups (lanes of one chunk differ in the low bits)
-  for (int off = 1; off < groups; off <<= 1) {
-#pragma unroll
-    for (int j = 0; j < kSigChunk; ++j) {
-      acc_r[j] += __shfl_xor(acc_r[j], off, kWave);
-      acc_i[j] += __shfl_xor(acc_i[j], off, kWave);
-    }
-  }
-  if (ray_ok && chunk_ok && grp == 0) {
-    const float inv_n = nv > 0 ? 1.0f / (float)nv : 0.0f;
+  if (queued > 0 && chunk_ok) synth(q, wave, queued, grp, groups, k0, acc_r, acc_i);
+
+  group_sum<kSigChunk>(acc_r, groups);
+  group_sum<kSigChunk>(acc_i, groups);
+  if (chunk_ok && grp == 0) {
+    const float inv_n = y.nv > 0 ? 1.0f / (float)y.nv : 0.0f;
+    float* orow = out_real + (size_t)r * a.K;
+    float* irow = out_imag + (size_t)r * a.K;
 #pragma unroll
     for (int j = 0; j < kSigChunk; ++j) {
       const int k = k0 + j;
       if (k < a.K) {
-        out_real[(size_t)rr * a.K + k] = acc_r[j] * inv_n;
-        out_imag[(size_t)rr * a.K + k] = acc_i[j] * inv_n;
+        st_row(orow, (unsigned)k, acc_r[j] * inv_n);
+        st_row(irow, (unsigned)k, acc_i[j] * inv_n);
       }
     }
   }
-  if (sigma_out && ray_ok)
-    for (int s = nv + lane; s < a.S; s += kWave) sigma_out[(size_t)rr * a.S + s] = 0.0f;
+  if (sigma_out)
+    for (int s = y.nv + lane; s < a.S; s += kWave) st_row(sigma_out + y.row, (unsigned)s, 0.0f);
 }
 
 // ------------------------------------------------------------------ backward
 
 // Per-wave LDS layout of the backward kernel.
 struct BwdLds {
-  double* btau;  // [2*kTile] tau of queued emitting samples
-  int* bsid;     // [2*kTile] their sample indices
+  double* btau;  // [kQueue] tau of queued emitting samples
+  int* bsid;     // [kQueue] their sample indices
   float* carry;  // [kMaxTiles] transmittance at each tile start
   float* h;      // [S] h_s (0 for samples that do not emit)
 };
 __host__ __device__ constexpr size_t bwd_lds_bytes_per_wave(int S) {
-  return 2 * kTile * sizeof(double) + 2 * kTile * sizeof(int) + kMaxTiles * sizeof(float) +
+  return kQueue * sizeof(double) + kQueue * sizeof(int) + kMaxTiles * sizeof(float) +
          (size_t)S * sizeof(float);
 }
 
@@ -264,26 +355,21 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
   extern __shared__ double lds_raw[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = blockIdx.x * kRaysPerBlock + wave;
+  if (r >= a.R) return;  // only wave-level synchronisation below
   unsigned char* base_ptr =
       reinterpret_cast<unsigned char*>(lds_raw) + (size_t)wave * bwd_lds_bytes_per_wave(a.S);
   BwdLds L;
   L.btau = reinterpret_cast<double*>(base_ptr);
-  L.bsid = reinterpret_cast<int*>(L.btau + 2 * kTile);
-  L.carry = reinterpret_cast<float*>(L.bsid + 2 * kTile);
+  L.bsid = reinterpret_cast<int*>(L.btau + kQueue);
+  L.carry = reinterpret_cast<float*>(L.bsid + kQueue);
   L.h = L.carry + kMaxTiles;
 
-  const int r = blockIdx.x * kRaysPerBlock + wave;
-  const bool ray_ok = r < a.R;
-  const int rr = ray_ok ? r : a.R - 1;
-  const int nv = a.n_valid ? min(max(a.n_valid[rr], 0), a.S) : a.S;
-  const V3 o = load3(a.origin + 3 * (size_t)rr);
-  const V3 dir = load3(a.dirs + 3 * (size_t)rr);
-  const float inv_s = a.inv_s[0];
-  const float power = a.power[0];
-  const float inv_n = nv > 0 ? 1.0f / (float)nv : 0.0f;
-  const int ntiles = (nv + kTile - 1) / kTile;
-  const float* gr_row = g_real + (size_t)rr * a.K;
-  const float* gi_row = g_imag + (size_t)rr * a.K;
+  const Ray y = load_ray(a, r);
+  const float inv_n = y.nv > 0 ? 1.0f / (float)y.nv : 0.0f;
+  const int ntiles = (y.nv + kTile - 1) / kTile;
+  const float* gr_row = g_real + (size_t)r * a.K;
+  const float* gi_row = g_imag + (size_t)r * a.K;
 
   // h_s = (1/n) Re sum_k g[r,k] exp(-j phi_sk) for the first `cnt` queued
   // samples, one per lane (the gradient row is wave-uniform: scalar loads).
@@ -297,19 +383,18 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
 
   // pass A (forward over tiles): transmittance carries, and h_s of every
   // emitting sample, evaluated in full 64-lane batches of such samples
-  for (int s = lane; s < nv; s += kWave) L.h[s] = 0.0f;
+  for (int s = lane; s < y.nv; s += kWave) L.h[s] = 0.0f;
   wave_sync();
   float carry = 1.0f;
   int queued = 0;
   for (int tile = 0; tile < ntiles; ++tile) {
     if (lane == 0) L.carry[tile] = carry;
     const int s = tile * kTile + lane;
-    const bool valid = s < nv;
-    SampleState x;
-    load_and_composite(a, (size_t)rr * a.S + (valid ? s : nv - 1), dir, inv_s, x);
-    const float incl = wave_incl_prod(valid ? x.t : 1.0f, lane);
-    carry *= __shfl(incl, kWave - 1, kWave);
-    const Emission e = emission(o, dir, x, valid);
+    const bool valid = s < y.nv;
+    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
+    const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
+    carry *= readlane(incl, kWave - 1);
+    const Geo e = geometry(y, x, valid);
     const unsigned long long mask = __ballot(e.live);
     if (e.live) {
       const int slot = queued + lane_prefix(mask);
@@ -342,36 +427,35 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
   // pass B: reverse over tiles
   float w_carry = 0.0f;  // W at the first sample of the next tile
   float gis = 0.0f, gpow = 0.0f;
+  float* gs_row = grad_sdf + y.row;
+  float* gg_row = grad_g + 3 * y.row;
+  float* gr_out = grad_refl ? grad_refl + y.row : nullptr;
   for (int tile = ntiles - 1; tile >= 0; --tile) {
     const int s = tile * kTile + lane;
-    const bool valid = s < nv;
-    const size_t idx = (size_t)rr * a.S + (valid ? s : nv - 1);
-    SampleState x;
-    load_and_composite(a, idx, dir, inv_s, x);
+    const bool valid = s < y.nv;
+    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
     const float tt = valid ? x.t : 1.0f;
-    const float incl = wave_incl_prod(tt, lane);
-    float excl = __shfl_up(incl, 1, kWave);
-    if (lane == 0) excl = 1.0f;
-    const float T = L.carry[tile] * excl;
+    const float incl = wave_incl_prod_dpp(tt);
+    const float T = L.carry[tile] * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
     const float Tc = T + x.off;
-    const float rp = x.refl * power;
+    const float rp = x.refl * y.power;
     const float m = rp * Tc;
     const bool unclamped = m >= 1.0e-6f;
     const float aTc = x.alpha * Tc;
     const float sigma = fmaxf(m, 1.0e-6f) * aTc;
-    const Emission e = emission(o, dir, x, valid);
+    const Geo e = geometry(y, x, valid);
 
     float h = 0.0f, att = 0.0f;
     if (e.live) {
-      att = (float)(1.0 / (e.dpath * e.dpath));
+      att = rcp_fast((float)(e.dpath * e.dpath));
       h = L.h[s];
     }
-    const float gsig = h * e.geo.spec * att;
+    const float gsig = h * e.spec * att;
     const float gspec = h * sigma * att;
-    const V3 gnhat = e.live ? gspec * dspec_dnormal(e.geo, e.n) : v3(0.0f, 0.0f, 0.0f);
+    const V3 gnhat = e.live ? gspec * dspec_dn_mono(e) : v3(0.0f, 0.0f, 0.0f);
     float grefl, gTc, galpha;
     if (unclamped) {
-      grefl = gsig * power * Tc * aTc;
+      grefl = gsig * y.power * Tc * aTc;
       gpow += gsig * x.refl * Tc * aTc;
       gTc = gsig * 2.0f * m * x.alpha;
       galpha = gsig * m * Tc;
@@ -382,30 +466,19 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
     }
     if (!valid) gTc = 0.0f;
     // reverse affine scan: W_l = gT_l + t_l * W_{l+1}
-    float mm = tt, bb = gTc;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const float m2 = __shfl_down(mm, off, kWave);
-      const float b2 = __shfl_down(bb, off, kWave);
-      if (lane + off < kWave) {
-        bb = fmaf(mm, b2, bb);
-        mm *= m2;
-      }
-    }
-    const float W = fmaf(mm, w_carry, bb);
-    float w_next = __shfl_down(W, 1, kWave);
-    if (lane == kWave - 1) w_next = w_carry;
-    w_carry = __shfl(W, 0, kWave);
+    float w_next, w_head;
+    wave_affine_suffix(tt, gTc, w_carry, lane, &w_next, &w_head);
+    w_carry = w_head;
 
     galpha -= T * w_next;
     const float gq = (valid && x.qmask) ? galpha : 0.0f;
-    const float den = x.pc + 1.0e-5f;
-    const float gpc = gq * x.nc / (den * den);
-    const float gnc = -gq / den;
+    const float rden = rcp_fast(x.pc + 1.0e-5f);
+    const float gpc = gq * x.nc * (rden * rden);
+    const float gnc = -gq * rden;
     const float sp = x.pc * (1.0f - x.pc);
     const float sn = x.nc * (1.0f - x.nc);
-    const float gprev = gpc * inv_s * sp;
-    const float gnext = gnc * inv_s * sn;
+    const float gprev = gpc * y.inv_s * sp;
+    const float gnext = gnc * y.inv_s * sn;
     gis += gpc * x.prev * sp + gnc * x.next * sn;
     const float gsdf = gprev + gnext;
     const float gic = (gnext - gprev) * x.dist * 0.5f;
@@ -413,31 +486,32 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
                             (x.c < 0.0f ? a.anneal : 0.0f));
     V3 gg;
     if (e.gl > 1.0e-12f)
-      gg = (1.0f / e.gl) * (gnhat - dot(e.n, gnhat) * e.n);
+      gg = rcp_fast(e.gl) * (gnhat - dot(e.n, gnhat) * e.n);
     else
       gg = 1.0e12f * gnhat;
-    gg = gg + gc * dir;
-    if (valid && ray_ok) {
-      grad_sdf[idx] = gsdf;
-      grad_g[3 * idx + 0] = gg.x;
-      grad_g[3 * idx + 1] = gg.y;
-      grad_g[3 * idx + 2] = gg.z;
-      if (grad_refl) grad_refl[idx] = grefl;
+    gg = gg + gc * y.dir;
+    if (valid) {
+      const unsigned u = (unsigned)s;
+      st_row(gs_row, u, gsdf);
+      st_row(gg_row, 3u * u, gg.x);
+      st_row(gg_row, 3u * u + 1u, gg.y);
+      st_row(gg_row, 3u * u + 2u, gg.z);
+      if (gr_out) st_row(gr_out, u, grefl);
     }
   }
-  if (ray_ok) {
-    for (int s = nv + lane; s < a.S; s += kWave) {
-      const size_t idx = (size_t)rr * a.S + s;
-      grad_sdf[idx] = 0.0f;
-      grad_g[3 * idx] = grad_g[3 * idx + 1] = grad_g[3 * idx + 2] = 0.0f;
-      if (grad_refl) grad_refl[idx] = 0.0f;
-    }
+  for (int s = y.nv + lane; s < a.S; s += kWave) {
+    const unsigned u = (unsigned)s;
+    st_row(gs_row, u, 0.0f);
+    st_row(gg_row, 3u * u, 0.0f);
+    st_row(gg_row, 3u * u + 1u, 0.0f);
+    st_row(gg_row, 3u * u + 2u, 0.0f);
+    if (gr_out) st_row(gr_out, u, 0.0f);
   }
   gis = wave_sum(gis);
   gpow = wave_sum(gpow);
-  if (ray_ok && lane == 0) {
-    grad_inv_s[rr] = gis;
-    grad_power[rr] = gpow;
+  if (lane == 0) {
+    grad_inv_s[r] = gis;
+    grad_power[r] = gpow;
   }
 }
 

@@ -178,6 +178,86 @@ __device__ __forceinline__ double dist3_f64(V3 a, V3 b) {
   return dist3_f64((double)a.x, (double)a.y, (double)a.z, (double)b.x, (double)b.y, (double)b.z);
 }
 
+// ------------------------------------------------------------ cross-lane (DPP)
+
+// v_mov_b32_dpp: each enabled lane reads `src` from the lane the DPP control
+// names; lanes whose source lies outside the row/wave, or which the row/bank
+// masks disable, keep `old` (bound_ctrl off).
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ float dpp_mov(float old, float src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                               __builtin_bit_cast(int, src), CTRL,
+                                                               ROW_MASK, BANK_MASK, false));
+}
+namespace dpp {
+constexpr int kQuadXor1 = 0xB1;    // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;    // quad_perm [2,3,0,1]
+constexpr int kRowShl = 0x100;     // + n: lane l reads l+n within its row of 16
+constexpr int kRowShr = 0x110;     // + n: lane l reads l-n within its row of 16
+constexpr int kWaveShl1 = 0x130;   // lane l reads l+1 (lane 63 keeps old)
+constexpr int kWaveShr1 = 0x138;   // lane l reads l-1 (lane 0 keeps old)
+constexpr int kRowMirror = 0x140;  // lane l reads 15-l within its row
+constexpr int kRowHalfMirror = 0x141;  // lane l reads 7-l within its half-row
+constexpr int kRowBcast15 = 0x142;     // lane 15 of each row feeds the next row
+constexpr int kRowBcast31 = 0x143;     // lane 31 feeds rows 2 and 3
+}  // namespace dpp
+
+__device__ __forceinline__ float readlane(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// Inclusive prefix product over the 64 lanes (row shifts, then the row
+// broadcasts 15 and 31).
+__device__ __forceinline__ float wave_incl_prod_dpp(float v) {
+  float p = v * dpp_mov<dpp::kRowShr + 1>(1.0f, v);
+  p *= dpp_mov<dpp::kRowShr + 2>(1.0f, v);
+  p *= dpp_mov<dpp::kRowShr + 3>(1.0f, v);
+  p *= dpp_mov<dpp::kRowShr + 4, 0xf, 0xe>(1.0f, p);
+  p *= dpp_mov<dpp::kRowShr + 8, 0xf, 0xc>(1.0f, p);
+  p *= dpp_mov<dpp::kRowBcast15, 0xa, 0xf>(1.0f, p);
+  p *= dpp_mov<dpp::kRowBcast31, 0xc, 0xf>(1.0f, p);
+  return p;
+}
+
+// Suffix composition of the affine maps W_l = b_l + m_l * W_{l+1} over the
+// wave, with W_64 = tail.  Returns W_l; *w_next = W_{l+1} and *head = W_0.
+// Rows of 16 are scanned with row_shl steps; the four row results are joined
+// through readlane.
+__device__ __forceinline__ float wave_affine_suffix(float m, float b, float tail, int lane,
+                                                    float* w_next, float* head) {
+#define GERAF_AFFINE_STEP(N)                                  \
+  {                                                           \
+    const float m2 = dpp_mov<dpp::kRowShl + N>(1.0f, m);      \
+    const float b2 = dpp_mov<dpp::kRowShl + N>(0.0f, b);      \
+    b = fmaf(m, b2, b);                                       \
+    m *= m2;                                                  \
+  }
+  GERAF_AFFINE_STEP(1)
+  GERAF_AFFINE_STEP(2)
+  GERAF_AFFINE_STEP(4)
+  GERAF_AFFINE_STEP(8)
+#undef GERAF_AFFINE_STEP
+  const float w48 = fmaf(readlane(m, 48), tail, readlane(b, 48));
+  const float w32 = fmaf(readlane(m, 32), w48, readlane(b, 32));
+  const float w16 = fmaf(readlane(m, 16), w32, readlane(b, 16));
+  *head = fmaf(readlane(m, 0), w16, readlane(b, 0));
+  const int row = lane >> 4;
+  const float after = row == 3 ? tail : row == 2 ? w48 : row == 1 ? w32 : w16;
+  const float w = fmaf(m, after, b);
+  *w_next = dpp_mov<dpp::kWaveShl1>(tail, w);
+  return w;
+}
+
+// ------------------------------------------------------------ fast FP32 math
+// Hardware approximations (about 1 ulp each) used where the reference formula
+// is a plain division, square root or exponential.
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+// 1 / (1 + exp(-x)); saturates cleanly to 0 and 1.
+__device__ __forceinline__ float sigmoid_fast(float x) {
+  return rcp_fast(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * x));
+}
+
 // Sum over the 64 lanes of a wave (butterfly, every lane gets the total).
 __device__ __forceinline__ float wave_sum(float x) {
 #pragma unroll

@@ -52,11 +52,20 @@ def test_fused_step_matches_fp32_autograd_step(cuda):
     assert worst <= 2e-2
 
 
-def test_fused_bf16_step_decreases_loss(cuda):
+def test_fused_bf16_step_tracks_fp32_step(cuda):
+    """BF16 MLP GEMMs (FP32 accumulation, 8-bit mantissa operands) move the
+    loss by about 1% and keep the full gradient direction; a few AdamW steps
+    stay finite."""
     vol, axes = _target(cuda)
-    s = st.StageStep(SMALL, vol, axes, cuda, form="fused")
-    first = s.step()
-    for _ in range(20):
-        last = s.step()
-    print(f"loss {first:.5f} -> {last:.5f}", flush=True)
-    assert np.isfinite(last) and last < first
+    b = st.StageStep(SMALL, vol, axes, cuda, form="fused")
+    f = st.StageStep(SMALL, vol, axes, cuda, form="fused", mlp_dtype=None)
+    lb, gb = _grads(b)
+    lf, gf = _grads(f)
+    vb = torch.cat([g.flatten() for g in gb])
+    vf = torch.cat([g.flatten() for g in gf])
+    cos = float(torch.dot(vb, vf) / (vb.norm() * vf.norm()))
+    print(f"loss bf16 {lb:.6f} fp32 {lf:.6f}; gradient cosine {cos:.5f}", flush=True)
+    assert abs(lb - lf) <= 3e-2 * abs(lf)
+    assert cos >= 0.99
+    for _ in range(5):
+        assert np.isfinite(b.step())
