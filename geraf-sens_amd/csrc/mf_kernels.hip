@@ -263,47 +263,87 @@ __global__ __launch_bounds__(256) void mf_forward_f64_kernel(
 
 // ---------------------------------------------------------------- adjoint
 
-// Block = (antenna a, 32-sample chunk).  Each lane walks a strided subset of
-// the voxels, rotating its seeded phasor through the 32 samples of the chunk
-// and accumulating into 32 private complex registers; the block then reduces
-// them (wave shuffles, then LDS across waves) in a fixed order.
+// Block = (antenna a, a group of up to kAdjWaves 32-sample chunks).  The
+// voxels are streamed in tiles of one voxel per thread: first every thread
+// forms its voxel's per-(a, v) terms once -- round-trip distance (FP64 from an
+// FP32 rsqrt seed), delay tau, the output gradient times the carrier phasor
+// exp(+j 2 pi f0 tau), the per-sample step phasor and the FP64 step in cycles
+// -- and stages them in LDS; then each wave takes one chunk (and, when the
+// block has fewer chunks than waves, a share of the tile's voxels), seeds the
+// chunk's first phasor from the FP64-reduced phase and rotates it through the
+// chunk's 32 samples into 32 private complex accumulators.  The per-lane
+// accumulators are reduced across the wave and the voxel shares in a fixed
+// order at the end.
 constexpr int kAdjChunk = 32;
-constexpr int kAdjWaves = 4;
+constexpr int kAdjWaves = 8;
+constexpr int kAdjTile = 64 * kAdjWaves;
 
-__global__ __launch_bounds__(64 * kAdjWaves) void mf_backward_kernel(
+struct AdjStage {
+  float2 cg[kAdjTile];      // output gradient times the carrier phasor
+  float2 w[kAdjTile];       // per-sample step phasor
+  double cyc1[kAdjTile];    // per-sample step [cycles]
+};
+
+__global__ __launch_bounds__(kAdjTile) void mf_backward_kernel(
     const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
     float* __restrict__ grad_real, float* __restrict__ grad_imag,
-    const float* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc) {
+    const float* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc,
+    int chunks_per_block) {
+  __shared__ AdjStage st;
   __shared__ float red[kAdjWaves][kAdjChunk][2];
   const int a = blockIdx.x;
-  const int k0 = blockIdx.y * kAdjChunk;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int cpb = chunks_per_block;
+  const int shares = kAdjWaves / cpb;  // voxel shares per chunk
+  const int my_chunk = blockIdx.y * cpb + wave % cpb;
+  const int share = wave / cpb;
+  const int k0 = my_chunk * kAdjChunk;
+  const bool wave_ok = wave < cpb * shares && k0 < K;
 
   const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
   const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+  const double k0d = (double)k0;
 
   float pr[kAdjChunk], pi[kAdjChunk];
 #pragma unroll
   for (int j = 0; j < kAdjChunk; ++j) pr[j] = pi[j] = 0.0f;
 
-  for (int v = threadIdx.x; v < N; v += blockDim.x) {
-    const float gr = gout_real[v], gi = gout_imag[v];
-    const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
-    const double d = dist3_f64(tx, ty, tz, px, py, pz) + dist3_f64(rx, ry, rz, px, py, pz);
-    const double tau = (d + pc.bias) * pc.inv_c;
-    const double cyc1 = pc.dfs * tau;
-    const float2 seed =
-        phasor_cycles((float)centered_frac(pc.f0 * tau + (double)k0 * cyc1), 1.0f);
-    const float2 w = phasor_cycles((float)centered_frac(cyc1), 1.0f);
-    float2 z = cmul(make_float2(gr, gi), seed);
-#pragma unroll
-    for (int j = 0; j < kAdjChunk; ++j) {
-      pr[j] += z.x;
-      pi[j] += z.y;
-      z = cmul(z, w);
+  for (int base = 0; base < N; base += kAdjTile) {
+    {
+      const int v = base + threadIdx.x;
+      float2 cg = make_float2(0.0f, 0.0f), w = make_float2(1.0f, 0.0f);
+      double cyc1 = 0.0;
+      if (v < N) {
+        const float gr = gout_real[v], gi = gout_imag[v];
+        const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
+        const double d =
+            dist3_f64_fast(tx, ty, tz, px, py, pz) + dist3_f64_fast(rx, ry, rz, px, py, pz);
+        const double tau = (d + pc.bias) * pc.inv_c;
+        cyc1 = pc.dfs * tau;
+        cg = cmul(make_float2(gr, gi), phasor_cycles((float)centered_frac(pc.f0 * tau), 1.0f));
+        w = phasor_cycles((float)centered_frac(cyc1), 1.0f);
+      }
+      st.cg[threadIdx.x] = cg;
+      st.w[threadIdx.x] = w;
+      st.cyc1[threadIdx.x] = cyc1;
     }
+    __syncthreads();
+    if (wave_ok) {
+      const int n = min(kAdjTile, N - base);
+      for (int i = share * kWave + lane; i < n; i += shares * kWave) {
+        const float2 w = st.w[i];
+        float2 z = cmul(st.cg[i], phasor_cycles((float)centered_frac(k0d * st.cyc1[i]), 1.0f));
+#pragma unroll
+        for (int j = 0; j < kAdjChunk; ++j) {
+          pr[j] += z.x;
+          pi[j] += z.y;
+          z = cmul(z, w);
+        }
+      }
+    }
+    __syncthreads();
   }
 
 #pragma unroll
@@ -316,14 +356,16 @@ __global__ __launch_bounds__(64 * kAdjWaves) void mf_backward_kernel(
     }
   }
   __syncthreads();
-  if (threadIdx.x < kAdjChunk) {
-    const int j = threadIdx.x;
-    const int k = k0 + j;
+  // thread (chunk c of the block, sample j) sums its chunk's voxel shares
+  const int c = threadIdx.x / kAdjChunk;
+  const int j = threadIdx.x % kAdjChunk;
+  if (c < cpb) {
+    const int k = (blockIdx.y * cpb + c) * kAdjChunk + j;
     if (k < K) {
       double sr = 0.0, si = 0.0;
-      for (int w = 0; w < kAdjWaves; ++w) {
-        sr += red[w][j][0];
-        si += red[w][j][1];
+      for (int s = 0; s < shares; ++s) {
+        sr += red[s * cpb + c][j][0];
+        si += red[s * cpb + c][j][1];
       }
       grad_real[(size_t)a * K + k] = (float)(sr * inv_norm);
       grad_imag[(size_t)a * K + k] = (float)(si * inv_norm);
@@ -459,11 +501,16 @@ extern "C" int geraf_mf_backward_f32(const float* grad_out_real, const float* gr
   if (A == 0 || K == 0) return GERAF_RF_OK;
   require_ptrs({tx_pos, rx_pos, grad_real, grad_imag}, true);
   require_ptrs({grad_out_real, grad_out_imag, voxel}, N > 0);
-  const dim3 grid((unsigned)A, (unsigned)((K + kAdjChunk - 1) / kAdjChunk));
-  hipLaunchKernelGGL(mf_backward_kernel, grid, dim3(64 * kAdjWaves), 0, (hipStream_t)stream,
+  // chunks per block: a power of two dividing kAdjWaves, at most the chunk count
+  const int chunks = (int)((K + kAdjChunk - 1) / kAdjChunk);
+  int cpb = kAdjWaves;
+  while (cpb > 1 && cpb / 2 >= chunks) cpb /= 2;
+  const dim3 grid((unsigned)A, (unsigned)((chunks + cpb - 1) / cpb));
+  if (grid.y > 65535u) throw ArgError("too many samples per antenna");
+  hipLaunchKernelGGL(mf_backward_kernel, grid, dim3(kAdjTile), 0, (hipStream_t)stream,
                      grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag, voxel,
                      (int)A, (int)K, (int)N, 1.0 / (double)norm_count,
-                     make_phase_consts(*params));
+                     make_phase_consts(*params), cpb);
   return check_launch("mf_backward_kernel");
   GERAF_TRY_END
 }

@@ -14,19 +14,124 @@ stages).
   exp(light_power) times the transmittance, clamped at 1e-6.
 * ``FixedPowerNetwork`` (sdf_network_debug.py:10-26).
 
-MI355X: ``compute_dtype=torch.bfloat16`` runs the linear layers under BF16
-autocast, i.e. hipBLASLt GEMMs on the MFMA units with FP32 accumulation; the
-parameters stay FP32.  The reference runs these networks under FP16 autocast
-(rf_rendering.py:740).
+MI355X: ``compute_dtype=torch.bfloat16`` runs the layers in BF16 (GEMMs on
+the MFMA units with FP32 accumulation, activations stored in BF16); the
+parameters stay FP32 and are cast once per forward.  The reference runs these
+networks under FP16 autocast (rf_rendering.py:740).  Every layer product goes
+through ``tall_matmul``: the batch of points is tall ([M, 256] with M ~ 5e5)
+and the weights are small, and the weight gradient A^T G of such a product is
+a GEMM with only (256/64)^2 output tiles when handed to the BLAS as one call.
+``tall_matmul`` forms it as a split-K batched product instead, and does so
+again in the double backward the SDF gradient needs.
 """
 from __future__ import annotations
 
-import contextlib
 import math
 from typing import Optional, Sequence
 
 import torch
 import torch.nn as nn
+
+_SPLITS = 32
+
+
+def _splits(m: int) -> int:
+    s = _SPLITS
+    while s > 1 and m % s:
+        s //= 2
+    return s
+
+
+def col_sum(x: torch.Tensor) -> torch.Tensor:
+    """Column sums of a tall [M, q] tensor, accumulated in at least FP32:
+    partial sums over up to 512 row blocks, then over the blocks.  (A one-pass
+    BF16 sum over dim 0 takes milliseconds when q is not a multiple of 8.)"""
+    m, q = x.shape
+    s = 512
+    while s > 1 and m % s:
+        s //= 2
+    acc = torch.float64 if x.dtype == torch.float64 else torch.float32
+    return x.reshape(s, m // s, q).sum(1, dtype=acc).sum(0).to(x.dtype)
+
+
+def _tn(a: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+    """a^T g for tall a [M, p], g [M, q]: split-K over M, then a sum.  A single
+    column on either side is a broadcast product and a column sum (the BLAS
+    takes milliseconds over such a BF16 product)."""
+    if g.shape[1] == 1:
+        return col_sum(a * g).unsqueeze(1)
+    if a.shape[1] == 1:
+        return col_sum(a * g).unsqueeze(0)
+    m = a.shape[0]
+    s = _splits(m)
+    if s == 1:
+        return a.t() @ g
+    return torch.bmm(a.reshape(s, m // s, a.shape[1]).transpose(1, 2),
+                     g.reshape(s, m // s, g.shape[1])).sum(0)
+
+
+class _TallMatmul(torch.autograd.Function):
+    """C = A @ B, A [M, p] tall, B [p, q] small."""
+
+    @staticmethod
+    def forward(ctx, a, b):
+        ctx.save_for_backward(a, b)
+        return a @ b
+
+    @staticmethod
+    def backward(ctx, gc):
+        a, b = ctx.saved_tensors
+        ga = tall_matmul(gc, b.t()) if ctx.needs_input_grad[0] else None
+        gb = tall_tn(a, gc) if ctx.needs_input_grad[1] else None
+        return ga, gb
+
+
+class _TallTN(torch.autograd.Function):
+    """C = A^T G, A [M, p] and G [M, q] tall; C [p, q] small."""
+
+    @staticmethod
+    def forward(ctx, a, g):
+        ctx.save_for_backward(a, g)
+        return _tn(a, g)
+
+    @staticmethod
+    def backward(ctx, gc):
+        a, g = ctx.saved_tensors
+        ga = tall_matmul(g, gc.t()) if ctx.needs_input_grad[0] else None
+        gg = tall_matmul(a, gc) if ctx.needs_input_grad[1] else None
+        return ga, gg
+
+
+def tall_matmul(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """A @ B for a tall A; differentiable to any order."""
+    return _TallMatmul.apply(a, b)
+
+
+def tall_tn(a: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+    """A^T G for tall A and G; differentiable to any order."""
+    return _TallTN.apply(a, g)
+
+
+def linear(x: torch.Tensor, lin: nn.Linear, dtype: Optional[torch.dtype]) -> torch.Tensor:
+    """x @ W^T + b in `dtype` (None: the parameters' dtype) for a batch of
+    points x [M, in]."""
+    w = lin.weight if dtype is None else lin.weight.to(dtype)
+    y = tall_matmul(x, w.t())
+    if lin.bias is None:
+        return y
+    return _AddBias.apply(y, lin.bias if dtype is None else lin.bias.to(dtype))
+
+
+class _AddBias(torch.autograd.Function):
+    """y + b, b broadcast over the rows; db is a blocked column sum."""
+
+    @staticmethod
+    def forward(ctx, y, b):
+        return y + b
+
+    @staticmethod
+    def backward(ctx, gy):
+        return gy, col_sum(gy) if ctx.needs_input_grad[1] else None
 
 
 def positional_encoding(x: torch.Tensor, multires: int) -> torch.Tensor:
@@ -41,21 +146,16 @@ def positional_encoding(x: torch.Tensor, multires: int) -> torch.Tensor:
     return torch.cat(feats, dim=-1)
 
 
-def _autocast(dtype: Optional[torch.dtype], device: torch.device):
-    if dtype is None or dtype == torch.float32 or device.type != "cuda":
-        return contextlib.nullcontext()
-    return torch.autocast(device_type="cuda", dtype=dtype)
-
-
 class SDFNetwork(nn.Module):
     def __init__(self, d_in: int, d_out: int, d_hidden: int, n_layers: int,
                  skip_in: Sequence[int] = (4,), multires: int = 0, bias: float = 0.5,
                  scale: float = 1.0, geometric_init: bool = True, weight_norm: bool = True,
                  inside_outside: bool = False, sdf_activation: str = "none",
                  layer_activation: str = "softplus",
-                 compute_dtype: Optional[torch.dtype] = None):
+                 compute_dtype: Optional[torch.dtype] = None, tall_products: bool = True):
         super().__init__()
         self.multires = multires
+        self.tall_products = tall_products
         in_ch = d_in * (1 + 2 * multires) if multires > 0 else d_in
         dims = [in_ch] + [d_hidden] * n_layers + [d_out]
         self.num_layers = len(dims)
@@ -98,15 +198,20 @@ class SDFNetwork(nn.Module):
 
     def forward(self, inputs: torch.Tensor) -> torch.Tensor:
         inputs = positional_encoding(inputs * self.scale, self.multires)
-        with _autocast(self.compute_dtype, inputs.device):
-            x = inputs
-            for l in range(self.num_layers - 1):
-                if l in self.skip_in:
-                    x = torch.cat([x, inputs.to(x.dtype)], dim=-1) / math.sqrt(2.0)
-                x = getattr(self, f"lin{l}")(x)
-                if l < self.num_layers - 2:
-                    x = self.activation(x)
-        return x.float()
+        dt = self.compute_dtype if self.compute_dtype not in (None, torch.float32) else None
+        shape = inputs.shape[:-1]
+        xin = inputs.reshape(-1, inputs.shape[-1])
+        if dt is not None:
+            xin = xin.to(dt)
+        x = xin
+        for l in range(self.num_layers - 1):
+            if l in self.skip_in:
+                x = torch.cat([x, xin], dim=-1) / math.sqrt(2.0)
+            lin = getattr(self, f"lin{l}")
+            x = linear(x, lin, dt) if self.tall_products else lin(x)
+            if l < self.num_layers - 2:
+                x = self.activation(x)
+        return x.float().reshape(*shape, -1)
 
     def sdf(self, x: torch.Tensor) -> torch.Tensor:
         return self.forward(x)[..., :1]
@@ -164,17 +269,24 @@ def make_predictor(feats_dim: int, output_dim: int, weight_norm: bool = True,
 
 class ReflectivePowerNetwork(nn.Module):
     def __init__(self, feats_dim: int = 256, light_power: float = 0.0,
-                 refelective_act: str = "sigmoid", compute_dtype: Optional[torch.dtype] = None):
+                 refelective_act: str = "sigmoid", compute_dtype: Optional[torch.dtype] = None,
+                 tall_products: bool = True):
         super().__init__()
         self.refelective_predictor = make_predictor(feats_dim + 3, 1, activation=refelective_act)
         self.light_power = nn.Parameter(torch.tensor(float(light_power)))
         self.compute_dtype = compute_dtype
+        self.tall_products = tall_products
 
     def reflectance(self, points: torch.Tensor, features: torch.Tensor) -> torch.Tensor:
         x = torch.cat([features, points], dim=-1)
-        with _autocast(self.compute_dtype, x.device):
-            r = self.refelective_predictor(x)
-        return r.float().squeeze(-1)
+        dt = self.compute_dtype if self.compute_dtype not in (None, torch.float32) else None
+        shape = x.shape[:-1]
+        x = x.reshape(-1, x.shape[-1])
+        if dt is not None:
+            x = x.to(dt)
+        for m in self.refelective_predictor:
+            x = linear(x, m, dt) if self.tall_products and isinstance(m, nn.Linear) else m(x)
+        return x.float().reshape(shape)
 
     def power(self) -> torch.Tensor:
         return torch.exp(self.light_power)

@@ -4,10 +4,14 @@ readings calibrate those counters (known traffic: 1 GiB read, 1 GiB written).
 
     rocprofv3 --pmc <counters> -d <dir> -o pmc -- python3 geraf-sens_amd/rf_workloads/profile_driver.py
 
-Kernels, in dispatch order: the copy; mf_forward_horner_kernel on the 64x64
-MIMO / K=512 / 128^3 workload; lensless_forward_kernel and
-lensless_backward_kernel on 4,096 rays x 128 samples, K=256.
+Kernels, in dispatch order: the copy; mf_forward_horner_kernel and
+mf_backward_kernel on the 64x64 MIMO / K=512 / 128^3 workload;
+lensless_forward_kernel and
+lensless_backward_kernel on 4,096 rays x 128 samples, K=256, then (with
+--stage2) once more on 32,768 rays x 256 samples, K=512.  --tracer-only skips
+the copy and the matched filter.
 """
+import argparse
 import os
 import sys
 
@@ -23,8 +27,38 @@ from geraf.ops.RFSignalTracer import _C as stC  # noqa: E402
 from rf_workloads import mf_scenes, tracer_scenes as ts  # noqa: E402
 
 
+def run_tracer(dev, bb) -> None:
+    R, S, Kt = bb.R, bb.S, bb.radar["numADCSample"]
+    tp = stC.tracer_params(bb.radar["fc_start"], bb.radar["As_sci"], bb.radar["adcSampleRate"])
+    P = _native.ptr
+    inv_s = bb.inv_s.reshape(1).contiguous()
+    power = bb.power.reshape(1).contiguous()
+    ins = [P(bb.sdf), P(bb.grad), P(bb.refl), P(bb.dists), P(bb.z), P(bb.o), P(bb.d),
+           P(bb.offset), P(bb.n_valid), P(inv_s), P(power)]
+    lr = torch.empty((R, Kt), device=dev)
+    li = torch.empty_like(lr)
+    st = _native.stream_of(lr)
+    _native.call("geraf_lensless_forward_f32", *ins, P(lr), P(li), P(None), R, S, Kt, bb.anneal,
+                 tp, st)
+    g_s = torch.empty((R, S), device=dev)
+    g_g = torch.empty((R, S, 3), device=dev)
+    g_r = torch.empty((R, S), device=dev)
+    g_i = torch.empty(R, device=dev)
+    g_p = torch.empty(R, device=dev)
+    _native.call("geraf_lensless_backward_f32", P(lr), P(li), *ins, P(g_s), P(g_g), P(g_r),
+                 P(g_i), P(g_p), R, S, Kt, bb.anneal, tp, st)
+    torch.cuda.synchronize()
+
+
 def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stage2", action="store_true")
+    ap.add_argument("--tracer-only", action="store_true")
+    args = ap.parse_args()
     dev = torch.device("cuda:0")
+    if args.tracer_only:
+        trace_sizes(dev, args.stage2)
+        return
     print("profile_driver: 1 GiB copy", flush=True)
     a = torch.empty(1 << 28, device=dev)
     b = torch.empty_like(a)
@@ -44,29 +78,22 @@ def main() -> None:
     _native.call("geraf_mf_forward_f32", *map(_native.ptr, (re, im, tx, rx, o_r, o_i, vox)),
                  A, K, N, prm, _native.stream_of(re))
     torch.cuda.synchronize()
-
-    print("profile_driver: lensless tracer 4096 x 128, K=256", flush=True)
-    bb = ts.config3(0).to(dev)
-    R, S, Kt = bb.R, bb.S, bb.radar["numADCSample"]
-    tp = stC.tracer_params(bb.radar["fc_start"], bb.radar["As_sci"], bb.radar["adcSampleRate"])
-    P = _native.ptr
-    inv_s = bb.inv_s.reshape(1).contiguous()
-    power = bb.power.reshape(1).contiguous()
-    ins = [P(bb.sdf), P(bb.grad), P(bb.refl), P(bb.dists), P(bb.z), P(bb.o), P(bb.d), P(None),
-           P(None), P(inv_s), P(power)]
-    lr = torch.empty((R, Kt), device=dev)
-    li = torch.empty_like(lr)
-    st = _native.stream_of(lr)
-    _native.call("geraf_lensless_forward_f32", *ins, P(lr), P(li), P(None), R, S, Kt, bb.anneal,
-                 tp, st)
-    g_s = torch.empty((R, S), device=dev)
-    g_g = torch.empty((R, S, 3), device=dev)
-    g_r = torch.empty((R, S), device=dev)
-    g_i = torch.empty(R, device=dev)
-    g_p = torch.empty(R, device=dev)
-    _native.call("geraf_lensless_backward_f32", P(lr), P(li), *ins, P(g_s), P(g_g), P(g_r),
-                 P(g_i), P(g_p), R, S, Kt, bb.anneal, tp, st)
+    print("profile_driver: matched-filter adjoint, same workload", flush=True)
+    a_r = torch.empty((A, K), device=dev)
+    a_i = torch.empty_like(a_r)
+    _native.call("geraf_mf_backward_f32", *map(_native.ptr, (o_r, o_i, tx, rx, a_r, a_i, vox)),
+                 A, K, N, A, prm, _native.stream_of(re))
     torch.cuda.synchronize()
+
+    trace_sizes(dev, args.stage2)
+
+
+def trace_sizes(dev, stage2: bool) -> None:
+    print("profile_driver: lensless tracer 4096 x 128, K=256", flush=True)
+    run_tracer(dev, ts.config3(0).to(dev))
+    if stage2:
+        print("profile_driver: lensless tracer 32768 x 256, K=512", flush=True)
+        run_tracer(dev, ts.rays(32768, 256, 512, seed=0).to(dev))
     print("profile_driver: done", flush=True)
 
 

@@ -15,10 +15,11 @@ One step (GeRaFStage1.loss, rf_rendering.py:721-820, in its per-ray form):
   5. backward (matched-filter adjoint, tracer adjoint, MLP double-backward)
      and an AdamW step.
 
-``form="fused"`` is the MI355X path (BF16 MLPs on MFMA via hipBLASLt, the
-hand-written tracer kernels); ``form="fp32_autograd"`` is the all-FP32
-autograd form of the same step: FP32 MLPs and the compositing + signal as
-torch FP32 ops (the reference's glue style), with the same matched filter.
+``form="fused"`` is the MI355X path (BF16 MLPs on MFMA with split-K weight
+gradients, the hand-written tracer kernels); ``form="fp32_autograd"`` is the
+all-FP32 autograd form of the same step: FP32 nn.Linear MLPs and the
+compositing + signal as torch FP32 ops (the reference's glue style), with the
+same matched filter.
 """
 from __future__ import annotations
 
@@ -102,10 +103,12 @@ class StageStep:
         if mlp_dtype == "auto":
             mlp_dtype = torch.bfloat16 if form == "fused" else None
         cd = mlp_dtype
-        self.sdf_net = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6,
-                                  compute_dtype=cd).to(device)
+        tall = form == "fused"  # the autograd form keeps plain nn.Linear layers
+        self.sdf_net = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6, compute_dtype=cd,
+                                  tall_products=tall).to(device)
         self.var_net = SingleVarianceNetwork(0.3).to(device)
-        self.sig_net = ReflectivePowerNetwork(256, light_power=0.0, compute_dtype=cd).to(device)
+        self.sig_net = ReflectivePowerNetwork(256, light_power=0.0, compute_dtype=cd,
+                                              tall_products=tall).to(device)
         params = list(self.sdf_net.parameters()) + list(self.var_net.parameters()) + \
             list(self.sig_net.parameters())
         self.opt = torch.optim.AdamW(params, lr=5e-4)

@@ -1,0 +1,71 @@
+"""The MLPs that feed the lensless tracer (CPU, float64): the split-K tall
+products are exact restatements of the plain layer products, to any
+derivative order."""
+import pytest
+import torch
+
+from geraf.models.networks.sdf_network import (ReflectivePowerNetwork, SDFNetwork, col_sum,
+                                                linear, tall_matmul, tall_tn)
+
+
+@pytest.mark.parametrize("m", [64, 96, 37])
+def test_tall_products_gradcheck(m):
+    torch.manual_seed(m)
+    d = torch.float64
+    a = torch.randn(m, 5, dtype=d, requires_grad=True)
+    one = torch.randn(m, 1, dtype=d, requires_grad=True)
+    b = torch.randn(5, 3, dtype=d, requires_grad=True)
+    g = torch.randn(m, 3, dtype=d, requires_grad=True)
+    assert torch.autograd.gradcheck(tall_matmul, (a, b))
+    assert torch.autograd.gradgradcheck(tall_matmul, (a, b))
+    assert torch.autograd.gradcheck(tall_tn, (a, g))
+    assert torch.autograd.gradgradcheck(tall_tn, (a, g))
+    assert torch.autograd.gradgradcheck(tall_tn, (a, one))
+    assert torch.autograd.gradgradcheck(tall_tn, (one, g))
+    lin = torch.nn.Linear(5, 3).double()
+    assert torch.autograd.gradgradcheck(lambda x: linear(x, lin, None), (a,))
+    # split-K sum equals the plain product; blocked column sums equal sum(0)
+    assert torch.allclose(tall_tn(a, g), a.t() @ g, rtol=1e-13, atol=1e-13)
+    assert torch.allclose(col_sum(g), g.sum(0), rtol=1e-13, atol=1e-13)
+    assert torch.autograd.gradgradcheck(col_sum, (g,))
+
+
+def _eikonal_step(net, x):
+    out = net.sdf_hidden_appearance(x)
+    grad = torch.autograd.grad(out[:, :1], x, torch.ones_like(out[:, :1]), create_graph=True)[0]
+    loss = out[:, 0].abs().mean() + ((grad.norm(dim=-1) - 1.0) ** 2).mean() + out[:, 1:].mean()
+    params = list(net.parameters())
+    gp = torch.autograd.grad(loss, params)
+    return out.detach(), grad.detach(), gp
+
+
+def test_sdf_network_tall_matches_plain():
+    torch.manual_seed(0)
+    kw = dict(d_in=3, d_out=17, d_hidden=64, n_layers=8, skip_in=(4,), multires=6)
+    tall = SDFNetwork(**kw, tall_products=True).double()
+    plain = SDFNetwork(**kw, tall_products=False).double()
+    plain.load_state_dict(tall.state_dict())
+    x = (torch.rand(160, 3, dtype=torch.float64) * 2 - 1).requires_grad_(True)
+    o1, g1, p1 = _eikonal_step(tall, x)
+    x2 = x.detach().clone().requires_grad_(True)
+    o2, g2, p2 = _eikonal_step(plain, x2)
+    assert torch.allclose(o1, o2, rtol=1e-12, atol=1e-12)
+    assert torch.allclose(g1, g2, rtol=1e-12, atol=1e-12)
+    for a, b in zip(p1, p2):
+        assert torch.allclose(a, b, rtol=1e-10, atol=1e-12)
+
+
+def test_reflectance_network_tall_matches_plain():
+    torch.manual_seed(1)
+    tall = ReflectivePowerNetwork(16, tall_products=True).double()
+    plain = ReflectivePowerNetwork(16, tall_products=False).double()
+    plain.load_state_dict(tall.state_dict())
+    pts = torch.randn(50, 3, dtype=torch.float64)
+    feat = torch.randn(50, 16, dtype=torch.float64, requires_grad=True)
+    r1 = tall.reflectance(pts, feat)
+    r2 = plain.reflectance(pts, feat)
+    assert torch.allclose(r1, r2, rtol=1e-12, atol=1e-12)
+    g1 = torch.autograd.grad(r1.sum(), list(tall.refelective_predictor.parameters()) + [feat])
+    g2 = torch.autograd.grad(r2.sum(), list(plain.refelective_predictor.parameters()) + [feat])
+    for a, b in zip(g1, g2):
+        assert torch.allclose(a, b, rtol=1e-10, atol=1e-12)
