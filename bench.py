@@ -182,6 +182,10 @@ def emitting_fraction(b) -> float:
     return float(((idn <= 0) & (spec >= 1e-6)).float().mean())
 
 
+def samples_per_call(R: int, S: int) -> int:
+    return R * S
+
+
 def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
                  rays: int = 4096, samples: int = 128, K: int = 256, oracle_check: bool = True):
     """Fused lensless tracer, forward + backward, on `rays` x `samples`, K
@@ -228,27 +232,31 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         fwd()
         bwd()
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    t0 = time.perf_counter()
-    for e in ev:
-        e[0].record()
-        fwd()
-        e[1].record()
-        bwd()
-        e[2].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    f_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
-    b_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+
+    def burst(fn, n):
+        """Mean time of n back-to-back launches between one pair of events on
+        the launch stream (launch gaps included, event overhead amortised)."""
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    n = max(steps, 4) if samples_per_call(R, S) > (1 << 22) else max(steps, 50)
+    f_ms = burst(fwd, n)
+    b_ms = burst(bwd, n)
+    pair_ms = burst(lambda: (fwd(), bwd()), n)
     samples = R * S
-    rate = samples * steps / wall
+    rate = samples / (pair_ms / 1e3)
     # algorithmic traffic and work (see DESIGN.md)
     bytes_fwd = samples * 28 + R * (24 + 8 * K) + 8
     bytes_bwd = samples * (28 + 20) + R * (24 + 8 * K + 8) + 8
     flops = 16.0 * K * samples * live  # only emitting samples carry signal work
-    gbs = (bytes_fwd + bytes_bwd) / ((f_ms + b_ms) / 1e3) / 1e9
-    tfs = flops / ((f_ms + b_ms) / 1e3) / 1e12
-    log(f"tracer {R}x{S} K={K}: fwd {f_ms:.3f} ms, bwd {b_ms:.3f} ms, {rate:.3e} samples/s, "
+    gbs = (bytes_fwd + bytes_bwd) / (pair_ms / 1e3) / 1e9
+    tfs = flops / (pair_ms / 1e3) / 1e12
+    log(f"tracer {R}x{S} K={K}: fwd {f_ms:.3f} ms, bwd {b_ms:.3f} ms, pair {pair_ms:.3f} ms, {rate:.3e} samples/s, "
         f"{gbs:.0f} GB/s, {tfs:.1f} TFLOP/s (emitting fraction {live:.3f})")
 
     # accuracy against the FP64 oracle on a ray subset
@@ -297,7 +305,7 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         "metric": "tracer fwd+bwd samples/s",
         "value": rate, "unit": "samples/s",
         "workload": f"fused lensless tracer, {R} rays x {S} samples, K={K}",
-        "fwd_ms": f_ms, "bwd_ms": b_ms, "emitting_fraction": live,
+        "fwd_ms": f_ms, "bwd_ms": b_ms, "fwd_bwd_ms": pair_ms, "emitting_fraction": live,
         "roofline": {"hbm_gbs": gbs, "hbm_frac": gbs / 8000.0, "valu_tflops": tfs,
                      "valu_frac": tfs / VALU_FP32_PEAK_TFLOPS,
                      "bytes_per_step": bytes_fwd + bytes_bwd, "flops_per_step": flops},

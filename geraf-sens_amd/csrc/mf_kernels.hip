@@ -401,12 +401,15 @@ __global__ __launch_bounds__(256) void mf_backward_f64_kernel(
   grad_imag[idx] = acc_i * inv_norm;
 }
 
-int pick_groups(int A, int N, int waves) {
-  // Enough waves to give every SIMD of the 256-CU part a few of them.
-  const long long want = 4LL * 1024;
-  const long long tiles = (N + kWave - 1) / kWave;
+// Waves wanted per launch: 8 per SIMD of the 256-CU part, so that two of them
+// are ready to dual-issue most of the time.
+constexpr long long kWantWaves = 8LL * 1024;
+
+// Antenna groups per voxel unit (a power of two, at most `waves` and A) so
+// that units * groups reaches kWantWaves.
+int pick_groups(int A, long long units, int waves) {
   int g = 1;
-  while (g < waves && g * 2 <= A && tiles * g < want) g *= 2;
+  while (g < waves && g * 2 <= A && units * g < kWantWaves) g *= 2;
   return g;
 }
 
@@ -429,18 +432,20 @@ extern "C" int geraf_mf_forward_f32(const float* real, const float* imag, const 
   const PhaseConsts pc = make_phase_consts(*params);
   const int n = (int)N;
   const int tiles = (n + kWave - 1) / kWave;
-  if (tiles < 4 * 1024 && A > 1) {
-    // small volume: split the antennas over the waves of a block
+  constexpr int V = GERAF_MF_VPL;
+  const int units = (tiles + V - 1) / V;  // waves of V voxel tiles each
+  if (units < kWantWaves && A > 1) {
+    // volume too small to fill the part: split the antennas over the waves of
+    // a block as well
     constexpr int W = 16;
-    const int g = pick_groups((int)A, n, W);
-    const int tiles_per_block = W / g;
-    const int blocks = (tiles + tiles_per_block - 1) / tiles_per_block;
-    hipLaunchKernelGGL((mf_forward_horner_kernel<W, 1>), dim3(blocks), dim3(64 * W), 0,
+    const int g = pick_groups((int)A, units, W);
+    const int units_per_block = W / g;
+    const int blocks = (units + units_per_block - 1) / units_per_block;
+    hipLaunchKernelGGL((mf_forward_horner_kernel<W, V>), dim3(blocks), dim3(64 * W), 0,
                        (hipStream_t)stream, real, imag, tx_pos, rx_pos, out_real, out_imag,
                        voxel, (int)A, (int)K, n, g, pc);
   } else {
     constexpr int W = GERAF_MF_WAVES;
-    constexpr int V = GERAF_MF_VPL;
     const int blocks = (tiles + W * V - 1) / (W * V);
     hipLaunchKernelGGL((mf_forward_horner_kernel<W, V>), dim3(blocks), dim3(64 * W), 0,
                        (hipStream_t)stream, real, imag, tx_pos, rx_pos, out_real, out_imag,

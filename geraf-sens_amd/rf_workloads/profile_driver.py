@@ -9,7 +9,11 @@ mf_backward_kernel on the 64x64 MIMO / K=512 / 128^3 workload;
 lensless_forward_kernel and
 lensless_backward_kernel on 4,096 rays x 128 samples, K=256, then (with
 --stage2) once more on 32,768 rays x 256 samples, K=512.  --tracer-only skips
-the copy and the matched filter.
+the copy and the matched filter.  --stage1-step N instead runs two warm-up
+and N timed stage-1-style training steps (8,192 rays x 64 samples, K=64;
+fused form, BF16 MLPs) for a kernel trace of the whole step; the target
+volume is a smooth synthetic blob on the 32^3 grid, which fixes the work but
+not the values.
 """
 import argparse
 import os
@@ -54,8 +58,13 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--stage2", action="store_true")
     ap.add_argument("--tracer-only", action="store_true")
+    ap.add_argument("--stage1-step", type=int, default=0)
+    ap.add_argument("--form", default="fused", choices=("fused", "fp32_autograd"))
     args = ap.parse_args()
     dev = torch.device("cuda:0")
+    if args.stage1_step:
+        stage_steps(dev, args.stage1_step, args.form)
+        return
     if args.tracer_only:
         trace_sizes(dev, args.stage2)
         return
@@ -86,6 +95,27 @@ def main() -> None:
     torch.cuda.synchronize()
 
     trace_sizes(dev, args.stage2)
+
+
+def stage_steps(dev, n: int, form: str) -> None:
+    import time
+
+    from rf_workloads import stage_step as st
+
+    sc = mf_scenes.config1(0)
+    g = [torch.tensor(a) for a in sc.axes]
+    X, Y, Z = torch.meshgrid(*g, indexing="ij")
+    vol = torch.exp(-((X ** 2 + Y ** 2 + (Z - 0.25) ** 2) / 0.03 ** 2)).to(dev)
+    step = st.StageStep(st.STAGE1, vol.to(torch.complex64), tuple(sc.axes), dev, form=form)
+    for _ in range(2):
+        step.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        step.step()
+    torch.cuda.synchronize()
+    print(f"profile_driver: stage-1 step [{form}] {(time.perf_counter() - t0) / n * 1e3:.1f} ms",
+          flush=True)
 
 
 def trace_sizes(dev, stage2: bool) -> None:
