@@ -242,3 +242,70 @@ def test_range_bin_mf_vs_oracle(cuda, dtype):
                                            701500.0, N)
     refg = orc.mf_fft_backward(g, tx, rx, vox, R, spacing, A, **prm)
     assert rel_l2(g_r.double().cpu().numpy() + 1j * g_i.double().cpu().numpy(), refg) <= tol
+
+
+def test_adc_bank_selected_rows_matched_filter(cuda):
+    """The device bank's selected-rows matched filter equals the full matched
+    filter of the mixed bank, in value and in the selected rows' gradients."""
+    from geraf.models.model_utils.adc_bank import ADCBank
+    from rf_workloads import mf_scenes
+
+    sc = mf_scenes.config1(0)
+    A, K = sc.A, sc.K
+    t_pos = torch.tensor(sc.tx, device=cuda)
+    r_pos = torch.tensor(sc.rx, device=cuda)
+    pts = torch.tensor(sc.voxels[::37], device=cuda)
+    radar = dict(numADCSample=K, adcSampleRate=sc.fs, As_sci=sc.As_sci)
+    bank = ADCBank(A, K, 4, cuda)
+    rng = np.random.default_rng(3)
+    for g in range(4):  # fill every group once
+        n = len(bank.groups[g])
+        bank.store(g, torch.tensor(rng.standard_normal((n, K)), dtype=torch.float32, device=cuda),
+                   torch.tensor(rng.standard_normal((n, K)), dtype=torch.float32, device=cuda))
+    bank.advance()
+    n = len(bank.selected)
+    sim_r = torch.tensor(rng.standard_normal((n, K)), dtype=torch.float32, device=cuda)
+    sim_i = torch.tensor(rng.standard_normal((n, K)), dtype=torch.float32, device=cuda)
+    outs = []
+    for fast in (True, False):
+        r = sim_r.clone().requires_grad_(True)
+        i = sim_i.clone().requires_grad_(True)
+        yr, yi = bank.matched_filter(r, i, t_pos, r_pos, pts, radar, selected_rows_only=fast)
+        (yr.square().sum() + 0.5 * yi.sum()).backward()
+        outs.append((yr.detach(), yi.detach(), r.grad, i.grad))
+    for a, b in zip(*outs):
+        rel = float((a - b).norm() / b.norm())
+        assert rel < 1e-6, rel
+
+
+def test_backproject_volume_matches_matched_filter(cuda, tmp_path):
+    from geraf.models.model_utils import mf_volume as mv
+    from geraf.ops.RFMatchedFilter import matched_filter_cuda
+    from rf_workloads import mf_scenes
+
+    sc = mf_scenes.config1(0)
+    A, K = sc.A, sc.K
+    re = torch.tensor(np.ascontiguousarray(sc.echo.real), device=cuda)
+    im = torch.tensor(np.ascontiguousarray(sc.echo.imag), device=cuda)
+    tx, rx = torch.tensor(sc.tx, device=cuda), torch.tensor(sc.rx, device=cuda)
+    radar = dict(numADCSample=K, adcSampleRate=sc.fs, As_sci=sc.As_sci)
+    bound, res = [-0.02, 0.02, -0.02, 0.02, 0.23, 0.27], [0.004, 0.004, 0.004]
+    vol = mv.backproject(re, im, tx, rx, bound, res, radar, slab_voxels=300)  # several slabs
+    ax = [torch.tensor(a, dtype=torch.float32, device=cuda) for a in mv.grid_axes(bound, res)]
+    g = torch.meshgrid(*ax, indexing="ij")
+    vox = torch.stack([x.reshape(-1) for x in g], 1)
+    r2, i2 = matched_filter_cuda(re, im, tx, rx, vox, A, K, sc.fs, sc.As_sci, vox.shape[0])
+    # slabs of different sizes launch with different antenna groupings, so the
+    # partial sums combine in another order: equal to FP32 rounding
+    ref = torch.hypot(r2, i2)
+    assert float((vol.reshape(-1) - ref).norm() / ref.norm()) < 1e-6
+    # the producer writes the same cube
+    d = tmp_path / "0_0_0"
+    d.mkdir()
+    for name, arr in (("adcr.npy", sc.echo.real), ("adci.npy", sc.echo.imag),
+                      ("tpos.npy", sc.tx), ("rpos.npy", sc.rx)):
+        np.save(d / name, np.asarray(arr, dtype=np.float32))
+    cube = mv.produce_frame(str(d), bound, res, radar, device=str(cuda))
+    assert np.array_equal(mv.read_sarimage(str(d / "sarimage.bin"), bound, res), cube)
+    v = vol.cpu().numpy()
+    assert np.linalg.norm(cube - v) / np.linalg.norm(v) < 1e-6
