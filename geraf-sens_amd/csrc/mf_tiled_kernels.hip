@@ -1,0 +1,341 @@
+// Fast matched filter by per-tile expansions: the operator of
+// MatchedFilter_cuda_kernel.cu:65-115,
+//   out[v] = (1/A) sum_a sum_k s[a,k] exp(-j 2 pi (f0 + k d) tau_av),  d = slope/fs,
+// evaluated to a stated truncation bound with O(N_T) work per voxel-antenna
+// pair instead of O(K).
+//
+// Voxels come in tiles of 512 (the caller orders them so that a tile is
+// compact in space).  A tile with centre c and radius r sees every antenna's
+// delays within H = 2 r / c of the centre delay tau_ac (triangle inequality).
+// With k = k_c + kappa, k_c = (K-1)/2, u = (tau_av - tau_ac)/H in [-1, 1] and
+// xi_k = kappa / k_c in [-1, 1]:
+//   sum_k s_k e^{-j2pi(f0+kd)tau} = e^{-j2pi(f0 + k_c d) tau} sum_n C_n u^n,
+//   C_n = (-j)^n Z^n / n! sum_k s_k e^{-j2pi kappa d tau_ac} xi_k^n,
+//   Z = pi (K-1) d H,
+// and truncating after N_T terms leaves at most sum_k |s_k| Z^N_T/N_T! e^Z
+// (the host picks N_T from the largest tile).
+//
+// Kernels: per antenna chunk, mf_tile_moments_kernel (wave = antenna, lane =
+// tile: the echo row and the xi^n table are wave-uniform SGPR operands; the
+// phasor e^{-j2pi kappa d tau_ac} is a rotation re-seeded every 32 samples
+// from the FP64-reduced phase), then mf_tile_eval_kernel (block = tile, two
+// voxels per lane, the tile's coefficients of one antenna are wave-uniform
+// SGPR operands; per pair: FP64 distances and phase, one phasor, an N_T-term
+// real Horner with complex coefficients, Kahan accumulation), adding each
+// chunk's sums into FP64 accumulators; a last kernel scales by 1/A.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include <algorithm>
+
+#include "abi_util.hpp"
+#include "rf_common.hpp"
+
+namespace geraf {
+namespace {
+
+constexpr int kTileVox = 512;  // voxels per tile (4 waves x 64 lanes x 2)
+constexpr int kEvalWaves = 4;
+constexpr int kMomentWaves = 4;
+
+struct TileGeom {
+  double cx, cy, cz;  // centre
+  double H;           // delay half-spread 2 r / c
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const float4* tile_info, int t, double inv_c) {
+  const float4 ti = tile_info[t];
+  TileGeom g;
+  g.cx = ti.x;
+  g.cy = ti.y;
+  g.cz = ti.z;
+  g.H = 2.0 * (double)ti.w * inv_c;
+  return g;
+}
+
+// xi_pow[k * NT + n] = xi_k^n, xi_k = (k - kc) / kc (0 when kc == 0).
+__global__ void xi_pow_kernel(float* __restrict__ xi_pow, int K, int NT, double kc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const double xi = kc > 0.0 ? ((double)k - kc) / kc : 0.0;
+  double p = 1.0;
+  for (int n = 0; n < NT; ++n) {
+    xi_pow[(size_t)k * NT + n] = (float)p;
+    p *= xi;
+  }
+}
+
+__global__ void fill_zero_kernel(double* __restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+template <int NT>
+__global__ __launch_bounds__(64 * kMomentWaves) void mf_tile_moments_kernel(
+    const float* __restrict__ real, const float* __restrict__ imag,
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    const float4* __restrict__ tile_info, const float* __restrict__ xi_pow, int a0, int na,
+    int K, int ntiles, PhaseConsts pc, double kc, float2* __restrict__ coef,
+    double* __restrict__ tau_c) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ai = blockIdx.y * kMomentWaves + wave;  // antenna within the chunk
+  if (ai >= na) return;                             // wave-uniform
+  const int a = a0 + ai;
+  const int t = blockIdx.x * kWave + lane;
+  const bool tile_ok = t < ntiles;
+  const TileGeom g = tile_geom(tile_info, tile_ok ? t : ntiles - 1, pc.inv_c);
+  const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+  const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+  const double tac = (dist3_f64_fast(tx, ty, tz, g.cx, g.cy, g.cz) +
+                      dist3_f64_fast(rx, ry, rz, g.cx, g.cy, g.cz) + pc.bias) *
+                     pc.inv_c;
+  const double step = pc.dfs * tac;  // cycles per sample at the centre delay
+  // e^{-j 2 pi d tau_ac}: one sample forward
+  const float2 w = phasor_cycles((float)centered_frac(step), -1.0f);
+
+  float Sr[NT], Si[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) Sr[n] = Si[n] = 0.0f;
+
+  const int row = __builtin_amdgcn_readfirstlane(a * K);
+  const float* __restrict__ er = real + row;
+  const float* __restrict__ ei = imag + row;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    // e^{-j 2 pi (k0 - kc) d tau_ac} from the FP64-reduced phase
+    float2 ph = phasor_cycles((float)centered_frac((kc - (double)k0) * step), 1.0f);
+    const int k1 = min(K, k0 + 32);
+    for (int k = k0; k < k1; ++k) {
+      const float sr = er[k], si = ei[k];
+      const float br = fmaf(sr, ph.x, -si * ph.y);
+      const float bi = fmaf(sr, ph.y, si * ph.x);
+      const float* __restrict__ xp = xi_pow + (size_t)k * NT;
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        Sr[n] = fmaf(br, xp[n], Sr[n]);
+        Si[n] = fmaf(bi, xp[n], Si[n]);
+      }
+      ph = cmul(ph, w);
+    }
+  }
+  if (!tile_ok) return;
+  // C_n = (-j)^n Z^n / n! S_n
+  const float Z = (float)(3.14159265358979323846 * (double)(K - 1) * pc.dfs * g.H);
+  float scale = 1.0f;
+  float2* __restrict__ out = coef + ((size_t)ai * ntiles + t) * NT;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const float xr = Sr[n] * scale, xi = Si[n] * scale;
+    float2 c;
+    switch (n & 3) {
+      case 0: c = make_float2(xr, xi); break;
+      case 1: c = make_float2(xi, -xr); break;
+      case 2: c = make_float2(-xr, -xi); break;
+      default: c = make_float2(-xi, xr); break;
+    }
+    out[n] = c;
+    scale *= Z / (float)(n + 1);
+  }
+  tau_c[(size_t)ai * ntiles + t] = tac;
+}
+
+template <int NT>
+__global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    const float* __restrict__ voxel, const float4* __restrict__ tile_info,
+    const float2* __restrict__ coef, const double* __restrict__ tau_c, int a0, int na, int N,
+    int ntiles, PhaseConsts pc, double fcen, double* __restrict__ acc_re,
+    double* __restrict__ acc_im) {
+  constexpr int VPL = 2;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = blockIdx.x;
+  const TileGeom g = tile_geom(tile_info, t, pc.inv_c);
+  const double invH = g.H > 0.0 ? 1.0 / g.H : 0.0;
+
+  int v[VPL];
+  bool valid[VPL];
+  double px[VPL], py[VPL], pz[VPL];
+#pragma unroll
+  for (int i = 0; i < VPL; ++i) {
+    v[i] = t * kTileVox + (wave * VPL + i) * kWave + lane;
+    valid[i] = v[i] < N;
+    const int vc = valid[i] ? v[i] : N - 1;
+    px[i] = voxel[vc];
+    py[i] = voxel[(size_t)N + vc];
+    pz[i] = voxel[2 * (size_t)N + vc];
+  }
+  KahanC acc[VPL];
+#pragma unroll
+  for (int i = 0; i < VPL; ++i) acc[i].init();
+
+  for (int ai = 0; ai < na; ++ai) {
+    const int a = a0 + ai;
+    const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+    const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+    const size_t ct = __builtin_amdgcn_readfirstlane((unsigned)ai) * (size_t)ntiles + t;
+    const double tac = tau_c[ct];
+    const float2* __restrict__ cf = coef + ct * NT;
+#pragma unroll
+    for (int i = 0; i < VPL; ++i) {
+      const double tau = (dist3_f64_fast(tx, ty, tz, px[i], py[i], pz[i]) +
+                          dist3_f64_fast(rx, ry, rz, px[i], py[i], pz[i]) + pc.bias) *
+                         pc.inv_c;
+      const float u = (float)((tau - tac) * invH);
+      float Pr = cf[NT - 1].x, Pi = cf[NT - 1].y;
+#pragma unroll
+      for (int n = NT - 2; n >= 0; --n) {
+        Pr = fmaf(Pr, u, cf[n].x);
+        Pi = fmaf(Pi, u, cf[n].y);
+      }
+      const float2 base = phasor_cycles((float)centered_frac(fcen * tau), -1.0f);
+      acc[i].add(cmul(base, make_float2(Pr, Pi)));
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < VPL; ++i) {
+    if (valid[i]) {
+      acc_re[v[i]] += acc[i].re();
+      acc_im[v[i]] += acc[i].im();
+    }
+  }
+}
+
+__global__ void mf_tile_finish_kernel(const double* __restrict__ acc_re,
+                                      const double* __restrict__ acc_im,
+                                      float* __restrict__ out_re, float* __restrict__ out_im,
+                                      int N, double inv_a) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  out_re[v] = (float)(acc_re[v] * inv_a);
+  out_im[v] = (float)(acc_im[v] * inv_a);
+}
+
+// Instantiated term counts; a request is rounded up to the next one.
+constexpr int kTermSets[] = {4, 6, 8, 10, 12, 14, 16, 20, 24, 32};
+
+int round_terms(int nterms) {
+  for (int t : kTermSets)
+    if (nterms <= t) return t;
+  return -1;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct TiledLayout {
+  size_t acc_bytes, xi_bytes, per_antenna, fixed;
+};
+
+TiledLayout tiled_layout(int64_t K, int64_t N, int NT) {
+  const int64_t ntiles = (N + kTileVox - 1) / kTileVox;
+  TiledLayout l;
+  l.acc_bytes = align256((size_t)N * sizeof(double));
+  l.xi_bytes = align256((size_t)K * NT * sizeof(float));
+  l.per_antenna = (size_t)ntiles * (NT * sizeof(float2) + sizeof(double));
+  l.fixed = 2 * l.acc_bytes + l.xi_bytes + 2 * 256;
+  return l;
+}
+
+template <int NT>
+void launch_chunk(const float* real, const float* imag, const float* tx, const float* rx,
+                  const float* voxel, const float4* tile_info, const float* xi_pow, int a0,
+                  int na, int K, int N, int ntiles, const PhaseConsts& pc, double kc,
+                  float2* coef, double* tau_c, double* acc_re, double* acc_im,
+                  hipStream_t stream) {
+  const dim3 mgrid((unsigned)((ntiles + kWave - 1) / kWave),
+                   (unsigned)((na + kMomentWaves - 1) / kMomentWaves));
+  hipLaunchKernelGGL((mf_tile_moments_kernel<NT>), mgrid, dim3(64 * kMomentWaves), 0, stream,
+                     real, imag, tx, rx, tile_info, xi_pow, a0, na, K, ntiles, pc, kc, coef,
+                     tau_c);
+  const double fcen = pc.f0 + kc * pc.dfs;
+  hipLaunchKernelGGL((mf_tile_eval_kernel<NT>), dim3((unsigned)ntiles), dim3(64 * kEvalWaves), 0,
+                     stream, tx, rx, voxel, tile_info, coef, tau_c, a0, na, N, ntiles, pc, fcen,
+                     acc_re, acc_im);
+}
+
+}  // namespace
+}  // namespace geraf
+
+using namespace geraf;
+
+extern "C" int64_t geraf_mf_tiled_workspace_bytes(int64_t A, int64_t K, int64_t N,
+                                                  int32_t nterms) {
+  const int NT = round_terms(nterms);
+  if (A < 0 || K < 0 || N < 0 || NT < 0) return -1;
+  const TiledLayout l = tiled_layout(K, N, NT);
+  return (int64_t)(l.fixed + 2 * 256 + (size_t)A * l.per_antenna);
+}
+
+extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
+                                          const float* tx_pos, const float* rx_pos,
+                                          float* out_real, float* out_imag, const float* voxel,
+                                          const float* tile_info, int64_t A, int64_t K,
+                                          int64_t N, int32_t nterms, void* workspace,
+                                          int64_t workspace_bytes,
+                                          const geraf_fmcw_params* params,
+                                          geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_params(params);
+  if (A < 1 || K < 1 || N < 0) throw ArgError("need A >= 1, K >= 1, N >= 0");
+  if (N == 0) return GERAF_RF_OK;
+  if (A * K >= (INT64_C(1) << 31) || N >= (INT64_C(1) << 31) - kTileVox)
+    throw ArgError("A*K and N must be < 2^31");
+  const int NT = round_terms(nterms);
+  if (nterms < 1 || NT < 0) throw ArgError("nterms must be in [1, 32]");
+  require_ptrs({real, imag, tx_pos, rx_pos, out_real, out_imag, voxel, tile_info, workspace},
+               true);
+  const TiledLayout l = tiled_layout(K, N, NT);
+  if (workspace_bytes < 0 || (size_t)workspace_bytes < l.fixed + 2 * 256 + l.per_antenna)
+    throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
+  const int ntiles = (int)((N + kTileVox - 1) / kTileVox);
+  const size_t base = align256((size_t)workspace) - (size_t)workspace;
+  char* ws = static_cast<char*>(workspace) + base;
+  double* acc_re = reinterpret_cast<double*>(ws);
+  double* acc_im = reinterpret_cast<double*>(ws + l.acc_bytes);
+  float* xi_pow = reinterpret_cast<float*>(ws + 2 * l.acc_bytes);
+  char* chunk = ws + 2 * l.acc_bytes + l.xi_bytes;
+  const size_t avail = (size_t)workspace_bytes - base - 2 * l.acc_bytes - l.xi_bytes - 256;
+  const int na_max = (int)std::min<int64_t>(A, (int64_t)(avail / l.per_antenna));
+  if (na_max < 1) throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
+  float2* coef = reinterpret_cast<float2*>(chunk);
+  double* tau_c = reinterpret_cast<double*>(
+      chunk + align256((size_t)na_max * ntiles * NT * sizeof(float2)));
+  if ((char*)(tau_c + (size_t)na_max * ntiles) > static_cast<char*>(workspace) + workspace_bytes)
+    throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
+
+  const hipStream_t st = (hipStream_t)stream;
+  const PhaseConsts pc = make_phase_consts(*params);
+  const double kc = 0.5 * (double)(K - 1);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((2 * l.acc_bytes / 8 + 255) / 256)),
+                     dim3(256), 0, st, acc_re, (long long)(2 * l.acc_bytes / 8));
+  hipLaunchKernelGGL(xi_pow_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, xi_pow,
+                     (int)K, NT, kc);
+  const float4* ti = reinterpret_cast<const float4*>(tile_info);
+  for (int a0 = 0; a0 < A; a0 += na_max) {
+    const int na = (int)std::min<int64_t>(na_max, A - a0);
+#define GERAF_TILED_CASE(T)                                                                    \
+  case T:                                                                                      \
+    launch_chunk<T>(real, imag, tx_pos, rx_pos, voxel, ti, xi_pow, a0, na, (int)K, (int)N,     \
+                    ntiles, pc, kc, coef, tau_c, acc_re, acc_im, st);                          \
+    break;
+    switch (NT) {
+      GERAF_TILED_CASE(4)
+      GERAF_TILED_CASE(6)
+      GERAF_TILED_CASE(8)
+      GERAF_TILED_CASE(10)
+      GERAF_TILED_CASE(12)
+      GERAF_TILED_CASE(14)
+      GERAF_TILED_CASE(16)
+      GERAF_TILED_CASE(20)
+      GERAF_TILED_CASE(24)
+      GERAF_TILED_CASE(32)
+      default: throw ArgError("unsupported term count");
+    }
+#undef GERAF_TILED_CASE
+  }
+  hipLaunchKernelGGL(mf_tile_finish_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+                     acc_re, acc_im, out_real, out_imag, (int)N, 1.0 / (double)A);
+  return check_launch("mf_tile_eval_kernel");
+  GERAF_TRY_END
+}

@@ -67,6 +67,8 @@ _SIGNATURES = {
     "geraf_bilinear_backward_f64": [_P] * 6 + [_I64] * 4 + [_P],
     "geraf_lensless_forward_f32": [_P] * 14 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
     "geraf_lensless_backward_f32": [_P] * 18 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
+    "geraf_mf_forward_tiled_f32": [_P] * 8 + [_I64] * 3 + [ctypes.c_int32, _P, _I64, _PARAMS,
+                                                            _P],
 }
 
 _lock = threading.Lock()
@@ -96,12 +98,15 @@ def load() -> ctypes.CDLL:
         lib.geraf_rf_last_error.argtypes = []
         lib.geraf_rf_version.restype = ctypes.c_char_p
         lib.geraf_rf_version.argtypes = []
+        lib.geraf_mf_tiled_workspace_bytes.restype = ctypes.c_int64
+        lib.geraf_mf_tiled_workspace_bytes.argtypes = [_I64, _I64, _I64, ctypes.c_int32]
         _lib = lib
     return _lib
 
 
 def exported_symbols():
-    return list(_SIGNATURES) + ["geraf_rf_last_error", "geraf_rf_version"]
+    return list(_SIGNATURES) + ["geraf_rf_last_error", "geraf_rf_version",
+                                "geraf_mf_tiled_workspace_bytes"]
 
 
 def version() -> str:
