@@ -77,12 +77,37 @@ def bench_tiled(dev, sc, re, im, tx, rx, sel, ref, direct_ms):
     nsc = len(np.unique(mf_scenes_nearest(sc)))
     peak = float(np.max(np.abs(np.abs(out[:nsc]) / np.abs(ref[:nsc]) - 1.0)))
     phase = float(np.max(np.abs(np.angle(out[:nsc] / ref[:nsc]))))
+    # the adjoint by the same expansion, against the direct adjoint kernel
+    from geraf.ops import _native
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import tiled_adjoint
+    plan = info["plan"]
+    a_r, a_i = tiled_adjoint(o_r, o_i, tx, rx, plan, sc.A, sc.K, sc.A, sc.fs, sc.As_sci)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        a_r, a_i = tiled_adjoint(o_r, o_i, tx, rx, plan, sc.A, sc.K, sc.A, sc.fs, sc.As_sci)
+    e1.record()
+    torch.cuda.synchronize()
+    adj_ms = e0.elapsed_time(e1) / reps
+    d_r, d_i = torch.empty_like(a_r), torch.empty_like(a_i)
+    vt = torch.tensor(np.ascontiguousarray(sc.voxels.T), device=dev)
+    _native.call("geraf_mf_backward_f32", *map(_native.ptr, (o_r, o_i, tx, rx, d_r, d_i, vt)),
+                 sc.A, sc.K, sc.N, sc.A,
+                 _native.make_params(sc.f0, sc.As_sci, sc.fs, sc.c, sc.bias),
+                 _native.stream_of(o_r))
+    torch.cuda.synchronize()
+    adj_rel = float((torch.complex(a_r, a_i) - torch.complex(d_r, d_i)).norm() /
+                    torch.complex(d_r, d_i).norm())
+    log(f"tiled adjoint: {adj_ms:.1f} ms, {terms / (adj_ms / 1e3):.3e} terms/s; rel-L2 vs the "
+        f"direct adjoint {adj_rel:.2e}")
     log(f"tiled expansion: {ms:.1f} ms ({info['terms']} terms, Z={info['Z']:.3f}), "
         f"{terms / (ms / 1e3):.3e} terms/s, {direct_ms / ms:.1f}x the direct kernel; rel-L2 vs "
         f"FP64 {rel:.3e}, peak {peak:.2e}, phase {phase:.2e} rad")
     return {"metric": "MF voxel*antenna*freq terms/s (per-tile expansion form)",
             "value": terms / (ms / 1e3), "unit": "terms/s", "ms_per_call": ms,
             "speedup_vs_direct": direct_ms / ms, "terms": info["terms"], "Z": info["Z"],
+            "adjoint": {"ms_per_call": adj_ms, "terms_per_s": terms / (adj_ms / 1e3),
+                        "rel_l2_vs_direct_adjoint": adj_rel},
             "truncation_tol": 1e-6, "errors_vs_fp64": {"volume_rel_l2": rel,
                                                       "peak_mag_err_max": peak,
                                                       "phase_err_max_rad": phase},

@@ -23,6 +23,15 @@
 // SGPR operands; per pair: FP64 distances and phase, one phasor, an N_T-term
 // real Horner with complex coefficients, Kahan accumulation), adding each
 // chunk's sums into FP64 accumulators; a last kernel scales by 1/A.
+//
+// The adjoint (MatchedFilter_cuda_kernel.cu:171-220) is the transpose of the
+// same expansion:
+//   g[a,k] = (1/norm) sum_t e^{+j2pi kappa d tau_ac} sum_n (j Z xi_k)^n / n! M_n(a,t),
+//   M_n(a,t) = sum_{v in t} gout_v e^{+j2pi(f0 + k_c d) tau_av} u_av^n,
+// computed by mf_tile_adj_moments_kernel (one wave per tile, eight voxels per
+// lane, a DPP wave sum per moment) and mf_tile_adj_eval_kernel (one wave per
+// antenna, consecutive frequencies per lane: a real Horner in Z xi_k per tile,
+// the phasor rotated along the lane's frequencies, Kahan over the tiles).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -212,6 +221,170 @@ __global__ void mf_tile_finish_kernel(const double* __restrict__ acc_re,
   out_im[v] = (float)(acc_im[v] * inv_a);
 }
 
+// ------------------------------------------------------------------ adjoint
+
+constexpr int kAdjVPL = kTileVox / kWave;  // 8 voxels per lane: one wave per tile
+
+template <int NT>
+__global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
+    const float* __restrict__ gout_re, const float* __restrict__ gout_im,
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    const float* __restrict__ voxel, const float4* __restrict__ tile_info, int a0, int na,
+    int N, int ntiles, PhaseConsts pc, double fcen, float2* __restrict__ mom,
+    double* __restrict__ tau_c) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = blockIdx.x * 4 + wave;
+  if (t >= ntiles) return;  // wave-uniform
+  const TileGeom g = tile_geom(tile_info, t, pc.inv_c);
+  const double invH = g.H > 0.0 ? 1.0 / g.H : 0.0;
+  double px[kAdjVPL], py[kAdjVPL], pz[kAdjVPL];
+  float cr[kAdjVPL], ci[kAdjVPL];
+#pragma unroll
+  for (int i = 0; i < kAdjVPL; ++i) {
+    const int v = t * kTileVox + i * kWave + lane;
+    const bool ok = v < N;
+    const int vc = ok ? v : N - 1;
+    px[i] = voxel[vc];
+    py[i] = voxel[(size_t)N + vc];
+    pz[i] = voxel[2 * (size_t)N + vc];
+    cr[i] = ok ? gout_re[vc] : 0.0f;
+    ci[i] = ok ? gout_im[vc] : 0.0f;
+  }
+  for (int ai = 0; ai < na; ++ai) {
+    const int a = a0 + ai;
+    const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+    const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+    const double tac = (dist3_f64_fast(tx, ty, tz, g.cx, g.cy, g.cz) +
+                        dist3_f64_fast(rx, ry, rz, g.cx, g.cy, g.cz) + pc.bias) *
+                       pc.inv_c;
+    float Pr[NT], Pi[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) Pr[n] = Pi[n] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kAdjVPL; ++i) {
+      const double tau = (dist3_f64_fast(tx, ty, tz, px[i], py[i], pz[i]) +
+                          dist3_f64_fast(rx, ry, rz, px[i], py[i], pz[i]) + pc.bias) *
+                         pc.inv_c;
+      const float u = (float)((tau - tac) * invH);
+      float2 p = cmul(make_float2(cr[i], ci[i]),
+                      phasor_cycles((float)centered_frac(fcen * tau), 1.0f));
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        Pr[n] += p.x;
+        Pi[n] += p.y;
+        p.x *= u;
+        p.y *= u;
+      }
+    }
+    // M'_n = j^n / n! * sum over the tile
+    float2* __restrict__ out = mom + ((size_t)ai * ntiles + t) * NT;
+    float inv_fact = 1.0f;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const float mr = wave_total_dpp(Pr[n]) * inv_fact;
+      const float mi = wave_total_dpp(Pi[n]) * inv_fact;
+      float2 c;
+      switch (n & 3) {
+        case 0: c = make_float2(mr, mi); break;
+        case 1: c = make_float2(-mi, mr); break;
+        case 2: c = make_float2(-mr, -mi); break;
+        default: c = make_float2(mi, -mr); break;
+      }
+      if (lane == 0) out[n] = c;
+      inv_fact /= (float)(n + 1);
+    }
+    if (lane == 0) tau_c[(size_t)ai * ntiles + t] = tac;
+  }
+}
+
+template <int NT, int KPL>
+__global__ __launch_bounds__(256) void mf_tile_adj_eval_kernel(
+    const float2* __restrict__ mom, const double* __restrict__ tau_c,
+    const float4* __restrict__ tile_info, int a0, int na, int K, int ntiles, PhaseConsts pc,
+    double kc, double inv_norm, float* __restrict__ grad_re, float* __restrict__ grad_im) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ai = blockIdx.x * 4 + wave;
+  if (ai >= na) return;  // wave-uniform
+  const int a = a0 + ai;
+  const int kb = lane * KPL;
+  float xk[KPL];
+#pragma unroll
+  for (int j = 0; j < KPL; ++j) xk[j] = kc > 0.0 ? (float)(((double)(kb + j) - kc) / kc) : 0.0f;
+  KahanC acc[KPL];
+#pragma unroll
+  for (int j = 0; j < KPL; ++j) acc[j].init();
+  const double zfac = 3.14159265358979323846 * (double)(K - 1) * pc.dfs;
+  for (int t = 0; t < ntiles; ++t) {
+    const size_t ct = __builtin_amdgcn_readfirstlane((unsigned)ai) * (size_t)ntiles + t;
+    const double tac = tau_c[ct];
+    const float Z = (float)(zfac * 2.0 * (double)tile_info[t].w * pc.inv_c);
+    const float2* __restrict__ m = mom + ct * NT;
+    const double step = pc.dfs * tac;
+    float2 ph = phasor_cycles((float)centered_frac(((double)kb - kc) * step), 1.0f);
+    const float2 w = phasor_cycles((float)centered_frac(step), 1.0f);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+      const float x = Z * xk[j];
+      float Sr = m[NT - 1].x, Si = m[NT - 1].y;
+#pragma unroll
+      for (int n = NT - 2; n >= 0; --n) {
+        Sr = fmaf(Sr, x, m[n].x);
+        Si = fmaf(Si, x, m[n].y);
+      }
+      acc[j].add(cmul(ph, make_float2(Sr, Si)));
+      ph = cmul(ph, w);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KPL; ++j) {
+    const int k = kb + j;
+    if (k < K) {
+      grad_re[(size_t)a * K + k] = (float)(acc[j].re() * inv_norm);
+      grad_im[(size_t)a * K + k] = (float)(acc[j].im() * inv_norm);
+    }
+  }
+}
+
+template <int NT, int KPL>
+void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const float* rx,
+                      const float* voxel, const float4* tile_info, int a0, int na, int K, int N,
+                      int ntiles, const PhaseConsts& pc, double kc, double inv_norm,
+                      float2* mom, double* tau_c, float* grad_re, float* grad_im,
+                      hipStream_t stream) {
+  const double fcen = pc.f0 + kc * pc.dfs;
+  hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT>), dim3((unsigned)((ntiles + 3) / 4)),
+                     dim3(256), 0, stream, gre, gim, tx, rx, voxel, tile_info, a0, na, N, ntiles,
+                     pc, fcen, mom, tau_c);
+  hipLaunchKernelGGL((mf_tile_adj_eval_kernel<NT, KPL>), dim3((unsigned)((na + 3) / 4)),
+                     dim3(256), 0, stream, mom, tau_c, tile_info, a0, na, K, ntiles, pc, kc,
+                     inv_norm, grad_re, grad_im);
+}
+
+template <int NT>
+void launch_adj_chunk_k(int kpl, const float* gre, const float* gim, const float* tx,
+                        const float* rx, const float* voxel, const float4* tile_info, int a0,
+                        int na, int K, int N, int ntiles, const PhaseConsts& pc, double kc,
+                        double inv_norm, float2* mom, double* tau_c, float* grad_re,
+                        float* grad_im, hipStream_t stream) {
+#define GERAF_ADJ_K(P)                                                                        \
+  case P:                                                                                     \
+    launch_adj_chunk<NT, P>(gre, gim, tx, rx, voxel, tile_info, a0, na, K, N, ntiles, pc, kc, \
+                            inv_norm, mom, tau_c, grad_re, grad_im, stream);                  \
+    break;
+  switch (kpl) {
+    GERAF_ADJ_K(1)
+    GERAF_ADJ_K(2)
+    GERAF_ADJ_K(4)
+    GERAF_ADJ_K(8)
+    GERAF_ADJ_K(16)
+    GERAF_ADJ_K(32)
+    default: throw ArgError("too many samples per antenna for the tiled adjoint");
+  }
+#undef GERAF_ADJ_K
+}
+
 // Instantiated term counts; a request is rounded up to the next one.
 constexpr int kTermSets[] = {4, 6, 8, 10, 12, 14, 16, 20, 24, 32};
 
@@ -259,6 +432,43 @@ void launch_chunk(const float* real, const float* imag, const float* tx, const f
 
 using namespace geraf;
 
+namespace geraf {
+namespace {
+// Per-chunk buffers of the tiled kernels inside the caller's workspace.
+struct TiledBuffers {
+  double* acc_re;
+  double* acc_im;
+  float* xi_pow;
+  float2* coef;
+  double* tau_c;
+  int na_max;
+};
+
+TiledBuffers carve(void* workspace, int64_t workspace_bytes, const TiledLayout& l, int64_t A,
+                   int ntiles, int NT) {
+  if (workspace_bytes < 0 || (size_t)workspace_bytes < l.fixed + 2 * 256 + l.per_antenna)
+    throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
+  const size_t base = align256((size_t)workspace) - (size_t)workspace;
+  char* ws = static_cast<char*>(workspace) + base;
+  TiledBuffers b;
+  b.acc_re = reinterpret_cast<double*>(ws);
+  b.acc_im = reinterpret_cast<double*>(ws + l.acc_bytes);
+  b.xi_pow = reinterpret_cast<float*>(ws + 2 * l.acc_bytes);
+  char* chunk = ws + 2 * l.acc_bytes + l.xi_bytes;
+  const size_t avail = (size_t)workspace_bytes - base - 2 * l.acc_bytes - l.xi_bytes - 256;
+  b.na_max = (int)std::min<int64_t>(A, (int64_t)(avail / l.per_antenna));
+  if (b.na_max < 1) throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
+  b.coef = reinterpret_cast<float2*>(chunk);
+  b.tau_c = reinterpret_cast<double*>(
+      chunk + align256((size_t)b.na_max * ntiles * NT * sizeof(float2)));
+  if ((char*)(b.tau_c + (size_t)b.na_max * ntiles) >
+      static_cast<char*>(workspace) + workspace_bytes)
+    throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
+  return b;
+}
+}  // namespace
+}  // namespace geraf
+
 extern "C" int64_t geraf_mf_tiled_workspace_bytes(int64_t A, int64_t K, int64_t N,
                                                   int32_t nterms) {
   const int NT = round_terms(nterms);
@@ -286,23 +496,14 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
   require_ptrs({real, imag, tx_pos, rx_pos, out_real, out_imag, voxel, tile_info, workspace},
                true);
   const TiledLayout l = tiled_layout(K, N, NT);
-  if (workspace_bytes < 0 || (size_t)workspace_bytes < l.fixed + 2 * 256 + l.per_antenna)
-    throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
   const int ntiles = (int)((N + kTileVox - 1) / kTileVox);
-  const size_t base = align256((size_t)workspace) - (size_t)workspace;
-  char* ws = static_cast<char*>(workspace) + base;
-  double* acc_re = reinterpret_cast<double*>(ws);
-  double* acc_im = reinterpret_cast<double*>(ws + l.acc_bytes);
-  float* xi_pow = reinterpret_cast<float*>(ws + 2 * l.acc_bytes);
-  char* chunk = ws + 2 * l.acc_bytes + l.xi_bytes;
-  const size_t avail = (size_t)workspace_bytes - base - 2 * l.acc_bytes - l.xi_bytes - 256;
-  const int na_max = (int)std::min<int64_t>(A, (int64_t)(avail / l.per_antenna));
-  if (na_max < 1) throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
-  float2* coef = reinterpret_cast<float2*>(chunk);
-  double* tau_c = reinterpret_cast<double*>(
-      chunk + align256((size_t)na_max * ntiles * NT * sizeof(float2)));
-  if ((char*)(tau_c + (size_t)na_max * ntiles) > static_cast<char*>(workspace) + workspace_bytes)
-    throw ArgError("workspace too small (see geraf_mf_tiled_workspace_bytes)");
+  const TiledBuffers b = carve(workspace, workspace_bytes, l, A, ntiles, NT);
+  double* acc_re = b.acc_re;
+  double* acc_im = b.acc_im;
+  float* xi_pow = b.xi_pow;
+  float2* coef = b.coef;
+  double* tau_c = b.tau_c;
+  const int na_max = b.na_max;
 
   const hipStream_t st = (hipStream_t)stream;
   const PhaseConsts pc = make_phase_consts(*params);
@@ -337,5 +538,63 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
   hipLaunchKernelGGL(mf_tile_finish_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
                      acc_re, acc_im, out_real, out_imag, (int)N, 1.0 / (double)A);
   return check_launch("mf_tile_eval_kernel");
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_mf_backward_tiled_f32(const float* grad_out_real, const float* grad_out_imag,
+                                           const float* tx_pos, const float* rx_pos,
+                                           float* grad_real, float* grad_imag,
+                                           const float* voxel, const float* tile_info,
+                                           int64_t A, int64_t K, int64_t N, int64_t norm_count,
+                                           int32_t nterms, void* workspace,
+                                           int64_t workspace_bytes,
+                                           const geraf_fmcw_params* params,
+                                           geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_params(params);
+  if (A < 1 || K < 1 || N < 1) throw ArgError("need A >= 1, K >= 1, N >= 1");
+  if (norm_count < 1) throw ArgError("norm_count must be >= 1");
+  if (A * K >= (INT64_C(1) << 31) || N >= (INT64_C(1) << 31) - kTileVox)
+    throw ArgError("A*K and N must be < 2^31");
+  const int NT = round_terms(nterms);
+  if (nterms < 1 || NT < 0) throw ArgError("nterms must be in [1, 32]");
+  int kpl = 1;
+  while (kpl * kWave < K) kpl *= 2;
+  if (kpl > 32) throw ArgError("the tiled adjoint takes at most 2048 samples per antenna");
+  require_ptrs({grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag, voxel,
+                tile_info, workspace},
+               true);
+  const TiledLayout l = tiled_layout(K, N, NT);
+  const int ntiles = (int)((N + kTileVox - 1) / kTileVox);
+  const TiledBuffers b = carve(workspace, workspace_bytes, l, A, ntiles, NT);
+  const hipStream_t st = (hipStream_t)stream;
+  const PhaseConsts pc = make_phase_consts(*params);
+  const double kc = 0.5 * (double)(K - 1);
+  const float4* ti = reinterpret_cast<const float4*>(tile_info);
+  const double inv_norm = 1.0 / (double)norm_count;
+  for (int a0 = 0; a0 < A; a0 += b.na_max) {
+    const int na = (int)std::min<int64_t>(b.na_max, A - a0);
+#define GERAF_TILED_ADJ(T)                                                                   \
+  case T:                                                                                    \
+    launch_adj_chunk_k<T>(kpl, grad_out_real, grad_out_imag, tx_pos, rx_pos, voxel, ti, a0,  \
+                          na, (int)K, (int)N, ntiles, pc, kc, inv_norm, b.coef, b.tau_c,     \
+                          grad_real, grad_imag, st);                                         \
+    break;
+    switch (NT) {
+      GERAF_TILED_ADJ(4)
+      GERAF_TILED_ADJ(6)
+      GERAF_TILED_ADJ(8)
+      GERAF_TILED_ADJ(10)
+      GERAF_TILED_ADJ(12)
+      GERAF_TILED_ADJ(14)
+      GERAF_TILED_ADJ(16)
+      GERAF_TILED_ADJ(20)
+      GERAF_TILED_ADJ(24)
+      GERAF_TILED_ADJ(32)
+      default: throw ArgError("unsupported term count");
+    }
+#undef GERAF_TILED_ADJ
+  }
+  return check_launch("mf_tile_adj_eval_kernel");
   GERAF_TRY_END
 }

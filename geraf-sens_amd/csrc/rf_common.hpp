@@ -219,6 +219,16 @@ __device__ __forceinline__ float wave_incl_prod_dpp(float v) {
   return p;
 }
 
+// Sum over the 64 lanes, returned wave-uniform: DPP sums within each row of
+// 16 lanes, then the four row totals through readlane (fixed order).
+__device__ __forceinline__ float wave_total_dpp(float v) {
+  v += dpp_mov<dpp::kQuadXor1>(0.0f, v);
+  v += dpp_mov<dpp::kQuadXor2>(0.0f, v);
+  v += dpp_mov<dpp::kRowHalfMirror>(0.0f, v);
+  v += dpp_mov<dpp::kRowMirror>(0.0f, v);
+  return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+}
+
 // Suffix composition of the affine maps W_l = b_l + m_l * W_{l+1} over the
 // wave, with W_64 = tail.  Returns W_l; *w_next = W_{l+1} and *head = W_0.
 // Rows of 16 are scanned with row_shl steps; the four row results are joined

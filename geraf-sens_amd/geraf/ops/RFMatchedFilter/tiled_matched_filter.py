@@ -13,7 +13,8 @@ expansion terms N_T from the largest tile:
 which bounds the truncation error of every antenna's contribution by
 tol * sum_k |s[a,k]|.  When N_T would exceed 32 (tiles too large against the
 range resolution) or K < 2 it runs the direct kernel instead.  The backward is
-the direct adjoint (the same as ``matched_filter_cuda``'s).
+the adjoint by the same expansion (same bound) when the forward ran tiled and
+K <= 2048, else the direct adjoint.
 """
 from __future__ import annotations
 
@@ -93,41 +94,71 @@ def expansion_terms(Z: float, tol: float) -> int:
     return n
 
 
+class TilePlan:
+    """Morton order, tile bounds and term count of one voxel set."""
+
+    def __init__(self, voxel_nx3: torch.Tensor, K: int, prm, tol: float):
+        self.N = voxel_nx3.shape[0]
+        self.perm = morton_order(voxel_nx3)
+        vs = voxel_nx3[self.perm].contiguous()
+        self.info = tile_bounds(vs)
+        self.vt = vs.t().contiguous()
+        r_max = float(self.info[:, 3].max())
+        self.Z = math.pi * (K - 1) * (prm.slope / prm.fs) * 2.0 * r_max / prm.c
+        self.terms = expansion_terms(self.Z, tol)
+        self.usable = K >= 2 and self.terms <= MAX_TERMS
+
+    def workspace(self, A: int, K: int, device) -> Tuple[torch.Tensor, int]:
+        lib = _native.load()
+        need = int(lib.geraf_mf_tiled_workspace_bytes(A, K, self.N, self.terms))
+        per_one = int(lib.geraf_mf_tiled_workspace_bytes(1, K, self.N, self.terms))
+        size = max(per_one, min(need, WORKSPACE_CAP))
+        return torch.empty(size, dtype=torch.uint8, device=device), size
+
+
 def tiled_forward(real, image, tpos, rpos, voxel_nx3, numADCSample: int, adcSampleRate: float,
-                  As_sci: float, tol: float = 1e-6):
+                  As_sci: float, tol: float = 1e-6, plan: "TilePlan" = None):
     """(out_real, out_imag, info) for float32 inputs on one device; info says
-    which path ran and with how many terms."""
+    which path ran and with how many terms (and carries the plan)."""
     A, K = real.shape
     N = voxel_nx3.shape[0]
     out_r = torch.empty(N, dtype=torch.float32, device=real.device)
     out_i = torch.empty_like(out_r)
     if N == 0:
-        return out_r, out_i, {"path": "tiled", "terms": 0, "Z": 0.0}
+        return out_r, out_i, {"path": "tiled", "terms": 0, "Z": 0.0, "plan": None}
     prm = _C._params(adcSampleRate, As_sci)
-    perm = morton_order(voxel_nx3)
-    vs = voxel_nx3[perm].contiguous()
-    info = tile_bounds(vs)
-    r_max = float(info[:, 3].max())
-    Z = math.pi * (K - 1) * (prm.slope / prm.fs) * 2.0 * r_max / prm.c
-    nt = expansion_terms(Z, tol)
-    if K < 2 or nt > MAX_TERMS:
+    plan = plan or TilePlan(voxel_nx3, K, prm, tol)
+    if not plan.usable:
         o_r, o_i = matched_filter_cuda(real, image, tpos, rpos, voxel_nx3, A, K, adcSampleRate,
                                        As_sci, N)
-        return o_r, o_i, {"path": "direct", "terms": nt, "Z": Z}
-    lib = _native.load()
-    need = int(lib.geraf_mf_tiled_workspace_bytes(A, K, N, nt))
-    per_one = int(lib.geraf_mf_tiled_workspace_bytes(1, K, N, nt))
-    size = max(per_one, min(need, WORKSPACE_CAP))
-    ws = torch.empty(size, dtype=torch.uint8, device=real.device)
-    vt = vs.t().contiguous()
+        return o_r, o_i, {"path": "direct", "terms": plan.terms, "Z": plan.Z, "plan": plan}
+    ws, size = plan.workspace(A, K, real.device)
     s_r = torch.empty(N, dtype=torch.float32, device=real.device)
     s_i = torch.empty_like(s_r)
     P = _native.ptr
     _native.call("geraf_mf_forward_tiled_f32", P(real), P(image), P(tpos), P(rpos), P(s_r),
-                 P(s_i), P(vt), P(info), A, K, N, nt, P(ws), size, prm, _native.stream_of(real))
-    out_r[perm] = s_r
-    out_i[perm] = s_i
-    return out_r, out_i, {"path": "tiled", "terms": nt, "Z": Z}
+                 P(s_i), P(plan.vt), P(plan.info), A, K, N, plan.terms, P(ws), size, prm,
+                 _native.stream_of(real))
+    out_r[plan.perm] = s_r
+    out_i[plan.perm] = s_i
+    return out_r, out_i, {"path": "tiled", "terms": plan.terms, "Z": plan.Z, "plan": plan}
+
+
+def tiled_adjoint(g_r, g_i, tpos, rpos, plan: "TilePlan", A: int, K: int, norm_count: int,
+                  adcSampleRate: float, As_sci: float):
+    """grad [A, K] (real, imag) of the matched filter by the tile expansion;
+    g_r/g_i [N] in the caller's voxel order."""
+    prm = _C._params(adcSampleRate, As_sci)
+    gr = g_r.float()[plan.perm].contiguous()
+    gi = g_i.float()[plan.perm].contiguous()
+    out_r = torch.empty((A, K), dtype=torch.float32, device=gr.device)
+    out_i = torch.empty_like(out_r)
+    ws, size = plan.workspace(A, K, gr.device)
+    P = _native.ptr
+    _native.call("geraf_mf_backward_tiled_f32", P(gr), P(gi), P(tpos), P(rpos), P(out_r),
+                 P(out_i), P(plan.vt), P(plan.info), A, K, plan.N, norm_count, plan.terms,
+                 P(ws), size, prm, _native.stream_of(gr))
+    return out_r, out_i
 
 
 class _TiledMatchedFilter(Function):
@@ -140,9 +171,10 @@ class _TiledMatchedFilter(Function):
             f"but got: {tuple(real.shape)}")
         assert voxel.shape == (N, 3), f"expected voxel ({N}, 3), got {tuple(voxel.shape)}"
         args = [t.float().contiguous() for t in (real, image, tpos, rpos, voxel)]
-        o_r, o_i, _ = tiled_forward(*args, numADCSample, adcSampleRate, As_sci, tol)
+        o_r, o_i, info = tiled_forward(*args, numADCSample, adcSampleRate, As_sci, tol)
         ctx.save_for_backward(*args)
         ctx.cfg = (total_ant_size, numADCSample, adcSampleRate, As_sci, N)
+        ctx.plan = info["plan"] if info["path"] == "tiled" and numADCSample <= 2048 else None
         ctx.dtype = real.dtype
         return o_r.to(real.dtype), o_i.to(real.dtype)
 
@@ -153,11 +185,15 @@ class _TiledMatchedFilter(Function):
         A, K, fs, As_sci, N = ctx.cfg
         grad_real = grad_image = None
         if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
-            grad_real = torch.empty_like(real)
-            grad_image = torch.empty_like(image)
-            _C.MatchedFilter_backward_realimag(
-                g_r.float().contiguous(), g_i.float().contiguous(), real, image, tpos, rpos,
-                grad_real, grad_image, voxel.t().contiguous(), A, K, fs, As_sci, N)
+            if ctx.plan is not None:
+                grad_real, grad_image = tiled_adjoint(g_r, g_i, tpos, rpos, ctx.plan, A, K, A,
+                                                      fs, As_sci)
+            else:
+                grad_real = torch.empty_like(real)
+                grad_image = torch.empty_like(image)
+                _C.MatchedFilter_backward_realimag(
+                    g_r.float().contiguous(), g_i.float().contiguous(), real, image, tpos, rpos,
+                    grad_real, grad_image, voxel.t().contiguous(), A, K, fs, As_sci, N)
             grad_real, grad_image = grad_real.to(ctx.dtype), grad_image.to(ctx.dtype)
         return (grad_real, grad_image) + (None,) * 9
 

@@ -69,6 +69,8 @@ _SIGNATURES = {
     "geraf_lensless_backward_f32": [_P] * 18 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
     "geraf_mf_forward_tiled_f32": [_P] * 8 + [_I64] * 3 + [ctypes.c_int32, _P, _I64, _PARAMS,
                                                             _P],
+    "geraf_mf_backward_tiled_f32": [_P] * 8 + [_I64] * 4 + [ctypes.c_int32, _P, _I64, _PARAMS,
+                                                             _P],
 }
 
 _lock = threading.Lock()

@@ -402,3 +402,42 @@ def test_tiled_op_gradient_is_the_adjoint(cuda):
         outs.append((yr.detach(), yi.detach(), re.grad, im.grad))
     for a, b in zip(*outs):
         assert float((a - b).norm() / b.norm()) < 1e-5
+
+
+@pytest.mark.parametrize("which", ["cfg1", "cfg2_block"])
+def test_tiled_adjoint_matches_direct_and_is_the_transpose(cuda, which):
+    from geraf.ops import _native
+    from geraf.ops.RFMatchedFilter import _C as mfC
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import (TilePlan, tiled_adjoint,
+                                                                tiled_forward)
+    if which == "cfg1":
+        sc = mf_scenes.config1(0)
+        vox = sc.voxels
+    else:
+        sc = mf_scenes.config2(0, device=str(cuda))
+        vox = sc.voxels.reshape(128, 128, 128, 3)[50:74, 40:64, 60:84].reshape(-1, 3)
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda)  # noqa: E731
+    A, K, N = sc.A, sc.K, vox.shape[0]
+    re, im, tx, rx, v = t(sc.echo.real), t(sc.echo.imag), t(sc.tx), t(sc.rx), t(vox)
+    rng = np.random.default_rng(5)
+    g_r = t(rng.standard_normal(N).astype(np.float32))
+    g_i = t(rng.standard_normal(N).astype(np.float32))
+    prm = mfC._params(sc.fs, sc.As_sci)
+    plan = TilePlan(v, K, prm, 1e-6)
+    assert plan.usable
+    a_r, a_i = tiled_adjoint(g_r, g_i, tx, rx, plan, A, K, A, sc.fs, sc.As_sci)
+    d_r, d_i = torch.empty((A, K), device=cuda), torch.empty((A, K), device=cuda)
+    vt = v.t().contiguous()
+    _native.call("geraf_mf_backward_f32", *map(_native.ptr, (g_r, g_i, tx, rx, d_r, d_i, vt)),
+                 A, K, N, A, _native.make_params(sc.f0, sc.As_sci, sc.fs, sc.c, sc.bias),
+                 _native.stream_of(g_r))
+    got, ref = torch.complex(a_r.double(), a_i.double()), torch.complex(d_r.double(), d_i.double())
+    rel = float((got - ref).norm() / ref.norm())
+    # <MF s, g> = <s, MF^T g> for the tiled pair (MF^T divided by A like MF)
+    f_r, f_i, _ = tiled_forward(re, im, tx, rx, v, K, sc.fs, sc.As_sci, 1e-6, plan=plan)
+    lhs = float((f_r.double() * g_r.double() + f_i.double() * g_i.double()).sum())
+    rhs = float((re.double() * a_r.double() + im.double() * a_i.double()).sum())
+    print(f"{which}: tiled adjoint vs direct rel-L2 {rel:.3e} ({plan.terms} terms); "
+          f"<MF s,g>={lhs:.6e} <s,MF^T g>={rhs:.6e}", flush=True)
+    assert rel <= 1e-5
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + abs(rhs))
