@@ -81,11 +81,11 @@ def bench_tiled(dev, sc, re, im, tx, rx, sel, ref, direct_ms):
     from geraf.ops import _native
     from geraf.ops.RFMatchedFilter.tiled_matched_filter import tiled_adjoint
     plan = info["plan"]
-    a_r, a_i = tiled_adjoint(o_r, o_i, tx, rx, plan, sc.A, sc.K, sc.A, sc.fs, sc.As_sci)
+    a_r, a_i = tiled_adjoint(o_r, o_i, tx, rx, plan, sc.A, sc.K, sc.A, sc.fs, sc.As_sci, vox)
     torch.cuda.synchronize()
     e0.record()
     for _ in range(reps):
-        a_r, a_i = tiled_adjoint(o_r, o_i, tx, rx, plan, sc.A, sc.K, sc.A, sc.fs, sc.As_sci)
+        a_r, a_i = tiled_adjoint(o_r, o_i, tx, rx, plan, sc.A, sc.K, sc.A, sc.fs, sc.As_sci, vox)
     e1.record()
     torch.cuda.synchronize()
     adj_ms = e0.elapsed_time(e1) / reps
@@ -196,7 +196,11 @@ def bench_stage_step(dev, target_volume, axes, steps: int = 5, warmup: int = 2):
         ms = 1e3 * (time.perf_counter() - t0) / steps
         samples = int(step.valid.sum())
         out[form] = {"ms_per_step": ms, "valid_samples_per_step": samples,
-                     "samples_per_s": samples / ms * 1e3, "last_loss": loss}
+                     "samples_per_s": samples / ms * 1e3, "last_loss": loss,
+                     "matched_filter": step.mf_algo,
+                     "mf_tiled_fraction": step.plan.tiled_fraction if step.plan else 0.0,
+                     "mf_bucket_terms": [b.terms for b in step.plan.buckets] if step.plan
+                     else None}
         log(f"stage step [{form}]: {ms:.2f} ms/step, {samples} valid samples, loss {loss:.4f}")
         del step
         torch.cuda.empty_cache()
@@ -237,8 +241,38 @@ def bench_large_target(dev):
     rate = A * K * N / (ms / 1e3)
     log(f"256^3 target slab ({N} voxels, A={A}, K={K}): {ms:.1f} ms, {rate:.3e} terms/s; "
         f"full volume {A * K * 256**3 / rate:.1f} s")
-    return {"workload": "MF target 256^3 (128x128 aperture, K=1024), 256x256x4 slab",
-            "terms_per_s": rate, "slab_ms": ms, "full_volume_s": A * K * 256 ** 3 / rate}
+    res = {"workload": "MF target 256^3 (128x128 aperture, K=1024), 256x256x4 slab",
+           "terms_per_s": rate, "slab_ms": ms, "full_volume_s": A * K * 256 ** 3 / rate}
+
+    # the whole 256^3 volume by the per-tile expansion form, checked against
+    # the direct kernel on the slab
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import tiled_forward
+
+    lin = np.linspace(-0.10, 0.10, 256)  # the grid of mf_scenes.voxel_grid(256, 0.20, centre)
+    ax = [torch.tensor((lin + c).astype(np.float32), device=dev) for c in (0.0, 0.0, 0.30)]
+    gx, gy, gz = torch.meshgrid(*ax, indexing="ij")
+    full = torch.stack([gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)], dim=1)
+    del gx, gy, gz
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f_r, f_i, info = tiled_forward(args[0], args[1], args[2], args[3], full, K, sc.fs, sc.As_sci,
+                                   1e-6)
+    torch.cuda.synchronize()
+    full_s = time.perf_counter() - t0
+    slab_idx = torch.arange(256 * 256, device=dev) * 256  # iz = 0 plane of the full grid
+    sl = torch.stack([slab_idx + z for z in range(4)], dim=1).reshape(-1)
+    got = torch.complex(f_r[sl].double(), f_i[sl].double())
+    ref = torch.complex(o_r.double(), o_i.double())
+    rel = float((got - ref).norm() / ref.norm())
+    terms_full = float(A) * K * 256 ** 3
+    log(f"256^3 target, whole volume by tile expansion: {full_s:.2f} s ({info['terms']} terms, "
+        f"Z={info['Z']:.3f}), {terms_full / full_s:.3e} terms/s; rel-L2 vs the direct kernel on "
+        f"the slab {rel:.2e}")
+    res["tiled_full_volume"] = {"seconds": full_s, "terms_per_s": terms_full / full_s,
+                                "terms": info["terms"], "Z": info["Z"],
+                                "rel_l2_vs_direct_on_slab": rel,
+                                "speedup_vs_direct": (A * K * 256 ** 3 / rate) / full_s}
+    return res
 
 
 def emitting_fraction(b) -> float:

@@ -155,8 +155,14 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
     const float* __restrict__ voxel, const float4* __restrict__ tile_info,
     const float2* __restrict__ coef, const double* __restrict__ tau_c, int a0, int na, int N,
     int ntiles, PhaseConsts pc, double fcen, double* __restrict__ acc_re,
-    double* __restrict__ acc_im) {
+    double* __restrict__ acc_im, int per_group, size_t acc_stride) {
   constexpr int VPL = 2;
+  // antenna group of this block: antennas [ai_lo, ai_hi) of the chunk, into
+  // its own accumulator slice
+  const int ai_lo = blockIdx.y * per_group;
+  const int ai_hi = min(na, ai_lo + per_group);
+  acc_re += blockIdx.y * acc_stride;
+  acc_im += blockIdx.y * acc_stride;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int t = blockIdx.x;
@@ -179,7 +185,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
 #pragma unroll
   for (int i = 0; i < VPL; ++i) acc[i].init();
 
-  for (int ai = 0; ai < na; ++ai) {
+  for (int ai = ai_lo; ai < ai_hi; ++ai) {
     const int a = a0 + ai;
     const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
     const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
@@ -214,11 +220,16 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
 __global__ void mf_tile_finish_kernel(const double* __restrict__ acc_re,
                                       const double* __restrict__ acc_im,
                                       float* __restrict__ out_re, float* __restrict__ out_im,
-                                      int N, double inv_a) {
+                                      int N, double inv_a, int groups, size_t acc_stride) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
-  out_re[v] = (float)(acc_re[v] * inv_a);
-  out_im[v] = (float)(acc_im[v] * inv_a);
+  double r = 0.0, i = 0.0;
+  for (int g = 0; g < groups; ++g) {
+    r += acc_re[g * acc_stride + v];
+    i += acc_im[g * acc_stride + v];
+  }
+  out_re[v] = (float)(r * inv_a);
+  out_im[v] = (float)(i * inv_a);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -231,10 +242,12 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
     const float* __restrict__ voxel, const float4* __restrict__ tile_info, int a0, int na,
     int N, int ntiles, PhaseConsts pc, double fcen, float2* __restrict__ mom,
-    double* __restrict__ tau_c) {
+    double* __restrict__ tau_c, int per_group) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int t = blockIdx.x * 4 + wave;
+  const int ai_lo = blockIdx.y * per_group;
+  const int ai_hi = min(na, ai_lo + per_group);
   if (t >= ntiles) return;  // wave-uniform
   const TileGeom g = tile_geom(tile_info, t, pc.inv_c);
   const double invH = g.H > 0.0 ? 1.0 / g.H : 0.0;
@@ -251,7 +264,7 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
     cr[i] = ok ? gout_re[vc] : 0.0f;
     ci[i] = ok ? gout_im[vc] : 0.0f;
   }
-  for (int ai = 0; ai < na; ++ai) {
+  for (int ai = ai_lo; ai < ai_hi; ++ai) {
     const int a = a0 + ai;
     const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
     const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
@@ -354,9 +367,15 @@ void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const
                       float2* mom, double* tau_c, float* grad_re, float* grad_im,
                       hipStream_t stream) {
   const double fcen = pc.f0 + kc * pc.dfs;
-  hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT>), dim3((unsigned)((ntiles + 3) / 4)),
-                     dim3(256), 0, stream, gre, gim, tx, rx, voxel, tile_info, a0, na, N, ntiles,
-                     pc, fcen, mom, tau_c);
+  // antenna ranges side by side when the tiles alone give fewer than ~8
+  // waves per SIMD (one wave per tile)
+  int g = 1;
+  while (g < 64 && g * 2 <= na && (int64_t)ntiles * g < 8192) g *= 2;
+  const int per_group = (na + g - 1) / g;
+  hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT>),
+                     dim3((unsigned)((ntiles + 3) / 4), (unsigned)g), dim3(256), 0, stream, gre,
+                     gim, tx, rx, voxel, tile_info, a0, na, N, ntiles, pc, fcen, mom, tau_c,
+                     per_group);
   hipLaunchKernelGGL((mf_tile_adj_eval_kernel<NT, KPL>), dim3((unsigned)((na + 3) / 4)),
                      dim3(256), 0, stream, mom, tau_c, tile_info, a0, na, K, ntiles, pc, kc,
                      inv_norm, grad_re, grad_im);
@@ -396,14 +415,26 @@ int round_terms(int nterms) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Antenna groups run side by side when the tiles alone would give fewer than
+// ~8 waves per SIMD (4 waves per tile block); each group has its own FP64
+// accumulator slice.
+int antenna_groups(int64_t ntiles) {
+  const int64_t want = 2048;  // tile blocks of 4 waves
+  int g = 1;
+  while (g < 16 && ntiles * g < want) g *= 2;
+  return g;
+}
+
 struct TiledLayout {
   size_t acc_bytes, xi_bytes, per_antenna, fixed;
+  int groups;
 };
 
 TiledLayout tiled_layout(int64_t K, int64_t N, int NT) {
   const int64_t ntiles = (N + kTileVox - 1) / kTileVox;
   TiledLayout l;
-  l.acc_bytes = align256((size_t)N * sizeof(double));
+  l.groups = antenna_groups(ntiles);
+  l.acc_bytes = align256((size_t)N * sizeof(double)) * l.groups;
   l.xi_bytes = align256((size_t)K * NT * sizeof(float));
   l.per_antenna = (size_t)ntiles * (NT * sizeof(float2) + sizeof(double));
   l.fixed = 2 * l.acc_bytes + l.xi_bytes + 2 * 256;
@@ -414,17 +445,19 @@ template <int NT>
 void launch_chunk(const float* real, const float* imag, const float* tx, const float* rx,
                   const float* voxel, const float4* tile_info, const float* xi_pow, int a0,
                   int na, int K, int N, int ntiles, const PhaseConsts& pc, double kc,
-                  float2* coef, double* tau_c, double* acc_re, double* acc_im,
-                  hipStream_t stream) {
+                  float2* coef, double* tau_c, double* acc_re, double* acc_im, int groups,
+                  size_t acc_stride, hipStream_t stream) {
   const dim3 mgrid((unsigned)((ntiles + kWave - 1) / kWave),
                    (unsigned)((na + kMomentWaves - 1) / kMomentWaves));
   hipLaunchKernelGGL((mf_tile_moments_kernel<NT>), mgrid, dim3(64 * kMomentWaves), 0, stream,
                      real, imag, tx, rx, tile_info, xi_pow, a0, na, K, ntiles, pc, kc, coef,
                      tau_c);
   const double fcen = pc.f0 + kc * pc.dfs;
-  hipLaunchKernelGGL((mf_tile_eval_kernel<NT>), dim3((unsigned)ntiles), dim3(64 * kEvalWaves), 0,
-                     stream, tx, rx, voxel, tile_info, coef, tau_c, a0, na, N, ntiles, pc, fcen,
-                     acc_re, acc_im);
+  const int g = std::min(groups, na);
+  const int per_group = (na + g - 1) / g;
+  hipLaunchKernelGGL((mf_tile_eval_kernel<NT>), dim3((unsigned)ntiles, (unsigned)g),
+                     dim3(64 * kEvalWaves), 0, stream, tx, rx, voxel, tile_info, coef, tau_c, a0,
+                     na, N, ntiles, pc, fcen, acc_re, acc_im, per_group, acc_stride);
 }
 
 }  // namespace
@@ -452,7 +485,7 @@ TiledBuffers carve(void* workspace, int64_t workspace_bytes, const TiledLayout& 
   char* ws = static_cast<char*>(workspace) + base;
   TiledBuffers b;
   b.acc_re = reinterpret_cast<double*>(ws);
-  b.acc_im = reinterpret_cast<double*>(ws + l.acc_bytes);
+  b.acc_im = reinterpret_cast<double*>(ws + l.acc_bytes);  // l.acc_bytes covers all groups
   b.xi_pow = reinterpret_cast<float*>(ws + 2 * l.acc_bytes);
   char* chunk = ws + 2 * l.acc_bytes + l.xi_bytes;
   const size_t avail = (size_t)workspace_bytes - base - 2 * l.acc_bytes - l.xi_bytes - 256;
@@ -508,6 +541,7 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
   const hipStream_t st = (hipStream_t)stream;
   const PhaseConsts pc = make_phase_consts(*params);
   const double kc = 0.5 * (double)(K - 1);
+  const size_t acc_stride = l.acc_bytes / l.groups / sizeof(double);  // doubles per slice
   hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((2 * l.acc_bytes / 8 + 255) / 256)),
                      dim3(256), 0, st, acc_re, (long long)(2 * l.acc_bytes / 8));
   hipLaunchKernelGGL(xi_pow_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, xi_pow,
@@ -518,7 +552,7 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
 #define GERAF_TILED_CASE(T)                                                                    \
   case T:                                                                                      \
     launch_chunk<T>(real, imag, tx_pos, rx_pos, voxel, ti, xi_pow, a0, na, (int)K, (int)N,     \
-                    ntiles, pc, kc, coef, tau_c, acc_re, acc_im, st);                          \
+                    ntiles, pc, kc, coef, tau_c, acc_re, acc_im, l.groups, acc_stride, st);    \
     break;
     switch (NT) {
       GERAF_TILED_CASE(4)
@@ -536,7 +570,8 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
 #undef GERAF_TILED_CASE
   }
   hipLaunchKernelGGL(mf_tile_finish_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
-                     acc_re, acc_im, out_real, out_imag, (int)N, 1.0 / (double)A);
+                     acc_re, acc_im, out_real, out_imag, (int)N, 1.0 / (double)A,
+                     l.groups, acc_stride);
   return check_launch("mf_tile_eval_kernel");
   GERAF_TRY_END
 }

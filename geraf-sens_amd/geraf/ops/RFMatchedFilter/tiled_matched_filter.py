@@ -5,15 +5,17 @@ MatchedFilter_cuda_kernel.cu:65-115, evaluated to a stated truncation bound).
 ``matched_filter_tiled`` takes the arguments of ``matched_filter_cuda`` plus a
 tolerance.  It orders the voxels along a Morton curve so that every 512
 consecutive voxels form a compact tile (exact 8x8x8 bricks when the voxels
-form a regular grid), measures each tile's radius, and picks the number of
-expansion terms N_T from the largest tile:
+form a regular grid), measures each tile's radius r_t, and gives each tile
+the smallest instantiated number of expansion terms N_T with
 
-    Z = pi (K-1) (slope/fs) 2 r_max / c,   Z^N_T / N_T! e^Z <= tol,
+    Z_t = pi (K-1) (slope/fs) 2 r_t / c,   Z_t^N_T / N_T! e^Z_t <= tol,
 
-which bounds the truncation error of every antenna's contribution by
-tol * sum_k |s[a,k]|.  When N_T would exceed 32 (tiles too large against the
-range resolution) or K < 2 it runs the direct kernel instead.  The backward is
-the adjoint by the same expansion (same bound) when the forward ran tiled and
+which bounds the truncation error of every antenna's contribution to the
+tile's voxels by tol * sum_k |s[a,k]|.  Tiles of one term count run in one
+launch sequence; tiles that would need more than 32 terms (too large against
+the range resolution) go through the direct kernel, as does everything when
+K < 2.  The backward is the adjoint by the same expansion (same bound, plus
+the direct adjoint for the direct voxels) when the forward ran tiled and
 K <= 2048, else the direct adjoint.
 """
 from __future__ import annotations
@@ -94,84 +96,178 @@ def expansion_terms(Z: float, tol: float) -> int:
     return n
 
 
+# Term counts the kernels are instantiated for.
+TERM_SETS = (4, 6, 8, 10, 12, 14, 16, 20, 24, 32)
+
+
+def _z_limit(nterms: int, tol: float) -> float:
+    """Largest Z with Z^nterms / nterms! * e^Z <= tol (bisection)."""
+    lo, hi = 0.0, 64.0
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        if mid ** nterms / math.factorial(nterms) * math.exp(mid) <= tol:
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
+class _Bucket:
+    """Tiles sharing one term count: slot i of the bucket holds voxel idx[i]
+    (padding slots repeat a voxel and are masked out)."""
+
+    def __init__(self, terms, idx, valid, voxel_nx3, info):
+        self.terms = terms
+        self.idx = idx
+        self.valid = valid
+        self.vt = voxel_nx3[idx].t().contiguous()
+        self.info = info.contiguous()
+        self.N = idx.numel()
+
+
 class TilePlan:
-    """Morton order, tile bounds and term count of one voxel set."""
+    """Morton order, tiles, and the tiles grouped by the term count each needs
+    (the smallest instantiated N_T with Z_t^N_T / N_T! e^Z_t <= tol); tiles
+    that would need more than 32 terms are left to the direct kernel."""
 
     def __init__(self, voxel_nx3: torch.Tensor, K: int, prm, tol: float):
-        self.N = voxel_nx3.shape[0]
-        self.perm = morton_order(voxel_nx3)
-        vs = voxel_nx3[self.perm].contiguous()
-        self.info = tile_bounds(vs)
-        self.vt = vs.t().contiguous()
-        r_max = float(self.info[:, 3].max())
-        self.Z = math.pi * (K - 1) * (prm.slope / prm.fs) * 2.0 * r_max / prm.c
+        dev = voxel_nx3.device
+        self.N = n = voxel_nx3.shape[0]
+        perm = morton_order(voxel_nx3)
+        nt = (n + TILE - 1) // TILE
+        pad = nt * TILE - n
+        slots = torch.cat([perm, perm[-1:].expand(pad)]) if pad else perm
+        valid = torch.ones(nt * TILE, dtype=torch.bool, device=dev)
+        if pad:
+            valid[n:] = False
+        info = tile_bounds(voxel_nx3[slots], TILE)
+        zfac = math.pi * (K - 1) * (prm.slope / prm.fs) * 2.0 / prm.c
+        z = info[:, 3].double() * zfac
+        self.Z = float(z.max()) if nt else 0.0
         self.terms = expansion_terms(self.Z, tol)
-        self.usable = K >= 2 and self.terms <= MAX_TERMS
+        self.buckets = []
+        direct = torch.ones(nt, dtype=torch.bool, device=dev)
+        lower = -1.0
+        if K >= 2:
+            for terms in TERM_SETS:
+                lim = _z_limit(terms, tol)
+                sel = (z > lower) & (z <= lim)
+                lower = lim
+                tiles = torch.nonzero(sel).reshape(-1)
+                if tiles.numel() == 0:
+                    continue
+                direct[tiles] = False
+                pos = (tiles[:, None] * TILE + torch.arange(TILE, device=dev)[None, :]).reshape(-1)
+                self.buckets.append(_Bucket(terms, slots[pos], valid[pos], voxel_nx3,
+                                            info[tiles]))
+        dpos = (torch.nonzero(direct).reshape(-1)[:, None] * TILE +
+                torch.arange(TILE, device=dev)[None, :]).reshape(-1)
+        self.direct_idx = slots[dpos][valid[dpos]]
+        self.usable = len(self.buckets) > 0
+        self.tiled_fraction = 1.0 - self.direct_idx.numel() / max(n, 1)
 
-    def workspace(self, A: int, K: int, device) -> Tuple[torch.Tensor, int]:
+    @staticmethod
+    def workspace(A: int, K: int, b: "_Bucket", device) -> Tuple[torch.Tensor, int]:
         lib = _native.load()
-        need = int(lib.geraf_mf_tiled_workspace_bytes(A, K, self.N, self.terms))
-        per_one = int(lib.geraf_mf_tiled_workspace_bytes(1, K, self.N, self.terms))
+        need = int(lib.geraf_mf_tiled_workspace_bytes(A, K, b.N, b.terms))
+        per_one = int(lib.geraf_mf_tiled_workspace_bytes(1, K, b.N, b.terms))
         size = max(per_one, min(need, WORKSPACE_CAP))
         return torch.empty(size, dtype=torch.uint8, device=device), size
+
+
+def _info(plan: "TilePlan", path: str):
+    return {"path": path, "terms": plan.terms, "Z": plan.Z, "plan": plan,
+            "tiled_fraction": plan.tiled_fraction,
+            "bucket_terms": [b.terms for b in plan.buckets]}
 
 
 def tiled_forward(real, image, tpos, rpos, voxel_nx3, numADCSample: int, adcSampleRate: float,
                   As_sci: float, tol: float = 1e-6, plan: "TilePlan" = None):
     """(out_real, out_imag, info) for float32 inputs on one device; info says
-    which path ran and with how many terms (and carries the plan)."""
+    which path ran, how many terms the largest tile needs, and which share of
+    the voxels went through the expansion (the rest through the direct
+    kernel)."""
     A, K = real.shape
     N = voxel_nx3.shape[0]
     out_r = torch.empty(N, dtype=torch.float32, device=real.device)
     out_i = torch.empty_like(out_r)
     if N == 0:
-        return out_r, out_i, {"path": "tiled", "terms": 0, "Z": 0.0, "plan": None}
+        return out_r, out_i, {"path": "tiled", "terms": 0, "Z": 0.0, "plan": None,
+                              "tiled_fraction": 1.0, "bucket_terms": []}
     prm = _C._params(adcSampleRate, As_sci)
     plan = plan or TilePlan(voxel_nx3, K, prm, tol)
     if not plan.usable:
         o_r, o_i = matched_filter_cuda(real, image, tpos, rpos, voxel_nx3, A, K, adcSampleRate,
                                        As_sci, N)
-        return o_r, o_i, {"path": "direct", "terms": plan.terms, "Z": plan.Z, "plan": plan}
-    ws, size = plan.workspace(A, K, real.device)
-    s_r = torch.empty(N, dtype=torch.float32, device=real.device)
-    s_i = torch.empty_like(s_r)
+        return o_r, o_i, _info(plan, "direct")
     P = _native.ptr
-    _native.call("geraf_mf_forward_tiled_f32", P(real), P(image), P(tpos), P(rpos), P(s_r),
-                 P(s_i), P(plan.vt), P(plan.info), A, K, N, plan.terms, P(ws), size, prm,
-                 _native.stream_of(real))
-    out_r[plan.perm] = s_r
-    out_i[plan.perm] = s_i
-    return out_r, out_i, {"path": "tiled", "terms": plan.terms, "Z": plan.Z, "plan": plan}
+    st = _native.stream_of(real)
+    for b in plan.buckets:
+        ws, size = TilePlan.workspace(A, K, b, real.device)
+        s_r = torch.empty(b.N, dtype=torch.float32, device=real.device)
+        s_i = torch.empty_like(s_r)
+        _native.call("geraf_mf_forward_tiled_f32", P(real), P(image), P(tpos), P(rpos), P(s_r),
+                     P(s_i), P(b.vt), P(b.info), A, K, b.N, b.terms, P(ws), size, prm, st)
+        out_r[b.idx[b.valid]] = s_r[b.valid]
+        out_i[b.idx[b.valid]] = s_i[b.valid]
+    if plan.direct_idx.numel():
+        v = voxel_nx3[plan.direct_idx]
+        d_r, d_i = matched_filter_cuda(real, image, tpos, rpos, v, A, K, adcSampleRate, As_sci,
+                                       v.shape[0])
+        out_r[plan.direct_idx] = d_r
+        out_i[plan.direct_idx] = d_i
+    return out_r, out_i, _info(plan, "tiled")
 
 
 def tiled_adjoint(g_r, g_i, tpos, rpos, plan: "TilePlan", A: int, K: int, norm_count: int,
-                  adcSampleRate: float, As_sci: float):
-    """grad [A, K] (real, imag) of the matched filter by the tile expansion;
-    g_r/g_i [N] in the caller's voxel order."""
+                  adcSampleRate: float, As_sci: float, voxel_nx3: torch.Tensor = None):
+    """grad [A, K] (real, imag) of the matched filter through the plan's
+    buckets (and the direct adjoint for its direct voxels, which needs
+    voxel_nx3); g_r/g_i [N] in the caller's voxel order."""
     prm = _C._params(adcSampleRate, As_sci)
-    gr = g_r.float()[plan.perm].contiguous()
-    gi = g_i.float()[plan.perm].contiguous()
-    out_r = torch.empty((A, K), dtype=torch.float32, device=gr.device)
-    out_i = torch.empty_like(out_r)
-    ws, size = plan.workspace(A, K, gr.device)
+    gr, gi = g_r.float(), g_i.float()
+    out_r = torch.zeros((A, K), dtype=torch.float32, device=gr.device)
+    out_i = torch.zeros_like(out_r)
     P = _native.ptr
-    _native.call("geraf_mf_backward_tiled_f32", P(gr), P(gi), P(tpos), P(rpos), P(out_r),
-                 P(out_i), P(plan.vt), P(plan.info), A, K, plan.N, norm_count, plan.terms,
-                 P(ws), size, prm, _native.stream_of(gr))
+    st = _native.stream_of(gr)
+    for b in plan.buckets:
+        br = (gr[b.idx] * b.valid).contiguous()
+        bi = (gi[b.idx] * b.valid).contiguous()
+        ws, size = TilePlan.workspace(A, K, b, gr.device)
+        p_r = torch.empty((A, K), dtype=torch.float32, device=gr.device)
+        p_i = torch.empty_like(p_r)
+        _native.call("geraf_mf_backward_tiled_f32", P(br), P(bi), P(tpos), P(rpos), P(p_r),
+                     P(p_i), P(b.vt), P(b.info), A, K, b.N, norm_count, b.terms, P(ws), size,
+                     prm, st)
+        out_r += p_r
+        out_i += p_i
+    if plan.direct_idx.numel():
+        if voxel_nx3 is None:
+            raise ValueError("the plan has direct voxels: pass voxel_nx3")
+        v = voxel_nx3[plan.direct_idx].t().contiguous()
+        dr = gr[plan.direct_idx].contiguous()
+        di = gi[plan.direct_idx].contiguous()
+        p_r = torch.empty((A, K), dtype=torch.float32, device=gr.device)
+        p_i = torch.empty_like(p_r)
+        _native.call("geraf_mf_backward_f32", P(dr), P(di), P(tpos), P(rpos), P(p_r), P(p_i),
+                     P(v), A, K, v.shape[1], norm_count,
+                     _native.make_params(prm.f0, prm.slope, prm.fs, prm.c, prm.path_bias), st)
+        out_r += p_r
+        out_i += p_i
     return out_r, out_i
 
 
 class _TiledMatchedFilter(Function):
     @staticmethod
     def forward(ctx, real, image, tpos, rpos, voxel, total_ant_size: int, numADCSample: int,
-                adcSampleRate: float, As_sci: float, N: int, tol: float):
+                adcSampleRate: float, As_sci: float, N: int, tol: float, plan):
         assert real.dim() == 2 and image.dim() == 2 and voxel.dim() == 2
         assert real.shape == (total_ant_size, numADCSample), (
             f"Size mismatch for real tensor: expected ({total_ant_size}, {numADCSample}), "
             f"but got: {tuple(real.shape)}")
         assert voxel.shape == (N, 3), f"expected voxel ({N}, 3), got {tuple(voxel.shape)}"
         args = [t.float().contiguous() for t in (real, image, tpos, rpos, voxel)]
-        o_r, o_i, info = tiled_forward(*args, numADCSample, adcSampleRate, As_sci, tol)
+        o_r, o_i, info = tiled_forward(*args, numADCSample, adcSampleRate, As_sci, tol, plan)
         ctx.save_for_backward(*args)
         ctx.cfg = (total_ant_size, numADCSample, adcSampleRate, As_sci, N)
         ctx.plan = info["plan"] if info["path"] == "tiled" and numADCSample <= 2048 else None
@@ -187,7 +283,7 @@ class _TiledMatchedFilter(Function):
         if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
             if ctx.plan is not None:
                 grad_real, grad_image = tiled_adjoint(g_r, g_i, tpos, rpos, ctx.plan, A, K, A,
-                                                      fs, As_sci)
+                                                      fs, As_sci, voxel)
             else:
                 grad_real = torch.empty_like(real)
                 grad_image = torch.empty_like(image)
@@ -195,16 +291,25 @@ class _TiledMatchedFilter(Function):
                     g_r.float().contiguous(), g_i.float().contiguous(), real, image, tpos, rpos,
                     grad_real, grad_image, voxel.t().contiguous(), A, K, fs, As_sci, N)
             grad_real, grad_image = grad_real.to(ctx.dtype), grad_image.to(ctx.dtype)
-        return (grad_real, grad_image) + (None,) * 9
+        return (grad_real, grad_image) + (None,) * 10
 
 
 def matched_filter_tiled(real: torch.Tensor, image: torch.Tensor, tpos: torch.Tensor,
                          rpos: torch.Tensor, voxel: torch.Tensor, total_ant_size: int,
                          numADCSample: int, adcSampleRate: float, As_sci: float, N: int,
-                         tol: float = 1e-6) -> Tuple[torch.Tensor, torch.Tensor]:
+                         tol: float = 1e-6, plan: "TilePlan" = None
+                         ) -> Tuple[torch.Tensor, torch.Tensor]:
     """``matched_filter_cuda`` evaluated by per-tile expansions (see the module
-    docstring for the error bound)."""
+    docstring for the error bound).  ``plan``: a ``TilePlan`` of the same voxel
+    set (and tolerance), for callers that evaluate one voxel set many times."""
     assert real.is_cuda and image.is_cuda and tpos.is_cuda and rpos.is_cuda and voxel.is_cuda, \
         "All inputs must be CUDA tensors."
     return _TiledMatchedFilter.apply(real, image, tpos, rpos, voxel, total_ant_size,
-                                     numADCSample, adcSampleRate, As_sci, N, tol)
+                                     numADCSample, adcSampleRate, As_sci, N, tol, plan)
+
+
+def make_plan(voxel: torch.Tensor, numADCSample: int, adcSampleRate: float, As_sci: float,
+              tol: float = 1e-6) -> "TilePlan":
+    """The tile plan of a voxel set [N, 3] for ``matched_filter_tiled``."""
+    return TilePlan(voxel.float().contiguous(), numADCSample, _C._params(adcSampleRate, As_sci),
+                    tol)

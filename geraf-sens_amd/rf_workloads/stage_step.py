@@ -34,6 +34,7 @@ import torch.nn.functional as F
 from geraf.models.networks.sdf_network import (ReflectivePowerNetwork, SDFNetwork,
                                                 SingleVarianceNetwork)
 from geraf.ops.RFMatchedFilter import matched_filter_cuda
+from geraf.ops.RFMatchedFilter.tiled_matched_filter import make_plan, matched_filter_tiled
 from geraf.ops.RFSignalTracer.lensless_tracer import lensless_signal_tracer
 
 from . import tracer_scenes as ts
@@ -93,9 +94,11 @@ STAGE2 = StageConfig(rays=32768, samples=256, K=512, min_valid=256, consistency_
 class StageStep:
     def __init__(self, cfg: StageConfig, target_volume: torch.Tensor, volume_axes,
                  device: torch.device, form: str = "fused", seed: int = 0,
-                 mlp_dtype: Optional[torch.dtype] = "auto"):
+                 mlp_dtype: Optional[torch.dtype] = "auto", mf_algo: str = "auto"):
         """``mlp_dtype``: compute dtype of the MLP GEMMs; "auto" is BF16 for the
-        fused form and FP32 for the autograd form."""
+        fused form and FP32 for the autograd form.  ``mf_algo``: "direct" or
+        "tiled" (the per-tile expansion matched filter, truncation bound 1e-6);
+        "auto" is tiled for the fused form and direct for the autograd form."""
         if form not in ("fused", "fp32_autograd"):
             raise ValueError(f"unknown step form {form!r}")
         self.cfg, self.form, self.dev = cfg, form, device
@@ -137,6 +140,13 @@ class StageStep:
             self.cons_pts = (torch.rand(cfg.consistency_points, 3, generator=gen) * 2 - 1) \
                 .to(device) * 0.6
         self.radar = b.radar
+        if mf_algo == "auto":
+            mf_algo = "tiled" if form == "fused" else "direct"
+        if mf_algo not in ("direct", "tiled"):
+            raise ValueError(f"unknown matched-filter algorithm {mf_algo!r}")
+        self.mf_algo = mf_algo
+        self.plan = (make_plan(self.tgt, cfg.K, self.radar["adcSampleRate"], self.radar["As_sci"])
+                     if mf_algo == "tiled" else None)
 
     def loss(self) -> torch.Tensor:
         cfg, b = self.cfg, self.b
@@ -157,8 +167,14 @@ class StageStep:
             sim_r, sim_i = torch_lensless(sdf, grad, b.dists, b.z, b.o, b.d, inv_s, power, refl,
                                           b.n_valid.float(), r["numADCSample"],
                                           r["adcSampleRate"], r["As_sci"], r["fc_start"])
-        mf_r, mf_i = matched_filter_cuda(sim_r, sim_i, b.o, b.o, self.tgt, R, r["numADCSample"],
-                                         r["adcSampleRate"], r["As_sci"], self.tgt.shape[0])
+        if self.mf_algo == "tiled":
+            mf_r, mf_i = matched_filter_tiled(sim_r, sim_i, b.o, b.o, self.tgt, R,
+                                              r["numADCSample"], r["adcSampleRate"], r["As_sci"],
+                                              self.tgt.shape[0], plan=self.plan)
+        else:
+            mf_r, mf_i = matched_filter_cuda(sim_r, sim_i, b.o, b.o, self.tgt, R,
+                                             r["numADCSample"], r["adcSampleRate"], r["As_sci"],
+                                             self.tgt.shape[0])
         pred = torch.hypot(mf_r, mf_i)
         pred = pred / pred.detach().max().clamp(min=1e-12)
         loss = (pred - self.gt).abs().mean()
