@@ -367,10 +367,10 @@ void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const
                       float2* mom, double* tau_c, float* grad_re, float* grad_im,
                       hipStream_t stream) {
   const double fcen = pc.f0 + kc * pc.dfs;
-  // antenna ranges side by side when the tiles alone give fewer than ~8
+  // antenna ranges side by side when the tiles alone give fewer than ~32
   // waves per SIMD (one wave per tile)
   int g = 1;
-  while (g < 64 && g * 2 <= na && (int64_t)ntiles * g < 8192) g *= 2;
+  while (g < 64 && g * 2 <= na && (int64_t)ntiles * g < 32768) g *= 2;
   const int per_group = (na + g - 1) / g;
   hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT>),
                      dim3((unsigned)((ntiles + 3) / 4), (unsigned)g), dim3(256), 0, stream, gre,
@@ -416,10 +416,10 @@ int round_terms(int nterms) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Antenna groups run side by side when the tiles alone would give fewer than
-// ~8 waves per SIMD (4 waves per tile block); each group has its own FP64
-// accumulator slice.
+// ~16 waves per SIMD (4 waves per tile block), so that two waves are ready to
+// dual-issue most of the time; each group has its own FP64 accumulator slice.
 int antenna_groups(int64_t ntiles) {
-  const int64_t want = 2048;  // tile blocks of 4 waves
+  const int64_t want = 4096;  // tile blocks of 4 waves
   int g = 1;
   while (g < 16 && ntiles * g < want) g *= 2;
   return g;

@@ -13,7 +13,9 @@ the copy and the matched filter.  --stage1-step N instead runs two warm-up
 and N timed stage-1-style training steps (8,192 rays x 64 samples, K=64;
 fused form, BF16 MLPs) for a kernel trace of the whole step; the target
 volume is a smooth synthetic blob on the 32^3 grid, which fixes the work but
-not the values.
+not the values.  --tiled runs the per-tile expansion matched filter and its
+adjoint on the 64x64 MIMO, K=512, 128^3 workload (twice: the second call is
+the one to read).
 """
 import argparse
 import os
@@ -60,8 +62,12 @@ def main() -> None:
     ap.add_argument("--tracer-only", action="store_true")
     ap.add_argument("--stage1-step", type=int, default=0)
     ap.add_argument("--form", default="fused", choices=("fused", "fp32_autograd"))
+    ap.add_argument("--tiled", action="store_true")
     args = ap.parse_args()
     dev = torch.device("cuda:0")
+    if args.tiled:
+        tiled(dev)
+        return
     if args.stage1_step:
         stage_steps(dev, args.stage1_step, args.form)
         return
@@ -95,6 +101,22 @@ def main() -> None:
     torch.cuda.synchronize()
 
     trace_sizes(dev, args.stage2)
+
+
+def tiled(dev) -> None:
+    """The per-tile expansion matched filter and its adjoint on the 64x64
+    MIMO, K=512, 128^3 workload (one call each, after one warm-up call)."""
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import tiled_adjoint, tiled_forward
+
+    sc = mf_scenes.config2(0, device=str(dev))
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), device=dev)  # noqa: E731
+    re, im, tx, rx, vox = t(sc.echo.real), t(sc.echo.imag), t(sc.tx), t(sc.rx), t(sc.voxels)
+    for rep in range(2):
+        print(f"profile_driver: tiled matched filter, call {rep}", flush=True)
+        o_r, o_i, info = tiled_forward(re, im, tx, rx, vox, sc.K, sc.fs, sc.As_sci, 1e-6)
+        tiled_adjoint(o_r, o_i, tx, rx, info["plan"], sc.A, sc.K, sc.A, sc.fs, sc.As_sci, vox)
+        torch.cuda.synchronize()
+    print("profile_driver: done", flush=True)
 
 
 def stage_steps(dev, n: int, form: str) -> None:
