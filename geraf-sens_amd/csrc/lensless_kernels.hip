@@ -243,6 +243,56 @@ __device__ __forceinline__ void synth(const FwdQueue& q, int wave, int n, int gr
   }
 }
 
+// Matrix-core synthesis.  With k = 16 a + b (b < 16) the ray's signal is a
+// complex product over its emitting samples s,
+//   out[16 a + b] = sum_s X[a,s] Y[s,b],
+//   X[a,s] = amp_s exp(+j 2 pi (f0 + 16 a dfs) tau_s),  Y[s,b] = exp(+j 2 pi b dfs tau_s),
+// taken four samples at a time by v_mfma_f32_16x16x4f32 (exact FP32
+// products, FP32 accumulation): four real products per 16-row tile of a.
+// Each lane forms one Y and one X phasor per step from FP64-reduced phases
+// (hardware sine and cosine);
+// the rows of further tiles follow by one complex rotation each (the
+// sample's exp(j 2 pi 256 dfs tau), queued with it).
+// Operand lanes of that instruction: A[i][k] and B[k][j] come from lane
+// i + 16 k (resp. j + 16 k); D[4 q + v][j] is element v of lane j + 16 q.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int MT>
+__device__ __forceinline__ void synth_mfma(const FwdQueue& q, int wave, int lane, int n,
+                                           f32x4* dr, f32x4* di) {
+  const int j = lane & 15;
+  for (int s0 = 0; s0 < n; s0 += 4) {
+    const int i = s0 + (lane >> 4);
+    float am = 0.0f, f0 = 0.0f;
+    double c1 = 0.0;
+    if (i < n) {
+      am = q.amp[wave][i];
+      f0 = q.fr0[wave][i];
+      c1 = q.cyc1[wave][i];
+    }
+    const float2 yv = phasor_cycles_hw((float)centered_frac((double)j * c1));
+    const float2 xv = phasor_cycles_hw((float)centered_frac(fma((double)(16 * j), c1, (double)f0)));
+    float2 x = make_float2(am * xv.x, am * xv.y);
+    float2 rot = make_float2(1.0f, 0.0f);
+    if constexpr (MT > 1) {
+      if (i < n) rot = q.w[wave][i];
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      if (m > 0) x = cmul(x, rot);  // X[a + 16, s] = X[a, s] exp(j 2 pi 256 dfs tau_s)
+      const float xr = x.x, xi = x.y;
+      dr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr, yv.x, dr[m], 0, 0, 0);
+      di[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr, yv.y, di[m], 0, 0, 0);
+      dr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(-xi, yv.y, dr[m], 0, 0, 0);
+      di[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi, yv.x, di[m], 0, 0, 0);
+    }
+  }
+}
+
+// MT = 0: rotation synthesis on the vector ALUs (lanes = frequency chunk x
+// sample group); MT > 0: matrix-core synthesis with MT tiles of 256
+// frequencies (K <= 256 MT).
+template <int MT>
 __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
     LenslessArgs a, float* __restrict__ out_real, float* __restrict__ out_imag,
     float* __restrict__ sigma_out, PhaseConsts pc, int groups, int chunks) {
@@ -258,9 +308,24 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
   const bool chunk_ok = chunk < chunks;
   const int k0 = chunk * kSigChunk;
 
-  float acc_r[kSigChunk], acc_i[kSigChunk];
+  constexpr int kAcc = MT ? 1 : kSigChunk;
+  constexpr int kTiles = MT ? MT : 1;
+  float acc_r[kAcc], acc_i[kAcc];
+  f32x4 dr[kTiles], di[kTiles];
+  if constexpr (MT == 0) {
 #pragma unroll
-  for (int j = 0; j < kSigChunk; ++j) acc_r[j] = acc_i[j] = 0.0f;
+    for (int j = 0; j < kSigChunk; ++j) acc_r[j] = acc_i[j] = 0.0f;
+  } else {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) dr[m] = di[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+  auto drain = [&](int n) {
+    if constexpr (MT == 0) {
+      if (chunk_ok) synth(q, wave, n, grp, groups, k0, acc_r, acc_i);
+    } else {
+      synth_mfma<MT>(q, wave, lane, n, dr, di);
+    }
+  };
 
   float carry = 1.0f;
   int queued = 0;
@@ -286,12 +351,14 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
       q.amp[wave][slot] = amp;
       q.fr0[wave][slot] = (float)centered_frac(pc.f0 * tau);
       q.cyc1[wave][slot] = c1;
-      q.w[wave][slot] = phasor_cycles((float)centered_frac(c1), 1.0f);
+      if constexpr (MT == 0) q.w[wave][slot] = phasor_cycles((float)centered_frac(c1), 1.0f);
+      if constexpr (MT > 1)
+        q.w[wave][slot] = phasor_cycles_hw((float)centered_frac(256.0 * c1));
     }
     queued += __popcll(mask);
     wave_sync();
     if (queued >= kTile) {
-      if (chunk_ok) synth(q, wave, kTile, grp, groups, k0, acc_r, acc_i);
+      drain(kTile);
       const int rest = queued - kTile;
       float am = 0.0f, f0 = 0.0f;
       float2 w = make_float2(0.0f, 0.0f);
@@ -299,36 +366,50 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
       if (lane < rest) {
         am = q.amp[wave][kTile + lane];
         f0 = q.fr0[wave][kTile + lane];
-        w = q.w[wave][kTile + lane];
+        if constexpr (MT != 1) w = q.w[wave][kTile + lane];
         c1 = q.cyc1[wave][kTile + lane];
       }
       wave_sync();
       if (lane < rest) {
         q.amp[wave][lane] = am;
         q.fr0[wave][lane] = f0;
-        q.w[wave][lane] = w;
+        if constexpr (MT != 1) q.w[wave][lane] = w;
         q.cyc1[wave][lane] = c1;
       }
       queued = rest;
       wave_sync();
     }
   }
-  if (queued > 0 && chunk_ok) synth(q, wave, queued, grp, groups, k0, acc_r, acc_i);
+  if (queued > 0) drain(queued);
 
-  group_sum<kSigChunk>(acc_r, groups);
-  group_sum<kSigChunk>(acc_i, groups);
-  if (chunk_ok && grp == 0) {
-    const float inv_n = y.nv > 0 ? 1.0f / (float)y.nv : 0.0f;
-    float* orow = out_real + (size_t)r * a.K;
-    float* irow = out_imag + (size_t)r * a.K;
+  const float inv_n = y.nv > 0 ? 1.0f / (float)y.nv : 0.0f;
+  float* orow = out_real + (size_t)r * a.K;
+  float* irow = out_imag + (size_t)r * a.K;
+  if constexpr (MT == 0) {
+    group_sum<kSigChunk>(acc_r, groups);
+    group_sum<kSigChunk>(acc_i, groups);
+    if (chunk_ok && grp == 0) {
 #pragma unroll
-    for (int j = 0; j < kSigChunk; ++j) {
-      const int k = k0 + j;
-      if (k < a.K) {
-        st_row(orow, (unsigned)k, acc_r[j] * inv_n);
-        st_row(irow, (unsigned)k, acc_i[j] * inv_n);
+      for (int j = 0; j < kSigChunk; ++j) {
+        const int k = k0 + j;
+        if (k < a.K) {
+          st_row(orow, (unsigned)k, acc_r[j] * inv_n);
+          st_row(irow, (unsigned)k, acc_i[j] * inv_n);
+        }
       }
     }
+  } else {
+    const int kl = 64 * (lane >> 4) + (lane & 15);
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = 256 * m + 16 * v + kl;
+        if (k < a.K) {
+          st_row(orow, (unsigned)k, dr[m][v] * inv_n);
+          st_row(irow, (unsigned)k, di[m][v] * inv_n);
+        }
+      }
   }
   if (sigma_out)
     for (int s = y.nv + lane; s < a.S; s += kWave) st_row(sigma_out + y.row, (unsigned)s, 0.0f);
@@ -569,9 +650,25 @@ extern "C" int geraf_lensless_forward_f32(
   int groups = 1;
   while (groups * 2 * chunks <= kWave) groups *= 2;
   const unsigned blocks = (unsigned)((R + kRaysPerBlock - 1) / kRaysPerBlock);
-  hipLaunchKernelGGL(lensless_forward_kernel, dim3(blocks), dim3(64 * kRaysPerBlock), 0,
-                     (hipStream_t)stream, a, out_real, out_imag, sigma_out,
-                     make_phase_consts(*params), groups, chunks);
+  const PhaseConsts pc = make_phase_consts(*params);
+  const dim3 grid(blocks), block(64 * kRaysPerBlock);
+  hipStream_t st = (hipStream_t)stream;
+  // K <= 128 leaves most of a 16 x 16 matrix tile idle: rotation synthesis
+  if (K <= 128)
+    hipLaunchKernelGGL(lensless_forward_kernel<0>, grid, block, 0, st, a, out_real, out_imag,
+                       sigma_out, pc, groups, chunks);
+  else if (K <= 256)
+    hipLaunchKernelGGL(lensless_forward_kernel<1>, grid, block, 0, st, a, out_real, out_imag,
+                       sigma_out, pc, groups, chunks);
+  else if (K <= 512)
+    hipLaunchKernelGGL(lensless_forward_kernel<2>, grid, block, 0, st, a, out_real, out_imag,
+                       sigma_out, pc, groups, chunks);
+  else if (K <= 1024)
+    hipLaunchKernelGGL(lensless_forward_kernel<4>, grid, block, 0, st, a, out_real, out_imag,
+                       sigma_out, pc, groups, chunks);
+  else
+    hipLaunchKernelGGL(lensless_forward_kernel<8>, grid, block, 0, st, a, out_real, out_imag,
+                       sigma_out, pc, groups, chunks);
   return check_launch("lensless_forward_kernel");
   GERAF_TRY_END
 }

