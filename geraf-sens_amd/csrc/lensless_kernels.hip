@@ -27,14 +27,17 @@
 // Forward signal: emitting samples (valid, not culled, amp != 0) are appended
 // to a per-wave LDS queue as (amplitude, reduced carrier phase, per-frequency
 // phase step in FP64 cycles, step phasor).  The queue is drained in batches
-// of 64 (and once at the end): lane (chunk j, group g) takes the entries
-// i = g (mod G), seeds exp(j*2*pi*(f0 + 32 j dfs) tau_i) from the FP64-reduced
-// phase and rotates it through its 32 frequencies into 32 private complex
-// accumulators; the G groups are summed with DPP at the end.
+// of 64 (and once at the end).  For K > 128 the drain is a complex matrix
+// product on the matrix cores (k = 16 a + b, see synth_mfma).  For K <= 128
+// lane (chunk j, group g) takes the entries i = g (mod G), seeds
+// exp(j*2*pi*(f0 + 32 j dfs) tau_i) from the FP64-reduced phase and rotates
+// it through its 32 frequencies into 32 private complex accumulators; the G
+// groups are summed with DPP at the end.
 // Backward: pass A walks the tiles forward, keeps the transmittance carry at
 // each tile start, and queues (tau, sample) of every emitting sample; full
-// 64-lane batches evaluate h_s = Re sum_k g[r,k] exp(-j phi_sk) by two-level
-// Horner (gradient row read through the scalar cache).  Pass B walks the
+// 64-sample batches evaluate h_s = Re sum_k g[r,k] exp(-j phi_sk), for
+// K > 128 as a matrix product (see h_mfma), else by two-level Horner (one
+// sample per lane, gradient row read through the scalar cache).  Pass B walks the
 // tiles backward, recomputes the compositing and geometry, and propagates
 // through the transmittance with a reverse affine scan
 // W_s = gT_s + t_s W_{s+1}, so dL/dalpha_i = (direct) - T_i W_{i+1} needs no
@@ -120,15 +123,33 @@ struct Sample {
   bool qmask;
 };
 
-__device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ray& y, int s) {
-  Sample x;
+// The raw per-sample inputs.
+struct RawSample {
+  float sdf, dist, zz, refl, off;
+  V3 g;
+};
+
+__device__ __forceinline__ RawSample load_raw(const LenslessArgs& a, const Ray& y, int s) {
+  RawSample w;
   const unsigned u = (unsigned)s;
-  x.sdf = ld_row(a.sdf + y.row, u);
-  x.dist = ld_row(a.dists + y.row, u);
-  x.zz = ld_row(a.z + y.row, u);
-  x.refl = a.refl ? ld_row(a.refl + y.row, u) : 1.0f;
-  x.off = a.offset ? ld_row(a.offset + y.row, u) : 0.0f;
-  x.g = ld_row3(a.grad + 3 * y.row, u);
+  w.sdf = ld_row(a.sdf + y.row, u);
+  w.dist = ld_row(a.dists + y.row, u);
+  w.zz = ld_row(a.z + y.row, u);
+  w.refl = a.refl ? ld_row(a.refl + y.row, u) : 1.0f;
+  w.off = a.offset ? ld_row(a.offset + y.row, u) : 0.0f;
+  w.g = ld_row3(a.grad + 3 * y.row, u);
+  return w;
+}
+
+__device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ray& y, int s) {
+  const RawSample w = load_raw(a, y, s);
+  Sample x;
+  x.sdf = w.sdf;
+  x.dist = w.dist;
+  x.zz = w.zz;
+  x.refl = w.refl;
+  x.off = w.off;
+  x.g = w.g;
   x.c = dot(y.dir, x.g);
   const float ic =
       -(fmaxf(-x.c * 0.5f + 0.5f, 0.0f) * (1.0f - a.anneal) + fmaxf(-x.c, 0.0f) * a.anneal);
@@ -248,7 +269,9 @@ __device__ __forceinline__ void synth(const FwdQueue& q, int wave, int n, int gr
 //   out[16 a + b] = sum_s X[a,s] Y[s,b],
 //   X[a,s] = amp_s exp(+j 2 pi (f0 + 16 a dfs) tau_s),  Y[s,b] = exp(+j 2 pi b dfs tau_s),
 // taken four samples at a time by v_mfma_f32_16x16x4f32 (exact FP32
-// products, FP32 accumulation): four real products per 16-row tile of a.
+// products, FP32 accumulation), three real products per 16-row tile of a:
+// P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi); Re = P1 - P2 and
+// Im = P3 - P1 - P2 (the error stays O(eps sum |X||Y|), as with four).
 // Each lane forms one Y and one X phasor per step from FP64-reduced phases
 // (hardware sine and cosine);
 // the rows of further tiles follow by one complex rotation each (the
@@ -259,7 +282,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int MT>
 __device__ __forceinline__ void synth_mfma(const FwdQueue& q, int wave, int lane, int n,
-                                           f32x4* dr, f32x4* di) {
+                                           f32x4* dr, f32x4* di, f32x4* ds) {
   const int j = lane & 15;
   for (int s0 = 0; s0 < n; s0 += 4) {
     const int i = s0 + (lane >> 4);
@@ -277,14 +300,13 @@ __device__ __forceinline__ void synth_mfma(const FwdQueue& q, int wave, int lane
     if constexpr (MT > 1) {
       if (i < n) rot = q.w[wave][i];
     }
+    const float ys = yv.x + yv.y;
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
       if (m > 0) x = cmul(x, rot);  // X[a + 16, s] = X[a, s] exp(j 2 pi 256 dfs tau_s)
-      const float xr = x.x, xi = x.y;
-      dr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr, yv.x, dr[m], 0, 0, 0);
-      di[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr, yv.y, di[m], 0, 0, 0);
-      dr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(-xi, yv.y, dr[m], 0, 0, 0);
-      di[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi, yv.x, di[m], 0, 0, 0);
+      dr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, yv.x, dr[m], 0, 0, 0);
+      di[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, yv.y, di[m], 0, 0, 0);
+      ds[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x + x.y, ys, ds[m], 0, 0, 0);
     }
   }
 }
@@ -311,19 +333,19 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
   constexpr int kAcc = MT ? 1 : kSigChunk;
   constexpr int kTiles = MT ? MT : 1;
   float acc_r[kAcc], acc_i[kAcc];
-  f32x4 dr[kTiles], di[kTiles];
+  f32x4 dr[kTiles], di[kTiles], ds[kTiles];
   if constexpr (MT == 0) {
 #pragma unroll
     for (int j = 0; j < kSigChunk; ++j) acc_r[j] = acc_i[j] = 0.0f;
   } else {
 #pragma unroll
-    for (int m = 0; m < MT; ++m) dr[m] = di[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int m = 0; m < MT; ++m) dr[m] = di[m] = ds[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   }
   auto drain = [&](int n) {
     if constexpr (MT == 0) {
       if (chunk_ok) synth(q, wave, n, grp, groups, k0, acc_r, acc_i);
     } else {
-      synth_mfma<MT>(q, wave, lane, n, dr, di);
+      synth_mfma<MT>(q, wave, lane, n, dr, di, ds);
     }
   };
 
@@ -357,17 +379,18 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
     }
     queued += __popcll(mask);
     wave_sync();
-    if (queued >= kTile) {
-      drain(kTile);
-      const int rest = queued - kTile;
+    const int ndrain = queued >= kTile ? kTile : 0;
+    if (ndrain > 0) {
+      drain(ndrain);
+      const int rest = queued - ndrain;
       float am = 0.0f, f0 = 0.0f;
       float2 w = make_float2(0.0f, 0.0f);
       double c1 = 0.0;
       if (lane < rest) {
-        am = q.amp[wave][kTile + lane];
-        f0 = q.fr0[wave][kTile + lane];
-        if constexpr (MT != 1) w = q.w[wave][kTile + lane];
-        c1 = q.cyc1[wave][kTile + lane];
+        am = q.amp[wave][ndrain + lane];
+        f0 = q.fr0[wave][ndrain + lane];
+        if constexpr (MT != 1) w = q.w[wave][ndrain + lane];
+        c1 = q.cyc1[wave][ndrain + lane];
       }
       wave_sync();
       if (lane < rest) {
@@ -399,6 +422,13 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
       }
     }
   } else {
+    // (Xr Yr - Xi Yi, (Xr + Xi)(Yr + Yi) - Xr Yr - Xi Yi)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const f32x4 p1 = dr[m], p2 = di[m];
+      dr[m] = p1 - p2;
+      di[m] = ds[m] - p1 - p2;
+    }
     const int kl = 64 * (lane >> 4) + (lane & 15);
 #pragma unroll
     for (int m = 0; m < MT; ++m)
@@ -429,6 +459,90 @@ __host__ __device__ constexpr size_t bwd_lds_bytes_per_wave(int S) {
          (size_t)S * sizeof(float);
 }
 
+// Matrix-core form of h_s (K > 128).  With k = 16 a + b,
+//   sum_k g[k] exp(-j phi_sk) = sum_a conj(Xt[a,s]) T[a,s],
+//   T[a,s] = sum_b G[a,b] conj(Y[s,b]),  G[a,b] = g[16 a + b],
+//   Xt[a,s] = exp(+j 2 pi (f0 + 16 a dfs) tau_s),  Y[s,b] = exp(+j 2 pi b dfs tau_s):
+// T is a product of the gradient row (the A operand, held in registers for
+// the whole ray: tile m, depth step t, lane i + 16 k holds g[256 m + 16 i +
+// 4 t + k]) and 16 queued samples' phasors (the B operand), four depth steps
+// of v_mfma_f32_16x16x4f32 per 16-row tile; lane j + 16 q then holds
+// T[16 m + 4 q + v, s_j] and contracts it with Xt (one phasor, rotations by
+// exp(j 2 pi 16 dfs tau) and exp(j 2 pi 256 dfs tau)); the four lane quarters
+// are summed with cross-lane shuffles.
+template <int MT>
+__device__ __forceinline__ void h_mfma(const double* __restrict__ btau,
+                                       const int* __restrict__ bsid, float* __restrict__ hrow,
+                                       int cnt, int lane, const float (*gr)[4],
+                                       const float (*gi)[4], const PhaseConsts& pc,
+                                       float inv_n) {
+  // three real products per complex one for up to two tiles (four beyond:
+  // the third accumulator set then costs more occupancy than it saves)
+  constexpr bool kThree = MT <= 2;
+  const int j = lane & 15, qd = lane >> 4;
+  for (int s0 = 0; s0 < cnt; s0 += 16) {
+    const int idx = s0 + j;
+    const bool ok = idx < cnt;
+    const double tau = ok ? btau[idx] : 0.0;
+    const double c1 = pc.dfs * tau;
+    f32x4 tr[MT], ti[MT], tq[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) tr[m] = ti[m] = tq[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float2 y = phasor_cycles_hw((float)centered_frac((double)(4 * t + qd) * c1));
+      const float yi_neg = -y.y;  // conj(Y) = (y.x, -y.y)
+      if constexpr (kThree) {
+        // P1 = G_r Y_r, P2 = G_i (-Y_i), P3 = (G_r + G_i)(Y_r - Y_i)
+        const float yd = y.x - y.y;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          tr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(gr[m][t], y.x, tr[m], 0, 0, 0);
+          ti[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(gi[m][t], yi_neg, ti[m], 0, 0, 0);
+          tq[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(gr[m][t] + gi[m][t], yd, tq[m], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          tr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(gr[m][t], y.x, tr[m], 0, 0, 0);
+          ti[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(gr[m][t], yi_neg, ti[m], 0, 0, 0);
+          tr[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(gi[m][t], y.y, tr[m], 0, 0, 0);
+          ti[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(gi[m][t], y.x, ti[m], 0, 0, 0);
+        }
+      }
+    }
+    if constexpr (kThree) {
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {  // T = (P1 - P2, P3 - P1 - P2)
+        const f32x4 p1 = tr[m], p2 = ti[m];
+        tr[m] = p1 - p2;
+        ti[m] = tq[m] - p1 - p2;
+      }
+    }
+    float2 x = phasor_cycles_hw((float)centered_frac(fma((double)(64 * qd), c1, pc.f0 * tau)));
+    const float2 st16 = phasor_cycles_hw((float)centered_frac(16.0 * c1));
+    float2 st256 = make_float2(1.0f, 0.0f);
+    if constexpr (MT > 1) st256 = phasor_cycles_hw((float)centered_frac(256.0 * c1));
+    float h = 0.0f;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      float2 xm = x;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        h = fmaf(xm.x, tr[m][v], fmaf(xm.y, ti[m][v], h));
+        if (v < 3) xm = cmul(xm, st16);
+      }
+      if (m + 1 < MT) x = cmul(x, st256);
+    }
+    h += __shfl_xor(h, 16, kWave);
+    h += __shfl_xor(h, 32, kWave);
+    if (ok && qd == 0) hrow[bsid[idx]] = h * inv_n;
+  }
+}
+
+// MT = 0: h_s by two-level Horner (one sample per lane); MT > 0: matrix-core
+// form with MT tiles of 256 frequencies (K <= 256 MT).
+template <int MT>
 __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
     LenslessArgs a, const float* __restrict__ g_real, const float* __restrict__ g_imag,
     float* __restrict__ grad_sdf, float* __restrict__ grad_g, float* __restrict__ grad_refl,
@@ -454,11 +568,27 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
 
   // h_s = (1/n) Re sum_k g[r,k] exp(-j phi_sk) for the first `cnt` queued
   // samples, one per lane (the gradient row is wave-uniform: scalar loads).
+  constexpr int kT = MT ? MT : 1;
+  float ga_r[kT][4], ga_i[kT][4];
+  if constexpr (MT > 0) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int k = 256 * m + 16 * (lane & 15) + 4 * t + (lane >> 4);
+        ga_r[m][t] = k < a.K ? ld_row(gr_row, (unsigned)k) : 0.0f;
+        ga_i[m][t] = k < a.K ? ld_row(gi_row, (unsigned)k) : 0.0f;
+      }
+  }
   auto horner_batch = [&](int cnt) {
-    if (lane < cnt) {
-      const PathPhasors ph = path_phasors(L.btau[lane], pc, -1.0f);
-      const float2 S = horner2(gr_row, gi_row, a.K, ph.w, ph.W);
-      L.h[L.bsid[lane]] = cmul(ph.base, S).x * inv_n;
+    if constexpr (MT == 0) {
+      if (lane < cnt) {
+        const PathPhasors ph = path_phasors(L.btau[lane], pc, -1.0f);
+        const float2 S = horner2(gr_row, gi_row, a.K, ph.w, ph.W);
+        L.h[L.bsid[lane]] = cmul(ph.base, S).x * inv_n;
+      }
+    } else {
+      h_mfma<MT>(L.btau, L.bsid, L.h, cnt, lane, ga_r, ga_i, pc, inv_n);
     }
   };
 
@@ -484,14 +614,15 @@ __global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
     }
     queued += __popcll(mask);
     wave_sync();
-    if (queued >= kWave) {
-      horner_batch(kWave);
-      const int rest = queued - kWave;
+    const int ndrain = queued >= kWave ? kWave : 0;
+    if (ndrain > 0) {
+      horner_batch(ndrain);
+      const int rest = queued - ndrain;
       double t_keep = 0.0;
       int s_keep = 0;
       if (lane < rest) {
-        t_keep = L.btau[kWave + lane];
-        s_keep = L.bsid[kWave + lane];
+        t_keep = L.btau[ndrain + lane];
+        s_keep = L.bsid[ndrain + lane];
       }
       wave_sync();
       if (lane < rest) {
@@ -684,6 +815,7 @@ extern "C" int geraf_lensless_backward_f32(
   check_params(params);
   const LenslessArgs a = make_args(sdf, grad, refl, dists, z, origin, dirs, trans_offset,
                                    n_valid, inv_s, power, R, S, K, cos_anneal_ratio);
+  if (K > 2048) throw ArgError("at most 2048 frequencies per ray");
   if (R == 0) return GERAF_RF_OK;
   require_ptrs({grad_inv_s_per_ray, grad_power_per_ray}, true);
   require_ptrs({grad_sdf, grad_grad}, S > 0);
@@ -691,10 +823,24 @@ extern "C" int geraf_lensless_backward_f32(
   if (S == 0) throw ArgError("need at least one sample per ray");
   const unsigned blocks = (unsigned)((R + kRaysPerBlock - 1) / kRaysPerBlock);
   const size_t lds = kRaysPerBlock * bwd_lds_bytes_per_wave((int)S);
-  hipLaunchKernelGGL(lensless_backward_kernel, dim3(blocks), dim3(64 * kRaysPerBlock), lds,
-                     (hipStream_t)stream, a, grad_out_real, grad_out_imag, grad_sdf, grad_grad,
-                     grad_refl, grad_inv_s_per_ray, grad_power_per_ray,
-                     make_phase_consts(*params));
+  const PhaseConsts pc = make_phase_consts(*params);
+  const dim3 grid(blocks), block(64 * kRaysPerBlock);
+  hipStream_t st = (hipStream_t)stream;
+#define GERAF_LL_BWD(MT_)                                                                      \
+  hipLaunchKernelGGL(lensless_backward_kernel<MT_>, grid, block, lds, st, a, grad_out_real,   \
+                     grad_out_imag, grad_sdf, grad_grad, grad_refl, grad_inv_s_per_ray,        \
+                     grad_power_per_ray, pc)
+  if (K <= 128)
+    GERAF_LL_BWD(0);
+  else if (K <= 256)
+    GERAF_LL_BWD(1);
+  else if (K <= 512)
+    GERAF_LL_BWD(2);
+  else if (K <= 1024)
+    GERAF_LL_BWD(4);
+  else
+    GERAF_LL_BWD(8);
+#undef GERAF_LL_BWD
   return check_launch("lensless_backward_kernel");
   GERAF_TRY_END
 }

@@ -214,15 +214,21 @@ def test_lensless_config3_subset(cuda):
 
 @pytest.mark.parametrize("R,S,K", [(96, 128, 200), (64, 96, 300), (64, 256, 512),
                                    (48, 160, 1000), (32, 80, 2048)])
-def test_lensless_forward_matrix_tiles(cuda, R, S, K):
-    """The matrix-core synthesis (K > 128): one to eight 256-frequency tiles,
-    partial tiles (K = 200, 300, 1000), rays whose emitting samples overflow
-    the 64-entry batch (S = 256), ragged rays with none."""
+def test_lensless_matrix_tiles(cuda, R, S, K):
+    """The matrix-core synthesis and its adjoint (K > 128): one to eight
+    256-frequency tiles, partial tiles (K = 200, 300, 1000), rays whose
+    emitting samples overflow the 64-entry batch (S = 256), ragged rays with
+    none."""
     b = ts.rays(R, S, K, seed=K, uneven=(0, S))
-    out, _ = _lensless_run(b, cuda, with_grads=False)
-    ref, _ = _lensless_oracle(b)
+    out, grads = _lensless_run(b, cuda)
+    ref, rgrads = _lensless_oracle(b, grads["_g"])
     err = rel(out, ref)
-    print(f"lensless fwd R{R} S{S} K{K}: rel-L2 {err:.2e}")
+    print(f"lensless R{R} S{S} K{K}: fwd rel-L2 {err:.2e}")
     assert err <= TOL_FWD
     empty = (b.n_valid == 0).numpy()
     assert np.all(out[empty] == 0)
+    for k in ("sdf", "grad", "refl"):
+        e = rel(grads[k].numpy().ravel(), rgrads[k].numpy().ravel())
+        print(f"  grad {k}: rel-L2 {e:.2e}")
+        assert e <= TOL_GRAD, k
+        assert np.all(grads[k].numpy()[empty] == 0)
