@@ -373,6 +373,123 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_kernel(
   }
 }
 
+// Matrix-core adjoint (128 < K <= 1024).  Block = antenna a, voxels staged in
+// tiles of 512 as above (cg = g_v exp(+j 2 pi f0 tau), the FP64 step cyc1, and
+// the tile rotation exp(+j 2 pi 256 cyc1)).  With k = 16 al + b (b < 16)
+//   grad[a, 16 al + b] = sum_v X[al, v] Y[v, b],
+//   X = cg_v exp(+j 2 pi 16 al cyc1_v),  Y = exp(+j 2 pi b cyc1_v),
+// so each wave takes 64 of the tile's voxels as 16 four-deep steps of
+// v_mfma_f32_16x16x4f32 over all MT 16-row tiles, three real products per
+// complex one (P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi)).  Operand
+// lanes: A[i][k] and B[k][j] from lane i + 16 k (j + 16 k); D[4 q + v][j] is
+// element v of lane j + 16 q.  Each lane forms one Y and one X phasor per
+// step (FP64-reduced phase, hardware sine and cosine); further tiles follow by
+// one rotation.  After every tile the FP32 matrix sums (64 voxels) are added
+// into FP64 registers; the eight waves' FP64 sums are reduced through LDS in
+// a fixed order.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct AdjMfmaStage {
+  float2 cg[kAdjTile];    // output gradient times the carrier phasor
+  float2 rot[kAdjTile];   // exp(+j 2 pi 256 cyc1): one 256-frequency tile on
+  double cyc1[kAdjTile];  // per-sample step [cycles]
+};
+
+template <int MT>
+__global__ __launch_bounds__(kAdjTile) void mf_backward_mfma_kernel(
+    const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    float* __restrict__ grad_real, float* __restrict__ grad_imag,
+    const float* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc) {
+  __shared__ AdjMfmaStage st;
+  __shared__ double red[kAdjWaves][kAdjTile];  // one tile's (Re, Im) x 4 rows per lane
+  const int a = blockIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = lane & 15;
+  const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+  const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+
+  double sr[MT][4], si[MT][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sr[m][v] = si[m][v] = 0.0;
+
+  for (int base = 0; base < N; base += kAdjTile) {
+    {
+      const int v = base + threadIdx.x;
+      float2 cg = make_float2(0.0f, 0.0f), rot = make_float2(1.0f, 0.0f);
+      double cyc1 = 0.0;
+      if (v < N) {
+        const float gr = gout_real[v], gi = gout_imag[v];
+        const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
+        const double d =
+            dist3_f64_fast(tx, ty, tz, px, py, pz) + dist3_f64_fast(rx, ry, rz, px, py, pz);
+        const double tau = (d + pc.bias) * pc.inv_c;
+        cyc1 = pc.dfs * tau;
+        cg = cmul(make_float2(gr, gi), phasor_cycles((float)centered_frac(pc.f0 * tau), 1.0f));
+        if (MT > 1) rot = phasor_cycles_hw((float)centered_frac(256.0 * cyc1));
+      }
+      st.cg[threadIdx.x] = cg;
+      st.rot[threadIdx.x] = rot;
+      st.cyc1[threadIdx.x] = cyc1;
+    }
+    __syncthreads();
+    f32x4 p1[MT], p2[MT], p3[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) p1[m] = p2[m] = p3[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
+    for (int t = 0; t < 16; ++t) {
+      const int i = wave * kWave + 4 * t + (lane >> 4);  // voxel of this lane's step
+      const double c1 = st.cyc1[i];
+      const float2 y = phasor_cycles_hw((float)centered_frac((double)j * c1));
+      float2 x = cmul(st.cg[i], phasor_cycles_hw((float)centered_frac((double)(16 * j) * c1)));
+      float2 rot = make_float2(1.0f, 0.0f);
+      if constexpr (MT > 1) rot = st.rot[i];
+      const float ys = y.x + y.y;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        if (m > 0) x = cmul(x, rot);  // X[al + 16, v] = X[al, v] exp(j 2 pi 256 cyc1)
+        p1[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, p1[m], 0, 0, 0);
+        p2[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, p2[m], 0, 0, 0);
+        p3[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x + x.y, ys, p3[m], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        sr[m][v] += (double)(p1[m][v] - p2[m][v]);
+        si[m][v] += (double)(p3[m][v] - p1[m][v] - p2[m][v]);
+      }
+    __syncthreads();
+  }
+
+  // reduce the waves, one 256-frequency tile at a time: thread t sums entry
+  // t = 8 lane + 2 v + (0: Re, 1: Im) over the waves
+  const int tl = threadIdx.x >> 3, te = threadIdx.x & 7;
+  const int k_of_t = 64 * (tl >> 4) + 16 * (te >> 1) + (tl & 15);
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      red[wave][8 * lane + 2 * v] = sr[m][v];
+      red[wave][8 * lane + 2 * v + 1] = si[m][v];
+    }
+    __syncthreads();
+    double acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < kAdjWaves; ++w) acc += red[w][threadIdx.x];
+    const int k = 256 * m + k_of_t;
+    if (k < K) {
+      float* out = (te & 1) ? grad_imag : grad_real;
+      out[(size_t)a * K + k] = (float)(acc * inv_norm);
+    }
+    __syncthreads();
+  }
+}
+
 // Plain FP64 adjoint: one thread per (a, k), FP64 phase and sincos per voxel.
 __global__ __launch_bounds__(256) void mf_backward_f64_kernel(
     const double* __restrict__ gout_real, const double* __restrict__ gout_imag,
@@ -506,6 +623,23 @@ extern "C" int geraf_mf_backward_f32(const float* grad_out_real, const float* gr
   if (A == 0 || K == 0) return GERAF_RF_OK;
   require_ptrs({tx_pos, rx_pos, grad_real, grad_imag}, true);
   require_ptrs({grad_out_real, grad_out_imag, voxel}, N > 0);
+  if (K > 128 && K <= 1024) {
+    const PhaseConsts pc = make_phase_consts(*params);
+    const double inv_norm = 1.0 / (double)norm_count;
+    hipStream_t st = (hipStream_t)stream;
+#define GERAF_MF_ADJ(MT_)                                                                      \
+  hipLaunchKernelGGL(mf_backward_mfma_kernel<MT_>, dim3((unsigned)A), dim3(kAdjTile), 0, st, \
+                     grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag,     \
+                     voxel, (int)A, (int)K, (int)N, inv_norm, pc)
+    if (K <= 256)
+      GERAF_MF_ADJ(1);
+    else if (K <= 512)
+      GERAF_MF_ADJ(2);
+    else
+      GERAF_MF_ADJ(4);
+#undef GERAF_MF_ADJ
+    return check_launch("mf_backward_mfma_kernel");
+  }
   // chunks per block: a power of two dividing kAdjWaves, at most the chunk count
   const int chunks = (int)((K + kAdjChunk - 1) / kAdjChunk);
   int cpb = kAdjWaves;

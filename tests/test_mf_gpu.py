@@ -147,6 +147,34 @@ def test_backward_vs_oracle_and_adjoint_identity(cuda):
     assert abs(lhs_gpu - rhs_gpu) <= 1e-5 * abs(lhs_gpu)
 
 
+@pytest.mark.parametrize("A,K,N", [(5, 200, 1500), (3, 300, 1100), (4, 512, 1030),
+                                   (2, 1000, 700), (2, 1030, 600)])
+def test_backward_matrix_tiles(cuda, A, K, N):
+    """The matrix-core adjoint (128 < K <= 1024: one, two and four 256-sample
+    tiles, partial tiles, several voxel tiles with a partial last one) and the
+    rotation adjoint beyond (K = 1030), against the FP64 oracle."""
+    tx, rx, vox, g = _adjoint_case(cuda, A, K, N, seed=K)
+    fs, As = 1.25e6 * K / 64, 701500.0
+    prm = dict(f0=mf_scenes.MF_F0, slope=As, fs=fs, c=mf_scenes.SPEED_OF_LIGHT,
+               bias=mf_scenes.PATH_BIAS)
+    T = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda)  # noqa: E731
+    g_r, g_i = T(g.real), T(g.imag)
+    o_r = torch.empty((A, K), device=cuda)
+    o_i = torch.empty_like(o_r)
+    from geraf.ops import _native
+
+    params = _native.make_params(prm["f0"], prm["slope"], prm["fs"], prm["c"], prm["bias"])
+    _native.call("geraf_mf_backward_f32", *map(_native.ptr, (g_r, g_i, T(tx), T(rx), o_r, o_i,
+                                                                 T(vox.T))),
+                 A, K, N, A, params, _native.stream_of(o_r))
+    torch.cuda.synchronize()
+    got = o_r.double().cpu().numpy() + 1j * o_i.double().cpu().numpy()
+    ref = orc.mf_backward(g, tx, rx, vox, K, norm=A, **prm)
+    err = rel_l2(got, ref)
+    print(f"adjoint A{A} K{K} N{N}: rel-L2 {err:.2e}")
+    assert err <= 1e-5
+
+
 def test_faster_matched_filter_equivalence(cuda):
     from geraf.ops.RFMatchedFilter import faster_matched_filter_cuda, matched_filter_cuda
 
