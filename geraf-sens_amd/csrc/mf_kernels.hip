@@ -383,16 +383,16 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_kernel(
 // complex one (P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi)).  Operand
 // lanes: A[i][k] and B[k][j] from lane i + 16 k (j + 16 k); D[4 q + v][j] is
 // element v of lane j + 16 q.  Each lane forms one Y and one X phasor per
-// step (FP64-reduced phase, hardware sine and cosine); further tiles follow by
-// one rotation.  After every tile the FP32 matrix sums (64 voxels) are added
+// step (phase j cyc1 and 16 j cyc1 reduced exactly in FP32 from the split
+// step, hardware sine and cosine); further tiles follow by one rotation.  After every tile the FP32 matrix sums (64 voxels) are added
 // into FP64 registers; the eight waves' FP64 sums are reduced through LDS in
 // a fixed order.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct AdjMfmaStage {
-  float2 cg[kAdjTile];    // output gradient times the carrier phasor
-  float2 rot[kAdjTile];   // exp(+j 2 pi 256 cyc1): one 256-frequency tile on
-  double cyc1[kAdjTile];  // per-sample step [cycles]
+  float2 cg[kAdjTile];        // output gradient times the carrier phasor
+  float2 rot[kAdjTile];       // exp(+j 2 pi 256 cyc1): one 256-frequency tile on
+  SplitCycles cyc1[kAdjTile];  // per-sample step [cycles], split for exact multiples
 };
 
 template <int MT>
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mfma_kernel(
       }
       st.cg[threadIdx.x] = cg;
       st.rot[threadIdx.x] = rot;
-      st.cyc1[threadIdx.x] = cyc1;
+      st.cyc1[threadIdx.x] = split_cycles(cyc1);
     }
     __syncthreads();
     f32x4 p1[MT], p2[MT], p3[MT];
@@ -442,9 +442,9 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mfma_kernel(
 #pragma unroll 4
     for (int t = 0; t < 16; ++t) {
       const int i = wave * kWave + 4 * t + (lane >> 4);  // voxel of this lane's step
-      const double c1 = st.cyc1[i];
-      const float2 y = phasor_cycles_hw((float)centered_frac((double)j * c1));
-      float2 x = cmul(st.cg[i], phasor_cycles_hw((float)centered_frac((double)(16 * j) * c1)));
+      const SplitCycles c1 = st.cyc1[i];
+      const float2 y = phasor_cycles_hw(frac_multiple((float)j, c1));
+      float2 x = cmul(st.cg[i], phasor_cycles_hw(frac_multiple((float)(16 * j), c1)));
       float2 rot = make_float2(1.0f, 0.0f);
       if constexpr (MT > 1) rot = st.rot[i];
       const float ys = y.x + y.y;

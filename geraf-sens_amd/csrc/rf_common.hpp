@@ -99,6 +99,26 @@ __device__ __forceinline__ float2 phasor_cycles_hw(float x) {
   return make_float2(__builtin_amdgcn_cosf(x), __builtin_amdgcn_sinf(x));
 }
 
+// A phase step in cycles split for exact small multiples in FP32 (Cody-Waite):
+// hi keeps 16 significant bits, so m * hi is exact for integers 0 <= m < 256;
+// lo = c - hi (|lo| <= 2^-16 |c|).
+struct SplitCycles {
+  float hi, lo;
+};
+__device__ __forceinline__ SplitCycles split_cycles(double c) {
+  SplitCycles s;
+  s.hi = __uint_as_float(__float_as_uint((float)c) & 0xFFFFFF00u);
+  s.lo = (float)(c - (double)s.hi);
+  return s;
+}
+// Centered fraction of m * c for an integer 0 <= m < 256: m * hi, its rint
+// and their difference are exact; only m * lo and the final add round (about
+// 3e-8 cycles, as a cast of the FP64 result to FP32 would).  |result| <= 0.5 + tiny.
+__device__ __forceinline__ float frac_multiple(float m, const SplitCycles& c) {
+  const float p = m * c.hi;
+  return fmaf(m, c.lo, p - rintf(p));
+}
+
 // Complex helpers on float2 = (re, im).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
