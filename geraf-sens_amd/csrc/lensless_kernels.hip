@@ -315,7 +315,8 @@ __device__ __forceinline__ void synth_mfma(const FwdQueue& q, int wave, int lane
 // sample group); MT > 0: matrix-core synthesis with MT tiles of 256
 // frequencies (K <= 256 MT).
 template <int MT>
-__global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_forward_kernel(
+__global__ __launch_bounds__(64 * kRaysPerBlock)
+__attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kernel(
     LenslessArgs a, float* __restrict__ out_real, float* __restrict__ out_imag,
     float* __restrict__ sigma_out, PhaseConsts pc, int groups, int chunks) {
   __shared__ FwdQueue q;
@@ -543,7 +544,8 @@ __device__ __forceinline__ void h_mfma(const double* __restrict__ btau,
 // MT = 0: h_s by two-level Horner (one sample per lane); MT > 0: matrix-core
 // form with MT tiles of 256 frequencies (K <= 256 MT).
 template <int MT>
-__global__ __launch_bounds__(64 * kRaysPerBlock) void lensless_backward_kernel(
+__global__ __launch_bounds__(64 * kRaysPerBlock)
+__attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_kernel(
     LenslessArgs a, const float* __restrict__ g_real, const float* __restrict__ g_imag,
     float* __restrict__ grad_sdf, float* __restrict__ grad_g, float* __restrict__ grad_refl,
     float* __restrict__ grad_inv_s, float* __restrict__ grad_power, PhaseConsts pc) {
