@@ -88,7 +88,7 @@ int geraf_mf_forward_f64(const double* real, const double* imag, const double* t
  * reference divides by its antenna argument; the selected-rows variant of
  * faster_matched_filter_cuda.py:229-238 divides by the total count).
  * Replaces _C.MatchedFilter_backward_realimag (vision.cpp:89-92;
- * MatchedFilter_cuda_kernel.cu:171-220). */
+ * MatchedFilter_cuda_kernel.cu:171-220). Deterministic (fixed-order sums). */
 int geraf_mf_backward_f32(const float* grad_out_real, const float* grad_out_imag,
                           const float* tx_pos, const float* rx_pos, float* grad_real,
                           float* grad_imag, const float* voxel, int64_t A, int64_t K,
@@ -229,7 +229,8 @@ int geraf_lensless_forward_f32(const float* sdf, const float* grad, const float*
  * SDF gradients, through iter_cos and the normalised normal), refl, and per-ray
  * partial sums of the inv_s and power gradients ([R] each; the caller sums
  * them). Recomputes the compositing instead of reading stored intermediates.
- * grad_refl may be NULL. Samples past n_valid get zeros. */
+ * grad_refl may be NULL. Samples past n_valid get zeros. Limits as the
+ * forward (S <= 2048, K <= 2048). */
 int geraf_lensless_backward_f32(const float* grad_out_real, const float* grad_out_imag,
                                 const float* sdf, const float* grad, const float* refl,
                                 const float* dists, const float* z, const float* origin,
