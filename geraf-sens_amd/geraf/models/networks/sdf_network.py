@@ -134,6 +134,74 @@ class _AddBias(torch.autograd.Function):
         return gy, col_sum(gy) if ctx.needs_input_grad[1] else None
 
 
+class _SoftplusBF16(torch.autograd.Function):
+    """Softplus of a CUDA BF16 tensor by the library's kernel (one pass, FP32
+    arithmetic); its backward is differentiable once more."""
+
+    @staticmethod
+    def forward(ctx, x, beta, threshold):
+        from geraf.ops import _native
+
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        _native.call("geraf_softplus_bf16", _native.ptr(x), _native.ptr(y), x.numel(), beta,
+                     threshold, _native.stream_of(x))
+        ctx.save_for_backward(x)
+        ctx.beta, ctx.threshold = beta, threshold
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        (x,) = ctx.saved_tensors
+        return _SoftplusGradBF16.apply(gy, x, ctx.beta, ctx.threshold), None, None
+
+
+class _SoftplusGradBF16(torch.autograd.Function):
+    """gx = g * s(x), s = dsoftplus/dx; its adjoint (both inputs) is one fused
+    kernel.  A third derivative is not provided."""
+
+    @staticmethod
+    def forward(ctx, g, x, beta, threshold):
+        from geraf.ops import _native
+
+        g = g.to(torch.bfloat16).contiguous()
+        gx = torch.empty_like(x)
+        _native.call("geraf_softplus_grad_bf16", _native.ptr(g), _native.ptr(x), _native.ptr(gx),
+                     x.numel(), beta, threshold, _native.stream_of(x))
+        ctx.save_for_backward(g, x)
+        ctx.beta, ctx.threshold = beta, threshold
+        return gx
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, ggx):
+        from geraf.ops import _native
+
+        g, x = ctx.saved_tensors
+        ggx = ggx.to(torch.bfloat16).contiguous()
+        out_g = torch.empty_like(x)
+        out_x = torch.empty_like(x)
+        _native.call("geraf_softplus_grad2_bf16", _native.ptr(ggx), _native.ptr(g), _native.ptr(x),
+                     _native.ptr(out_g), _native.ptr(out_x), x.numel(), ctx.beta, ctx.threshold,
+                     _native.stream_of(x))
+        return out_g, out_x, None, None
+
+
+class SoftplusBF16(nn.Module):
+    """nn.Softplus(beta, threshold); CUDA BF16 inputs go through the fused
+    kernels (forward, backward and double backward in one pass each), other
+    inputs through torch."""
+
+    def __init__(self, beta: float = 1.0, threshold: float = 20.0):
+        super().__init__()
+        self.beta, self.threshold = float(beta), float(threshold)
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        if x.is_cuda and x.dtype == torch.bfloat16:
+            return _SoftplusBF16.apply(x, self.beta, self.threshold)
+        return nn.functional.softplus(x, self.beta, self.threshold)
+
+
 def positional_encoding(x: torch.Tensor, multires: int) -> torch.Tensor:
     """[x, sin(2^i x), cos(2^i x) for i < multires] along the last dim."""
     if multires <= 0:
@@ -171,7 +239,7 @@ class SDFNetwork(nn.Module):
                 lin = nn.utils.parametrizations.weight_norm(lin)
             setattr(self, f"lin{l}", lin)
         if layer_activation == "softplus":
-            self.activation = nn.Softplus(beta=100)
+            self.activation = SoftplusBF16(beta=100)
         elif layer_activation == "relu":
             self.activation = nn.ReLU()
         else:

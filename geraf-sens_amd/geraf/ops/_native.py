@@ -67,6 +67,9 @@ _SIGNATURES = {
     "geraf_bilinear_backward_f64": [_P] * 6 + [_I64] * 4 + [_P],
     "geraf_lensless_forward_f32": [_P] * 14 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
     "geraf_lensless_backward_f32": [_P] * 18 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
+    "geraf_softplus_bf16": [_P] * 2 + [_I64, ctypes.c_float, ctypes.c_float, _P],
+    "geraf_softplus_grad_bf16": [_P] * 3 + [_I64, ctypes.c_float, ctypes.c_float, _P],
+    "geraf_softplus_grad2_bf16": [_P] * 5 + [_I64, ctypes.c_float, ctypes.c_float, _P],
     "geraf_mf_forward_tiled_f32": [_P] * 8 + [_I64] * 3 + [ctypes.c_int32, _P, _I64, _PARAMS,
                                                             _P],
     "geraf_mf_backward_tiled_f32": [_P] * 8 + [_I64] * 4 + [ctypes.c_int32, _P, _I64, _PARAMS,

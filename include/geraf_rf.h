@@ -272,6 +272,27 @@ int geraf_mf_backward_tiled_f32(const float* grad_out_real, const float* grad_ou
                                 int32_t nterms, void* workspace, int64_t workspace_bytes,
                                 const geraf_fmcw_params* params, geraf_stream_t stream);
 
+/* ---------------------------------------------------------------- MLP activation (BF16) */
+
+/* Softplus of the SDF network's BF16 hidden layers and its derivatives
+ * (torch.nn.Softplus(beta, threshold) semantics; the network uses beta = 100,
+ * threshold = 20, sdf_network.py:78-199). x, y, g, ... are n raw BF16 values
+ * (16-bit patterns); arithmetic is FP32, results are rounded to nearest even.
+ * With t = beta x and s = (t > threshold ? 1 : sigmoid(t)):
+ *   geraf_softplus_bf16:       y = (t > threshold ? x : log1p(exp(t)) / beta)
+ *   geraf_softplus_grad_bf16:  gx = g s
+ *   geraf_softplus_grad2_bf16: out_g = ggx s,  out_x = ggx g beta s (1 - s)
+ *                              (the adjoint of the second, for double backward)
+ * No counterpart in the reference's C++: they replace torch's softplus
+ * kernels (forward, backward and its double backward) for these layers. */
+int geraf_softplus_bf16(const uint16_t* x, uint16_t* y, int64_t n, float beta, float threshold,
+                        geraf_stream_t stream);
+int geraf_softplus_grad_bf16(const uint16_t* g, const uint16_t* x, uint16_t* gx, int64_t n,
+                             float beta, float threshold, geraf_stream_t stream);
+int geraf_softplus_grad2_bf16(const uint16_t* ggx, const uint16_t* g, const uint16_t* x,
+                              uint16_t* out_g, uint16_t* out_x, int64_t n, float beta,
+                              float threshold, geraf_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

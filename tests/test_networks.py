@@ -69,3 +69,42 @@ def test_reflectance_network_tall_matches_plain():
     g2 = torch.autograd.grad(r2.sum(), list(plain.refelective_predictor.parameters()) + [feat])
     for a, b in zip(g1, g2):
         assert torch.allclose(a, b, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [4096 * 256, 1000 * 39 + 3])
+def test_softplus_bf16_kernels(cuda, n):
+    """The fused BF16 softplus (forward, backward, double backward) against
+    FP64 autograd on the same BF16 inputs: every output within one BF16
+    rounding (2^-8 relative) of the FP64 value, and never farther from it than
+    torch's own BF16 chain plus one rounding.  Inputs straddle the linear
+    threshold (beta x in [-60, 60]); n = 39003 covers the unaligned tail.
+    (FP32 autograd is not the reference: its 1 - sigmoid cancels to 0 for
+    beta x in about [16.6, 20], where the curvature is ~beta exp(-beta x).)"""
+    from torch.nn import functional as F
+
+    from geraf.models.networks.sdf_network import SoftplusBF16
+
+    torch.manual_seed(0)
+    x = ((torch.rand(n, device=cuda) - 0.5) * 1.2).bfloat16().requires_grad_(True)
+    g = torch.randn(n, device=cuda).bfloat16().requires_grad_(True)
+    gg = torch.randn(n, device=cuda).bfloat16()
+
+    def chain(act, xx, gg_in, g_in):
+        y = act(xx)
+        (gx,) = torch.autograd.grad(y, xx, g_in, create_graph=True)
+        gx_x, gx_g = torch.autograd.grad(gx, (xx, g_in), gg_in)
+        return [t.detach().double() for t in (y, gx, gx_x, gx_g)]
+
+    ours = chain(SoftplusBF16(beta=100), x, gg, g)
+    tb = chain(torch.nn.Softplus(beta=100), x, gg, g)
+    xf = x.detach().double().requires_grad_(True)
+    gf = g.detach().double().requires_grad_(True)
+    ref = chain(lambda t: F.softplus(t, beta=100), xf, gg.double(), gf)
+    for name, o, t, r in zip(("y", "dy", "d2y/dx", "d2y/dg"), ours, tb, ref):
+        tol = 2.0 ** -8 * r.abs() + 1e-30
+        err = (o - r).abs()
+        print(f"softplus {name}: max err/|ref| ours {float((err / (r.abs() + 1e-30)).max()):.2e}, "
+              f"torch bf16 {float(((t - r).abs() / (r.abs() + 1e-30)).max()):.2e}")
+        assert bool((err <= tol).all()), name
+        assert bool((err <= (t - r).abs() + tol).all()), name
