@@ -232,3 +232,61 @@ def test_lensless_matrix_tiles(cuda, R, S, K):
         print(f"  grad {k}: rel-L2 {e:.2e}")
         assert e <= TOL_GRAD, k
         assert np.all(grads[k].numpy()[empty] == 0)
+
+
+@pytest.mark.parametrize("K", [512, 64])
+def test_lensless_gradients_vs_finite_differences(cuda, K):
+    """The GPU adjoint against FP64 central differences (h = 1e-4) of the loss
+    Re<out, g> through the oracle, on the matrix-core (K = 512) and vector
+    (K = 64) paths: every checked entry of the sdf, gradient, reflectance,
+    inv_s and power gradients within 1e-4 of the difference quotient (scale
+    max(|fd|, 1e-3 max|grad|)); picks whose +-h step crosses a branch of the
+    composition (culling, the iter_cos relu) are skipped, as in the oracle's
+    own check."""
+    b = ts.rays(6, 96, K, seed=20 + K, uneven=(8, 96))
+    _, grads = _lensless_run(b, cuda, seed=K)
+    gK = torch.tensor(grads["_g"])
+    f64 = lambda t: None if t is None else t.double()  # noqa: E731
+    pts = L.points_f32(b.o, b.z, b.d)
+    r = b.radar
+    kw = dict(K=r["numADCSample"], fs=r["adcSampleRate"], As_sci=r["As_sci"], f0=r["fc_start"],
+              c=ts.SPEED_OF_LIGHT, bias=ts.PATH_BIAS)
+    nv = None if b.n_valid is None else b.n_valid.long()
+    base = {k: getattr(b, k).double() for k in ("sdf", "grad", "refl", "inv_s", "power")}
+
+    def loss(v):
+        o = L.lensless_forward(v["sdf"], v["grad"], f64(b.dists), pts, f64(b.o), f64(b.d),
+                               v["inv_s"], v["power"], refl=v["refl"], offset=f64(b.offset),
+                               n_valid=nv, anneal=b.anneal, **kw)
+        return float((o.real * gK.real + o.imag * gK.imag).sum())
+
+    def masks(v):
+        return L.masks(v["sdf"], v["grad"], f64(b.dists), pts, f64(b.o), f64(b.d), v["inv_s"],
+                       v["power"], refl=v["refl"], offset=f64(b.offset), n_valid=nv,
+                       anneal=b.anneal)
+
+    rng = np.random.default_rng(K)
+    h = 1e-4
+    checked = 0
+    for name in ("inv_s", "power", "sdf", "refl", "grad"):
+        flat = base[name].reshape(-1)
+        gpu = grads[name].reshape(-1)
+        scale0 = 1e-3 * float(gpu.abs().max())
+        picks = [0] if flat.numel() == 1 else rng.choice(flat.numel(), 10, False)
+        for i in picks:
+            def shifted(dv):
+                v = dict(base)
+                t = flat.clone()
+                t[i] += dv
+                v[name] = t.reshape(base[name].shape)
+                return v
+            mp, mm = masks(shifted(h)), masks(shifted(-h))
+            if not torch.equal(mp["live"], mm["live"]) or not torch.equal(
+                    mp["true_cos"] < 0, mm["true_cos"] < 0):
+                continue
+            fd = (loss(shifted(h)) - loss(shifted(-h))) / (2 * h)
+            ad = float(gpu[i])
+            assert abs(fd - ad) <= 1e-4 * max(abs(fd), scale0), (name, int(i), fd, ad)
+            checked += 1
+    print(f"lensless K={K}: {checked} gradient entries within 1e-4 of central differences")
+    assert checked >= 25
