@@ -16,6 +16,11 @@ gfx950 kernel, inputs resident in HBM.  Prints ONE JSON line with:
                  (the FP64 form itself pinned to the CPU FP64 oracle);
 * plain forms  - the plain FP32 (per-term sincosf) and plain FP64 GPU forms
                  timed on a voxel slab, and the tuned speed-up over them.
+Further legs in the same line: the matched-filter adjoint, the tiled
+expansion form, the fused lensless tracer (4,096 x 128, K=256 with its CPU
+baseline, and 32,768 x 256, K=512), the stage-1-style step against its
+all-FP32 autograd form, the 256^3 target volume (direct slab, whole volume by
+tiles, FP64 check on the central 64^3) and the stage-2-style step.
 """
 from __future__ import annotations
 
@@ -272,7 +277,69 @@ def bench_large_target(dev):
                                 "terms": info["terms"], "Z": info["Z"],
                                 "rel_l2_vs_direct_on_slab": rel,
                                 "speedup_vs_direct": (A * K * 256 ** 3 / rate) / full_s}
-    return res
+
+    # errors against the FP64 GPU form on voxels of the central 64^3 subvolume
+    # (a full 64^3 subvolume is 4.4e15 FP64 terms at this aperture: a sample)
+    rng = np.random.default_rng(64)
+    ijk = rng.integers(96, 160, (24, 3))
+    idx = torch.tensor((ijk[:, 0] * 256 + ijk[:, 1]) * 256 + ijk[:, 2], device=dev)
+    d64 = lambda x: torch.tensor(np.ascontiguousarray(x), device=dev, dtype=torch.float64)  # noqa: E731
+    ops = [d64(sc.echo.real), d64(sc.echo.imag), d64(sc.tx), d64(sc.rx)]
+    sub = full[idx].double().t().contiguous()
+    s_r = torch.empty(idx.numel(), device=dev, dtype=torch.float64)
+    s_i = torch.empty_like(s_r)
+    t0 = time.perf_counter()
+    _native.call("geraf_mf_forward_f64", *map(_native.ptr, ops + [s_r, s_i, sub]), A, K,
+                 idx.numel(), prm, _native.stream_of(s_r))
+    torch.cuda.synchronize()
+    f64_s = time.perf_counter() - t0
+    ref64 = torch.complex(s_r, s_i).cpu().numpy()
+    got64 = torch.complex(f_r[idx].double(), f_i[idx].double()).cpu().numpy()
+    err64 = float(np.linalg.norm(got64 - ref64) / np.linalg.norm(ref64))
+    mag = float(np.max(np.abs(np.abs(got64) - np.abs(ref64))) / np.max(np.abs(ref64)))
+    ph = float(np.max(np.abs(np.angle(got64 * np.conj(ref64)))))
+    log(f"256^3 target vs FP64 on {idx.numel()} voxels of the central 64^3 ({f64_s:.1f} s of "
+        f"FP64): rel-L2 {err64:.2e}, peak-magnitude {mag:.2e}, max phase {ph:.2e} rad")
+    res["tiled_full_volume"]["vs_fp64_central_64c_sample"] = {
+        "voxels": int(idx.numel()), "rel_l2": err64, "peak_magnitude_err": mag,
+        "max_phase_err_rad": ph}
+    vol = torch.complex(f_r, f_i).reshape(256, 256, 256)
+    return res, vol, tuple(a.cpu() for a in ax)
+
+
+def bench_stage2_step(dev, target_volume, axes, steps: int = 2, warmup: int = 1):
+    """Stage-2-style step: 32,768 rays x 256 samples (8.4M samples), K=512,
+    fused tracer, BF16 SDF + signal MLPs, the vision-SDF consistency term on
+    262,144 points, matched filter of all rays onto all samples (tiled), L1
+    against |MF| of the 256^3 target."""
+    import torch
+
+    from rf_workloads import stage_step as st
+
+    torch.cuda.reset_peak_memory_stats()
+    step = st.StageStep(st.STAGE2, target_volume, axes, dev, form="fused")
+    for _ in range(warmup):
+        step.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = step.step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    samples = int(step.valid.sum())
+    peak = torch.cuda.max_memory_allocated() / 2 ** 30
+    out = {"workload": "stage-2-style step: 32768 rays x 256 samples, K=512, SDF MLP 8x256 + "
+                       "signal MLP 4x256 (BF16), consistency term on 262144 points, L1 vs |MF| "
+                       "of the 256^3 target",
+           "ms_per_step": ms, "samples_per_step": samples, "samples_per_s": samples / ms * 1e3,
+           "last_loss": loss, "peak_memory_gib": peak,
+           "mf_tiled_fraction": step.plan.tiled_fraction,
+           "mf_bucket_terms": [b.terms for b in step.plan.buckets]}
+    log(f"stage-2 step: {ms:.0f} ms/step, {samples} samples, peak {peak:.1f} GiB, "
+        f"MF tiled fraction {step.plan.tiled_fraction:.3f}")
+    del step
+    torch.cuda.empty_cache()
+    return out
 
 
 def emitting_fraction(b) -> float:
@@ -546,12 +613,14 @@ def main() -> int:
                          f"(per-term cos/sin) on {desc}"}
         log(f"cpu baseline {rate:.3e} terms/s on {threads} threads")
 
-    stage = large = None
+    stage = large = stage2 = None
     if not args.no_stage:
         axes = tuple(torch.tensor(a) for a in sc.axes)
         vol = torch.complex(o_r, o_i).reshape(sc.grid_shape)
         stage = bench_stage_step(dev, vol, axes)
-        large = bench_large_target(dev)
+        large, vol2, axes2 = bench_large_target(dev)
+        stage2 = bench_stage2_step(dev, vol2, axes2)
+        del vol2
 
     tracer = None
     if not args.no_tracer:
@@ -594,6 +663,7 @@ def main() -> int:
         "tracer": tracer,
         "stage1_step": stage,
         "stage2_target_volume": large,
+        "stage2_step": stage2,
     }
     print(json.dumps(line), flush=True)
     return 0
