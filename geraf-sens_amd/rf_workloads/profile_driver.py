@@ -13,7 +13,9 @@ the copy and the matched filter.  --stage1-step N instead runs two warm-up
 and N timed stage-1-style training steps (8,192 rays x 64 samples, K=64;
 fused form, BF16 MLPs) for a kernel trace of the whole step; the target
 volume is a smooth synthetic blob on the 32^3 grid, which fixes the work but
-not the values.  --tiled runs the per-tile expansion matched filter and its
+not the values.  --stage2-step N does the same for the stage-2-style step
+(32,768 rays x 256 samples, K=512, consistency term; one warm-up step).
+--tiled runs the per-tile expansion matched filter and its
 adjoint on the 64x64 MIMO, K=512, 128^3 workload (twice: the second call is
 the one to read).
 """
@@ -61,6 +63,7 @@ def main() -> None:
     ap.add_argument("--stage2", action="store_true")
     ap.add_argument("--tracer-only", action="store_true")
     ap.add_argument("--stage1-step", type=int, default=0)
+    ap.add_argument("--stage2-step", type=int, default=0)
     ap.add_argument("--form", default="fused", choices=("fused", "fp32_autograd"))
     ap.add_argument("--tiled", action="store_true")
     args = ap.parse_args()
@@ -70,6 +73,9 @@ def main() -> None:
         return
     if args.stage1_step:
         stage_steps(dev, args.stage1_step, args.form)
+        return
+    if args.stage2_step:
+        stage_steps(dev, args.stage2_step, "fused", stage=2)
         return
     if args.tracer_only:
         trace_sizes(dev, args.stage2)
@@ -119,7 +125,7 @@ def tiled(dev) -> None:
     print("profile_driver: done", flush=True)
 
 
-def stage_steps(dev, n: int, form: str) -> None:
+def stage_steps(dev, n: int, form: str, stage: int = 1) -> None:
     import time
 
     from rf_workloads import stage_step as st
@@ -128,16 +134,17 @@ def stage_steps(dev, n: int, form: str) -> None:
     g = [torch.tensor(a) for a in sc.axes]
     X, Y, Z = torch.meshgrid(*g, indexing="ij")
     vol = torch.exp(-((X ** 2 + Y ** 2 + (Z - 0.25) ** 2) / 0.03 ** 2)).to(dev)
-    step = st.StageStep(st.STAGE1, vol.to(torch.complex64), tuple(sc.axes), dev, form=form)
-    for _ in range(2):
+    cfg = st.STAGE1 if stage == 1 else st.STAGE2
+    step = st.StageStep(cfg, vol.to(torch.complex64), tuple(sc.axes), dev, form=form)
+    for _ in range(2 if stage == 1 else 1):
         step.step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(n):
         step.step()
     torch.cuda.synchronize()
-    print(f"profile_driver: stage-1 step [{form}] {(time.perf_counter() - t0) / n * 1e3:.1f} ms",
-          flush=True)
+    print(f"profile_driver: stage-{stage} step [{form}] "
+          f"{(time.perf_counter() - t0) / n * 1e3:.1f} ms", flush=True)
 
 
 def trace_sizes(dev, stage2: bool) -> None:
