@@ -20,8 +20,9 @@
 // phasor e^{-j2pi kappa d tau_ac} is a rotation re-seeded every 32 samples
 // from the FP64-reduced phase), then mf_tile_eval_kernel (block = tile, two
 // voxels per lane, the tile's coefficients of one antenna are wave-uniform
-// SGPR operands; per pair: FP64 distances and phase, one phasor, an N_T-term
-// real Horner with complex coefficients, Kahan accumulation), adding each
+// SGPR operands; per pair: FP64 distances and phase reduction, one phasor
+// from the hardware sine and cosine, an N_T-term real Horner with complex
+// coefficients, Kahan accumulation), adding each
 // chunk's sums into FP64 accumulators; a last kernel scales by 1/A.
 //
 // The adjoint (MatchedFilter_cuda_kernel.cu:171-220) is the transpose of the
@@ -204,8 +205,8 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
         Pr = fmaf(Pr, u, cf[n].x);
         Pi = fmaf(Pi, u, cf[n].y);
       }
-      const float2 base = phasor_cycles((float)centered_frac(fcen * tau), -1.0f);
-      acc[i].add(cmul(base, make_float2(Pr, Pi)));
+      const float2 cs = phasor_cycles_hw((float)centered_frac(fcen * tau));
+      acc[i].add(cmul(make_float2(cs.x, -cs.y), make_float2(Pr, Pi)));
     }
   }
 #pragma unroll
@@ -280,14 +281,14 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
                           dist3_f64_fast(rx, ry, rz, px[i], py[i], pz[i]) + pc.bias) *
                          pc.inv_c;
       const float u = (float)((tau - tac) * invH);
-      float2 p = cmul(make_float2(cr[i], ci[i]),
-                      phasor_cycles((float)centered_frac(fcen * tau), 1.0f));
+      const float2 p = cmul(make_float2(cr[i], ci[i]),
+                            phasor_cycles_hw((float)centered_frac(fcen * tau)));
+      float q = 1.0f;  // u^n
 #pragma unroll
       for (int n = 0; n < NT; ++n) {
-        Pr[n] += p.x;
-        Pi[n] += p.y;
-        p.x *= u;
-        p.y *= u;
+        Pr[n] = fmaf(p.x, q, Pr[n]);
+        Pi[n] = fmaf(p.y, q, Pi[n]);
+        q *= u;
       }
     }
     // M'_n = j^n / n! * sum over the tile
