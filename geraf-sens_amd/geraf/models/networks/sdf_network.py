@@ -142,10 +142,12 @@ class _SoftplusBF16(torch.autograd.Function):
     def forward(ctx, x, beta, threshold):
         from geraf.ops import _native
 
-        x = x.contiguous()
-        y = torch.empty_like(x)
-        _native.call("geraf_softplus_bf16", _native.ptr(x), _native.ptr(y), x.numel(), beta,
-                     threshold, _native.stream_of(x))
+        xc = x.contiguous()
+        y = torch.empty_like(xc)
+        _native.call("geraf_softplus_bf16", _native.ptr(xc), _native.ptr(y), xc.numel(), beta,
+                     threshold, _native.stream_of(xc))
+        # the input itself (not a copy): the double backward reaches it through
+        # _SoftplusGradBF16
         ctx.save_for_backward(x)
         ctx.beta, ctx.threshold = beta, threshold
         return y
@@ -164,13 +166,14 @@ class _SoftplusGradBF16(torch.autograd.Function):
     def forward(ctx, g, x, beta, threshold):
         from geraf.ops import _native
 
-        g = g.to(torch.bfloat16).contiguous()
-        gx = torch.empty_like(x)
-        _native.call("geraf_softplus_grad_bf16", _native.ptr(g), _native.ptr(x), _native.ptr(gx),
-                     x.numel(), beta, threshold, _native.stream_of(x))
-        ctx.save_for_backward(g, x)
+        gc = g.to(torch.bfloat16).contiguous()
+        xc = x.contiguous()
+        gx = torch.empty_like(xc)
+        _native.call("geraf_softplus_grad_bf16", _native.ptr(gc), _native.ptr(xc),
+                     _native.ptr(gx), xc.numel(), beta, threshold, _native.stream_of(xc))
+        ctx.save_for_backward(gc, xc)
         ctx.beta, ctx.threshold = beta, threshold
-        return gx
+        return gx.view(x.shape)
 
     @staticmethod
     @torch.autograd.function.once_differentiable

@@ -72,13 +72,15 @@ def test_reflectance_network_tall_matches_plain():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [4096 * 256, 1000 * 39 + 3])
+@pytest.mark.parametrize("n", [4096 * 256, 1000 * 39 + 3, -(256 * 64)])
 def test_softplus_bf16_kernels(cuda, n):
     """The fused BF16 softplus (forward, backward, double backward) against
     FP64 autograd on the same BF16 inputs: every output within one BF16
     rounding (2^-8 relative) of the FP64 value, and never farther from it than
     torch's own BF16 chain plus one rounding.  Inputs straddle the linear
-    threshold (beta x in [-60, 60]); n = 39003 covers the unaligned tail.
+    threshold (beta x in [-60, 60]); n = 39003 covers the unaligned tail; a
+    negative n feeds a transposed (non-contiguous) view, whose second-order
+    gradients must still reach it.
     (FP32 autograd is not the reference: its 1 - sigmoid cancels to 0 for
     beta x in about [16.6, 20], where the curvature is ~beta exp(-beta x).)"""
     from torch.nn import functional as F
@@ -86,9 +88,16 @@ def test_softplus_bf16_kernels(cuda, n):
     from geraf.models.networks.sdf_network import SoftplusBF16
 
     torch.manual_seed(0)
-    x = ((torch.rand(n, device=cuda) - 0.5) * 1.2).bfloat16().requires_grad_(True)
-    g = torch.randn(n, device=cuda).bfloat16().requires_grad_(True)
-    gg = torch.randn(n, device=cuda).bfloat16()
+    if n > 0:
+        x = ((torch.rand(n, device=cuda) - 0.5) * 1.2).bfloat16().requires_grad_(True)
+        shape = (n,)
+    else:
+        base = ((torch.rand(256, -n // 256, device=cuda) - 0.5) * 1.2).bfloat16()
+        x = base.requires_grad_(True).t()
+        assert not x.is_contiguous()
+        shape = x.shape
+    g = torch.randn(shape, device=cuda).bfloat16().requires_grad_(True)
+    gg = torch.randn(shape, device=cuda).bfloat16()
 
     def chain(act, xx, gg_in, g_in):
         y = act(xx)
