@@ -4,8 +4,9 @@
 // evaluated to a stated truncation bound with O(N_T) work per voxel-antenna
 // pair instead of O(K).
 //
-// Voxels come in tiles of 512 (the caller orders them so that a tile is
-// compact in space).  A tile with centre c and radius r sees every antenna's
+// Voxels come in tiles of 512 or 128 (the caller orders them so that a tile
+// is compact in space and picks the size: smaller tiles need fewer terms and
+// cost more per-tile moments).  A tile with centre c and radius r sees every antenna's
 // delays within H = 2 r / c of the centre delay tau_ac (triangle inequality).
 // With k = k_c + kappa, k_c = (K-1)/2, u = (tau_av - tau_ac)/H in [-1, 1] and
 // xi_k = kappa / k_c in [-1, 1]:
@@ -45,7 +46,7 @@
 namespace geraf {
 namespace {
 
-constexpr int kTileVox = 512;  // voxels per tile (4 waves x 64 lanes x 2)
+constexpr int kTileVox = 512;  // the larger tile: 4 waves x 64 lanes x 2 voxels
 constexpr int kEvalWaves = 4;
 constexpr int kMomentWaves = 4;
 
@@ -150,7 +151,9 @@ __global__ __launch_bounds__(64 * kMomentWaves) void mf_tile_moments_kernel(
   tau_c[(size_t)ai * ntiles + t] = tac;
 }
 
-template <int NT>
+// TW waves per tile (tiles of 128 TW voxels); a block of kEvalWaves waves
+// takes kEvalWaves / TW tiles.
+template <int NT, int TW>
 __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
     const float* __restrict__ voxel, const float4* __restrict__ tile_info,
@@ -158,6 +161,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
     int ntiles, PhaseConsts pc, double fcen, double* __restrict__ acc_re,
     double* __restrict__ acc_im, int per_group, size_t acc_stride) {
   constexpr int VPL = 2;
+  constexpr int TV = kWave * VPL * TW;  // voxels per tile
   // antenna group of this block: antennas [ai_lo, ai_hi) of the chunk, into
   // its own accumulator slice
   const int ai_lo = blockIdx.y * per_group;
@@ -166,7 +170,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
   acc_im += blockIdx.y * acc_stride;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int t = blockIdx.x;
+  const int t = blockIdx.x * (kEvalWaves / TW) + wave / TW;
+  if (t >= ntiles) return;  // wave-uniform; no block-level synchronisation here
+  const int wt = wave % TW;  // wave within the tile
   const TileGeom g = tile_geom(tile_info, t, pc.inv_c);
   const double invH = g.H > 0.0 ? 1.0 / g.H : 0.0;
 
@@ -175,7 +181,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
   double px[VPL], py[VPL], pz[VPL];
 #pragma unroll
   for (int i = 0; i < VPL; ++i) {
-    v[i] = t * kTileVox + (wave * VPL + i) * kWave + lane;
+    v[i] = t * TV + (wt * VPL + i) * kWave + lane;
     valid[i] = v[i] < N;
     const int vc = valid[i] ? v[i] : N - 1;
     px[i] = voxel[vc];
@@ -235,9 +241,8 @@ __global__ void mf_tile_finish_kernel(const double* __restrict__ acc_re,
 
 // ------------------------------------------------------------------ adjoint
 
-constexpr int kAdjVPL = kTileVox / kWave;  // 8 voxels per lane: one wave per tile
-
-template <int NT>
+// One wave per tile of kWave * AVPL voxels (AVPL voxels per lane).
+template <int NT, int AVPL>
 __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
     const float* __restrict__ gout_re, const float* __restrict__ gout_im,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
@@ -252,11 +257,11 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
   if (t >= ntiles) return;  // wave-uniform
   const TileGeom g = tile_geom(tile_info, t, pc.inv_c);
   const double invH = g.H > 0.0 ? 1.0 / g.H : 0.0;
-  double px[kAdjVPL], py[kAdjVPL], pz[kAdjVPL];
-  float cr[kAdjVPL], ci[kAdjVPL];
+  double px[AVPL], py[AVPL], pz[AVPL];
+  float cr[AVPL], ci[AVPL];
 #pragma unroll
-  for (int i = 0; i < kAdjVPL; ++i) {
-    const int v = t * kTileVox + i * kWave + lane;
+  for (int i = 0; i < AVPL; ++i) {
+    const int v = t * (kWave * AVPL) + i * kWave + lane;
     const bool ok = v < N;
     const int vc = ok ? v : N - 1;
     px[i] = voxel[vc];
@@ -276,7 +281,7 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
 #pragma unroll
     for (int n = 0; n < NT; ++n) Pr[n] = Pi[n] = 0.0f;
 #pragma unroll
-    for (int i = 0; i < kAdjVPL; ++i) {
+    for (int i = 0; i < AVPL; ++i) {
       const double tau = (dist3_f64_fast(tx, ty, tz, px[i], py[i], pz[i]) +
                           dist3_f64_fast(rx, ry, rz, px[i], py[i], pz[i]) + pc.bias) *
                          pc.inv_c;
@@ -364,7 +369,7 @@ __global__ __launch_bounds__(256) void mf_tile_adj_eval_kernel(
 template <int NT, int KPL>
 void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const float* rx,
                       const float* voxel, const float4* tile_info, int a0, int na, int K, int N,
-                      int ntiles, const PhaseConsts& pc, double kc, double inv_norm,
+                      int ntiles, int tv, const PhaseConsts& pc, double kc, double inv_norm,
                       float2* mom, double* tau_c, float* grad_re, float* grad_im,
                       hipStream_t stream) {
   const double fcen = pc.f0 + kc * pc.dfs;
@@ -373,10 +378,15 @@ void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const
   int g = 1;
   while (g < 64 && g * 2 <= na && (int64_t)ntiles * g < 32768) g *= 2;
   const int per_group = (na + g - 1) / g;
-  hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT>),
-                     dim3((unsigned)((ntiles + 3) / 4), (unsigned)g), dim3(256), 0, stream, gre,
-                     gim, tx, rx, voxel, tile_info, a0, na, N, ntiles, pc, fcen, mom, tau_c,
-                     per_group);
+  const dim3 mgrid((unsigned)((ntiles + 3) / 4), (unsigned)g);
+  if (tv == 512)
+    hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT, 8>), mgrid, dim3(256), 0, stream, gre,
+                       gim, tx, rx, voxel, tile_info, a0, na, N, ntiles, pc, fcen, mom, tau_c,
+                       per_group);
+  else
+    hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT, 2>), mgrid, dim3(256), 0, stream, gre,
+                       gim, tx, rx, voxel, tile_info, a0, na, N, ntiles, pc, fcen, mom, tau_c,
+                       per_group);
   hipLaunchKernelGGL((mf_tile_adj_eval_kernel<NT, KPL>), dim3((unsigned)((na + 3) / 4)),
                      dim3(256), 0, stream, mom, tau_c, tile_info, a0, na, K, ntiles, pc, kc,
                      inv_norm, grad_re, grad_im);
@@ -385,13 +395,13 @@ void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const
 template <int NT>
 void launch_adj_chunk_k(int kpl, const float* gre, const float* gim, const float* tx,
                         const float* rx, const float* voxel, const float4* tile_info, int a0,
-                        int na, int K, int N, int ntiles, const PhaseConsts& pc, double kc,
-                        double inv_norm, float2* mom, double* tau_c, float* grad_re,
+                        int na, int K, int N, int ntiles, int tv, const PhaseConsts& pc,
+                        double kc, double inv_norm, float2* mom, double* tau_c, float* grad_re,
                         float* grad_im, hipStream_t stream) {
 #define GERAF_ADJ_K(P)                                                                        \
   case P:                                                                                     \
-    launch_adj_chunk<NT, P>(gre, gim, tx, rx, voxel, tile_info, a0, na, K, N, ntiles, pc, kc, \
-                            inv_norm, mom, tau_c, grad_re, grad_im, stream);                  \
+    launch_adj_chunk<NT, P>(gre, gim, tx, rx, voxel, tile_info, a0, na, K, N, ntiles, tv, pc, \
+                            kc, inv_norm, mom, tau_c, grad_re, grad_im, stream);              \
     break;
   switch (kpl) {
     GERAF_ADJ_K(1)
@@ -431,10 +441,11 @@ struct TiledLayout {
   int groups;
 };
 
-TiledLayout tiled_layout(int64_t K, int64_t N, int NT) {
-  const int64_t ntiles = (N + kTileVox - 1) / kTileVox;
+TiledLayout tiled_layout(int64_t K, int64_t N, int NT, int tv) {
+  const int64_t ntiles = (N + tv - 1) / tv;
   TiledLayout l;
-  l.groups = antenna_groups(ntiles);
+  // evaluation blocks: one per 512-voxel tile, one per four 128-voxel tiles
+  l.groups = antenna_groups(tv == 512 ? ntiles : (ntiles + 3) / 4);
   l.acc_bytes = align256((size_t)N * sizeof(double)) * l.groups;
   l.xi_bytes = align256((size_t)K * NT * sizeof(float));
   l.per_antenna = (size_t)ntiles * (NT * sizeof(float2) + sizeof(double));
@@ -445,7 +456,7 @@ TiledLayout tiled_layout(int64_t K, int64_t N, int NT) {
 template <int NT>
 void launch_chunk(const float* real, const float* imag, const float* tx, const float* rx,
                   const float* voxel, const float4* tile_info, const float* xi_pow, int a0,
-                  int na, int K, int N, int ntiles, const PhaseConsts& pc, double kc,
+                  int na, int K, int N, int ntiles, int tv, const PhaseConsts& pc, double kc,
                   float2* coef, double* tau_c, double* acc_re, double* acc_im, int groups,
                   size_t acc_stride, hipStream_t stream) {
   const dim3 mgrid((unsigned)((ntiles + kWave - 1) / kWave),
@@ -456,9 +467,15 @@ void launch_chunk(const float* real, const float* imag, const float* tx, const f
   const double fcen = pc.f0 + kc * pc.dfs;
   const int g = std::min(groups, na);
   const int per_group = (na + g - 1) / g;
-  hipLaunchKernelGGL((mf_tile_eval_kernel<NT>), dim3((unsigned)ntiles, (unsigned)g),
-                     dim3(64 * kEvalWaves), 0, stream, tx, rx, voxel, tile_info, coef, tau_c, a0,
-                     na, N, ntiles, pc, fcen, acc_re, acc_im, per_group, acc_stride);
+  if (tv == 512)
+    hipLaunchKernelGGL((mf_tile_eval_kernel<NT, 4>), dim3((unsigned)ntiles, (unsigned)g),
+                       dim3(64 * kEvalWaves), 0, stream, tx, rx, voxel, tile_info, coef, tau_c,
+                       a0, na, N, ntiles, pc, fcen, acc_re, acc_im, per_group, acc_stride);
+  else
+    hipLaunchKernelGGL((mf_tile_eval_kernel<NT, 1>),
+                       dim3((unsigned)((ntiles + kEvalWaves - 1) / kEvalWaves), (unsigned)g),
+                       dim3(64 * kEvalWaves), 0, stream, tx, rx, voxel, tile_info, coef, tau_c,
+                       a0, na, N, ntiles, pc, fcen, acc_re, acc_im, per_group, acc_stride);
 }
 
 }  // namespace
@@ -503,11 +520,18 @@ TiledBuffers carve(void* workspace, int64_t workspace_bytes, const TiledLayout& 
 }  // namespace
 }  // namespace geraf
 
+namespace geraf {
+namespace {
+// Tile sizes the kernels are instantiated for.
+bool tile_size_ok(int32_t tv) { return tv == 128 || tv == kTileVox; }
+}  // namespace
+}  // namespace geraf
+
 extern "C" int64_t geraf_mf_tiled_workspace_bytes(int64_t A, int64_t K, int64_t N,
-                                                  int32_t nterms) {
+                                                  int32_t nterms, int32_t tile_voxels) {
   const int NT = round_terms(nterms);
-  if (A < 0 || K < 0 || N < 0 || NT < 0) return -1;
-  const TiledLayout l = tiled_layout(K, N, NT);
+  if (A < 0 || K < 0 || N < 0 || NT < 0 || !tile_size_ok(tile_voxels)) return -1;
+  const TiledLayout l = tiled_layout(K, N, NT, tile_voxels);
   return (int64_t)(l.fixed + 2 * 256 + (size_t)A * l.per_antenna);
 }
 
@@ -515,13 +539,14 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
                                           const float* tx_pos, const float* rx_pos,
                                           float* out_real, float* out_imag, const float* voxel,
                                           const float* tile_info, int64_t A, int64_t K,
-                                          int64_t N, int32_t nterms, void* workspace,
-                                          int64_t workspace_bytes,
+                                          int64_t N, int32_t nterms, int32_t tile_voxels,
+                                          void* workspace, int64_t workspace_bytes,
                                           const geraf_fmcw_params* params,
                                           geraf_stream_t stream) {
   GERAF_TRY_BEGIN
   check_params(params);
   if (A < 1 || K < 1 || N < 0) throw ArgError("need A >= 1, K >= 1, N >= 0");
+  if (!tile_size_ok(tile_voxels)) throw ArgError("tile_voxels must be 128 or 512");
   if (N == 0) return GERAF_RF_OK;
   if (A * K >= (INT64_C(1) << 31) || N >= (INT64_C(1) << 31) - kTileVox)
     throw ArgError("A*K and N must be < 2^31");
@@ -529,8 +554,9 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
   if (nterms < 1 || NT < 0) throw ArgError("nterms must be in [1, 32]");
   require_ptrs({real, imag, tx_pos, rx_pos, out_real, out_imag, voxel, tile_info, workspace},
                true);
-  const TiledLayout l = tiled_layout(K, N, NT);
-  const int ntiles = (int)((N + kTileVox - 1) / kTileVox);
+  const int tv = tile_voxels;
+  const TiledLayout l = tiled_layout(K, N, NT, tv);
+  const int ntiles = (int)((N + tv - 1) / tv);
   const TiledBuffers b = carve(workspace, workspace_bytes, l, A, ntiles, NT);
   double* acc_re = b.acc_re;
   double* acc_im = b.acc_im;
@@ -553,7 +579,8 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
 #define GERAF_TILED_CASE(T)                                                                    \
   case T:                                                                                      \
     launch_chunk<T>(real, imag, tx_pos, rx_pos, voxel, ti, xi_pow, a0, na, (int)K, (int)N,     \
-                    ntiles, pc, kc, coef, tau_c, acc_re, acc_im, l.groups, acc_stride, st);    \
+                    ntiles, tv, pc, kc, coef, tau_c, acc_re, acc_im, l.groups, acc_stride,     \
+                    st);                                                                       \
     break;
     switch (NT) {
       GERAF_TILED_CASE(4)
@@ -582,13 +609,14 @@ extern "C" int geraf_mf_backward_tiled_f32(const float* grad_out_real, const flo
                                            float* grad_real, float* grad_imag,
                                            const float* voxel, const float* tile_info,
                                            int64_t A, int64_t K, int64_t N, int64_t norm_count,
-                                           int32_t nterms, void* workspace,
+                                           int32_t nterms, int32_t tile_voxels, void* workspace,
                                            int64_t workspace_bytes,
                                            const geraf_fmcw_params* params,
                                            geraf_stream_t stream) {
   GERAF_TRY_BEGIN
   check_params(params);
   if (A < 1 || K < 1 || N < 1) throw ArgError("need A >= 1, K >= 1, N >= 1");
+  if (!tile_size_ok(tile_voxels)) throw ArgError("tile_voxels must be 128 or 512");
   if (norm_count < 1) throw ArgError("norm_count must be >= 1");
   if (A * K >= (INT64_C(1) << 31) || N >= (INT64_C(1) << 31) - kTileVox)
     throw ArgError("A*K and N must be < 2^31");
@@ -600,8 +628,9 @@ extern "C" int geraf_mf_backward_tiled_f32(const float* grad_out_real, const flo
   require_ptrs({grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag, voxel,
                 tile_info, workspace},
                true);
-  const TiledLayout l = tiled_layout(K, N, NT);
-  const int ntiles = (int)((N + kTileVox - 1) / kTileVox);
+  const int tv = tile_voxels;
+  const TiledLayout l = tiled_layout(K, N, NT, tv);
+  const int ntiles = (int)((N + tv - 1) / tv);
   const TiledBuffers b = carve(workspace, workspace_bytes, l, A, ntiles, NT);
   const hipStream_t st = (hipStream_t)stream;
   const PhaseConsts pc = make_phase_consts(*params);
@@ -613,7 +642,7 @@ extern "C" int geraf_mf_backward_tiled_f32(const float* grad_out_real, const flo
 #define GERAF_TILED_ADJ(T)                                                                   \
   case T:                                                                                    \
     launch_adj_chunk_k<T>(kpl, grad_out_real, grad_out_imag, tx_pos, rx_pos, voxel, ti, a0,  \
-                          na, (int)K, (int)N, ntiles, pc, kc, inv_norm, b.coef, b.tau_c,     \
+                          na, (int)K, (int)N, ntiles, tv, pc, kc, inv_norm, b.coef, b.tau_c, \
                           grad_real, grad_imag, st);                                         \
     break;
     switch (NT) {

@@ -3,10 +3,10 @@ reference; the operator is the reference's matched filter,
 MatchedFilter_cuda_kernel.cu:65-115, evaluated to a stated truncation bound).
 
 ``matched_filter_tiled`` takes the arguments of ``matched_filter_cuda`` plus a
-tolerance.  It orders the voxels along a Morton curve so that every 512
-consecutive voxels form a compact tile (exact 8x8x8 bricks when the voxels
-form a regular grid), measures each tile's radius r_t, and gives each tile
-the smallest instantiated number of expansion terms N_T with
+tolerance.  It orders the voxels along a Morton curve so that every 512 (or
+128) consecutive voxels form a compact tile (exact 8x8x8 bricks when the
+voxels form a regular grid), measures each tile's radius r_t, and gives each
+tile the smallest instantiated number of expansion terms N_T with
 
     Z_t = pi (K-1) (slope/fs) 2 r_t / c,   Z_t^N_T / N_T! e^Z_t <= tol,
 
@@ -17,6 +17,16 @@ the range resolution) go through the direct kernel, as does everything when
 K < 2.  The backward is the adjoint by the same expansion (same bound, plus
 the direct adjoint for the direct voxels) when the forward ran tiled and
 K <= 2048, else the direct adjoint.
+
+Voxels of 512-voxel tiles that would need more than 32 terms can be tiled
+again in 128-voxel tiles (smaller radius, fewer terms).  A 128-voxel tile
+carries the same per-tile work as a 512-voxel one (per antenna: the forward
+moments and adjoint evaluation, K (18 + 4 N_T), and the adjoint moments'
+cross-lane sums and centre delay, ~1024 N_T + 1920 lane operations), against
+(110 + 5 N_T) per tiled voxel and ~10 K + 140 per direct voxel for forward
+plus adjoint.  The leftovers are re-tiled only when that count, with a 1.5x
+margin for the extra launches, beats the direct kernels: at K = 512 by ~6x,
+at K = 64 not at all.
 """
 from __future__ import annotations
 
@@ -32,6 +42,7 @@ from .matched_filter_cuda import matched_filter_cuda
 from .. import _native
 
 TILE = 512
+TILE_SIZES = (512, 128)
 MAX_TERMS = 32
 WORKSPACE_CAP = 2 << 30
 
@@ -116,8 +127,9 @@ class _Bucket:
     """Tiles sharing one term count: slot i of the bucket holds voxel idx[i]
     (padding slots repeat a voxel and are masked out)."""
 
-    def __init__(self, terms, idx, valid, voxel_nx3, info):
+    def __init__(self, terms, idx, valid, voxel_nx3, info, tile=TILE):
         self.terms = terms
+        self.tile = tile
         self.idx = idx
         self.valid = valid
         self.vt = voxel_nx3[idx].t().contiguous()
@@ -127,50 +139,90 @@ class _Bucket:
 
 class TilePlan:
     """Morton order, tiles, and the tiles grouped by the term count each needs
-    (the smallest instantiated N_T with Z_t^N_T / N_T! e^Z_t <= tol); tiles
-    that would need more than 32 terms are left to the direct kernel."""
+    (the smallest instantiated N_T with Z_t^N_T / N_T! e^Z_t <= tol).
+    ``tile`` None: 512-voxel tiles, their leftovers (tiles that would need
+    more than 32 terms) in 128-voxel tiles, the rest to the direct kernel;
+    512 or 128: that size only."""
 
-    def __init__(self, voxel_nx3: torch.Tensor, K: int, prm, tol: float):
+    def __init__(self, voxel_nx3: torch.Tensor, K: int, prm, tol: float, tile=None):
+        if tile is not None and tile not in TILE_SIZES:
+            raise ValueError(f"tile must be one of {TILE_SIZES}")
         dev = voxel_nx3.device
         self.N = n = voxel_nx3.shape[0]
-        perm = morton_order(voxel_nx3)
-        nt = (n + TILE - 1) // TILE
-        pad = nt * TILE - n
-        slots = torch.cat([perm, perm[-1:].expand(pad)]) if pad else perm
-        valid = torch.ones(nt * TILE, dtype=torch.bool, device=dev)
-        if pad:
-            valid[n:] = False
-        info = tile_bounds(voxel_nx3[slots], TILE)
         zfac = math.pi * (K - 1) * (prm.slope / prm.fs) * 2.0 / prm.c
-        z = info[:, 3].double() * zfac
-        self.Z = float(z.max()) if nt else 0.0
-        self.terms = expansion_terms(self.Z, tol)
         self.buckets = []
-        direct = torch.ones(nt, dtype=torch.bool, device=dev)
-        lower = -1.0
-        if K >= 2:
-            for terms in TERM_SETS:
-                lim = _z_limit(terms, tol)
-                sel = (z > lower) & (z <= lim)
-                lower = lim
-                tiles = torch.nonzero(sel).reshape(-1)
-                if tiles.numel() == 0:
-                    continue
-                direct[tiles] = False
-                pos = (tiles[:, None] * TILE + torch.arange(TILE, device=dev)[None, :]).reshape(-1)
-                self.buckets.append(_Bucket(terms, slots[pos], valid[pos], voxel_nx3,
-                                            info[tiles]))
-        dpos = (torch.nonzero(direct).reshape(-1)[:, None] * TILE +
-                torch.arange(TILE, device=dev)[None, :]).reshape(-1)
-        self.direct_idx = slots[dpos][valid[dpos]]
+        self.Z = 0.0
+        left = morton_order(voxel_nx3)  # voxels not yet in a tiled bucket, in Morton order
+        for tv in ((tile,) if tile is not None else TILE_SIZES):
+            if left.numel() == 0 or K < 2:
+                break
+            if tile is None and tv != TILE and not self._retile_pays(voxel_nx3, left, tv, zfac,
+                                                                    K, tol):
+                break
+            left = self._tile(voxel_nx3, left, tv, zfac, tol)
+        self.direct_idx = left
+        self.tile = tile if tile is not None else TILE
+        self.terms = expansion_terms(self.Z, tol)
         self.usable = len(self.buckets) > 0
         self.tiled_fraction = 1.0 - self.direct_idx.numel() / max(n, 1)
 
     @staticmethod
+    def _retile_pays(voxel_nx3, order, tv, zfac, K, tol) -> bool:
+        """Work count per antenna of tiling `order` in tv-voxel tiles against
+        leaving it to the direct kernels (forward + adjoint)."""
+        m = order.numel()
+        nt = (m + tv - 1) // tv
+        pad = nt * tv - m
+        slots = torch.cat([order, order[-1:].expand(pad)]) if pad else order
+        z = tile_bounds(voxel_nx3[slots], tv)[:, 3].double() * zfac
+        tiled = 0.0
+        direct_vox = float(m)
+        lower = -1.0
+        for terms in TERM_SETS:
+            lim = _z_limit(terms, tol)
+            cnt = int(((z > lower) & (z <= lim)).sum())
+            lower = lim
+            tiled += cnt * (K * (18 + 4 * terms) + 1024 * terms + 1920 + tv * (110 + 5 * terms))
+            direct_vox -= cnt * tv
+        direct_cost = (10 * K + 140)
+        return 1.5 * (tiled + max(direct_vox, 0.0) * direct_cost) < m * direct_cost
+
+    def _tile(self, voxel_nx3, order, tv, zfac, tol):
+        """Tiles of tv consecutive voxels of `order` into buckets; returns the
+        voxels of tiles that would need more than MAX_TERMS terms."""
+        dev = voxel_nx3.device
+        m = order.numel()
+        nt = (m + tv - 1) // tv
+        pad = nt * tv - m
+        slots = torch.cat([order, order[-1:].expand(pad)]) if pad else order
+        valid = torch.ones(nt * tv, dtype=torch.bool, device=dev)
+        if pad:
+            valid[m:] = False
+        info = tile_bounds(voxel_nx3[slots], tv)
+        z = info[:, 3].double() * zfac
+        direct = torch.ones(nt, dtype=torch.bool, device=dev)
+        lower = -1.0
+        for terms in TERM_SETS:
+            lim = _z_limit(terms, tol)
+            sel = (z > lower) & (z <= lim)
+            lower = lim
+            tiles = torch.nonzero(sel).reshape(-1)
+            if tiles.numel() == 0:
+                continue
+            direct[tiles] = False
+            self.Z = max(self.Z, float(z[tiles].max()))
+            pos = (tiles[:, None] * tv + torch.arange(tv, device=dev)[None, :]).reshape(-1)
+            self.buckets.append(_Bucket(terms, slots[pos], valid[pos], voxel_nx3, info[tiles],
+                                        tv))
+        dpos = (torch.nonzero(direct).reshape(-1)[:, None] * tv +
+                torch.arange(tv, device=dev)[None, :]).reshape(-1)
+        return slots[dpos][valid[dpos]]
+
+    @staticmethod
     def workspace(A: int, K: int, b: "_Bucket", device) -> Tuple[torch.Tensor, int]:
         lib = _native.load()
-        need = int(lib.geraf_mf_tiled_workspace_bytes(A, K, b.N, b.terms))
-        per_one = int(lib.geraf_mf_tiled_workspace_bytes(1, K, b.N, b.terms))
+        need = int(lib.geraf_mf_tiled_workspace_bytes(A, K, b.N, b.terms, b.tile))
+        per_one = int(lib.geraf_mf_tiled_workspace_bytes(1, K, b.N, b.terms, b.tile))
         size = max(per_one, min(need, WORKSPACE_CAP))
         return torch.empty(size, dtype=torch.uint8, device=device), size
 
@@ -207,7 +259,8 @@ def tiled_forward(real, image, tpos, rpos, voxel_nx3, numADCSample: int, adcSamp
         s_r = torch.empty(b.N, dtype=torch.float32, device=real.device)
         s_i = torch.empty_like(s_r)
         _native.call("geraf_mf_forward_tiled_f32", P(real), P(image), P(tpos), P(rpos), P(s_r),
-                     P(s_i), P(b.vt), P(b.info), A, K, b.N, b.terms, P(ws), size, prm, st)
+                     P(s_i), P(b.vt), P(b.info), A, K, b.N, b.terms, b.tile, P(ws), size, prm,
+                     st)
         out_r[b.idx[b.valid]] = s_r[b.valid]
         out_i[b.idx[b.valid]] = s_i[b.valid]
     if plan.direct_idx.numel():
@@ -237,8 +290,8 @@ def tiled_adjoint(g_r, g_i, tpos, rpos, plan: "TilePlan", A: int, K: int, norm_c
         p_r = torch.empty((A, K), dtype=torch.float32, device=gr.device)
         p_i = torch.empty_like(p_r)
         _native.call("geraf_mf_backward_tiled_f32", P(br), P(bi), P(tpos), P(rpos), P(p_r),
-                     P(p_i), P(b.vt), P(b.info), A, K, b.N, norm_count, b.terms, P(ws), size,
-                     prm, st)
+                     P(p_i), P(b.vt), P(b.info), A, K, b.N, norm_count, b.terms, b.tile, P(ws),
+                     size, prm, st)
         out_r += p_r
         out_i += p_i
     if plan.direct_idx.numel():

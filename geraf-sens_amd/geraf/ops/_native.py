@@ -70,10 +70,10 @@ _SIGNATURES = {
     "geraf_softplus_bf16": [_P] * 2 + [_I64, ctypes.c_float, ctypes.c_float, _P],
     "geraf_softplus_grad_bf16": [_P] * 3 + [_I64, ctypes.c_float, ctypes.c_float, _P],
     "geraf_softplus_grad2_bf16": [_P] * 5 + [_I64, ctypes.c_float, ctypes.c_float, _P],
-    "geraf_mf_forward_tiled_f32": [_P] * 8 + [_I64] * 3 + [ctypes.c_int32, _P, _I64, _PARAMS,
-                                                            _P],
-    "geraf_mf_backward_tiled_f32": [_P] * 8 + [_I64] * 4 + [ctypes.c_int32, _P, _I64, _PARAMS,
-                                                             _P],
+    "geraf_mf_forward_tiled_f32": [_P] * 8 + [_I64] * 3 + [ctypes.c_int32] * 2 +
+                                  [_P, _I64, _PARAMS, _P],
+    "geraf_mf_backward_tiled_f32": [_P] * 8 + [_I64] * 4 + [ctypes.c_int32] * 2 +
+                                   [_P, _I64, _PARAMS, _P],
 }
 
 _lock = threading.Lock()
@@ -104,7 +104,8 @@ def load() -> ctypes.CDLL:
         lib.geraf_rf_version.restype = ctypes.c_char_p
         lib.geraf_rf_version.argtypes = []
         lib.geraf_mf_tiled_workspace_bytes.restype = ctypes.c_int64
-        lib.geraf_mf_tiled_workspace_bytes.argtypes = [_I64, _I64, _I64, ctypes.c_int32]
+        lib.geraf_mf_tiled_workspace_bytes.argtypes = [_I64, _I64, _I64, ctypes.c_int32,
+                                                       ctypes.c_int32]
         _lib = lib
     return _lib
 

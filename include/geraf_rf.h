@@ -244,20 +244,22 @@ int geraf_lensless_backward_f32(const float* grad_out_real, const float* grad_ou
 /* Fast matched filter: the operator of geraf_mf_forward_f32 (and of
  * _C.MatchedFilter_forward, vision.cpp:70-74; MatchedFilter_cuda_kernel.cu:65-115)
  * evaluated by per-tile expansions in the delay offset.  It has no counterpart
- * in the reference.  voxel [3, N] must be ordered so that every tile of 512
- * consecutive voxels is compact in space; tile_info [ceil(N/512), 4] holds
- * each tile's centre (x, y, z) and radius r (every voxel of the tile within r
- * of the centre).  With Z = pi*(K-1)*(slope/fs)*2*r_max/c, nterms = N_T
- * bounds the truncation by sum_k |s[a,k]| * Z^N_T / N_T! * e^Z per antenna
- * (rounded up to one of 4, 6, 8, 10, 12, 14, 16, 20, 24, 32; at most 32).
- * workspace: device memory of workspace_bytes >= geraf_mf_tiled_workspace_bytes
- * for A=1 (the antennas are then processed in chunks that fit); the caller
- * owns it, nothing is allocated. */
-int64_t geraf_mf_tiled_workspace_bytes(int64_t A, int64_t K, int64_t N, int32_t nterms);
+ * in the reference.  voxel [3, N] must be ordered so that every tile of
+ * tile_voxels (128 or 512) consecutive voxels is compact in space; tile_info
+ * [ceil(N/tile_voxels), 4] holds each tile's centre (x, y, z) and radius r
+ * (every voxel of the tile within r of the centre).  With
+ * Z = pi*(K-1)*(slope/fs)*2*r_max/c, nterms = N_T bounds the truncation by
+ * sum_k |s[a,k]| * Z^N_T / N_T! * e^Z per antenna (rounded up to one of 4, 6,
+ * 8, 10, 12, 14, 16, 20, 24, 32; at most 32).  workspace: device memory of
+ * workspace_bytes >= geraf_mf_tiled_workspace_bytes for A=1 (the antennas are
+ * then processed in chunks that fit); the caller owns it, nothing is
+ * allocated.  geraf_mf_tiled_workspace_bytes returns -1 for invalid sizes. */
+int64_t geraf_mf_tiled_workspace_bytes(int64_t A, int64_t K, int64_t N, int32_t nterms,
+                                       int32_t tile_voxels);
 int geraf_mf_forward_tiled_f32(const float* real, const float* imag, const float* tx_pos,
                                const float* rx_pos, float* out_real, float* out_imag,
                                const float* voxel, const float* tile_info, int64_t A, int64_t K,
-                               int64_t N, int32_t nterms, void* workspace,
+                               int64_t N, int32_t nterms, int32_t tile_voxels, void* workspace,
                                int64_t workspace_bytes, const geraf_fmcw_params* params,
                                geraf_stream_t stream);
 /* Adjoint of geraf_mf_forward_tiled_f32 by the same expansion (the operator
@@ -269,8 +271,9 @@ int geraf_mf_backward_tiled_f32(const float* grad_out_real, const float* grad_ou
                                 const float* tx_pos, const float* rx_pos, float* grad_real,
                                 float* grad_imag, const float* voxel, const float* tile_info,
                                 int64_t A, int64_t K, int64_t N, int64_t norm_count,
-                                int32_t nterms, void* workspace, int64_t workspace_bytes,
-                                const geraf_fmcw_params* params, geraf_stream_t stream);
+                                int32_t nterms, int32_t tile_voxels, void* workspace,
+                                int64_t workspace_bytes, const geraf_fmcw_params* params,
+                                geraf_stream_t stream);
 
 /* ---------------------------------------------------------------- MLP activation (BF16) */
 

@@ -396,14 +396,19 @@ def test_tiled_small_and_fallback_cases(cuda):
     sc = ms.config1(0)
     t = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda)  # noqa: E731
     re, im, tx, rx = t(sc.echo.real), t(sc.echo.imag), t(sc.tx), t(sc.rx)
-    for n in (1, 511, 513):
-        v = t(sc.voxels[1000:1000 + n])
-        o_r, o_i, info = _tiled(re, im, tx, rx, v, sc.K, sc.fs, sc.As_sci)
-        ref = orc.mf_forward(sc.echo.real, sc.echo.imag, sc.tx, sc.rx, sc.voxels[1000:1000 + n],
-                             **sc.params())
-        got = torch.complex(o_r.double(), o_i.double()).cpu().numpy()
-        rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
-        assert rel <= 1e-5, (n, rel, info)
+    from geraf.ops.RFMatchedFilter import _C as mfC
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import TilePlan, tiled_forward
+    for tile in (512, 128):
+        for n in (1, 127, 129, 511, 513):
+            v = t(sc.voxels[1000:1000 + n])
+            plan = TilePlan(v, sc.K, mfC._params(sc.fs, sc.As_sci), 1e-6, tile=tile)
+            o_r, o_i, info = tiled_forward(re, im, tx, rx, v, sc.K, sc.fs, sc.As_sci, 1e-6,
+                                           plan=plan)
+            ref = orc.mf_forward(sc.echo.real, sc.echo.imag, sc.tx, sc.rx,
+                                 sc.voxels[1000:1000 + n], **sc.params())
+            got = torch.complex(o_r.double(), o_i.double()).cpu().numpy()
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            assert rel <= 1e-5, (tile, n, rel, info)
     # K = 1 runs the direct kernel
     o_r, o_i, info = _tiled(re[:, :1].contiguous(), im[:, :1].contiguous(), tx, rx,
                             t(sc.voxels[:600]), 1, sc.fs, sc.As_sci)
@@ -432,8 +437,9 @@ def test_tiled_op_gradient_is_the_adjoint(cuda):
         assert float((a - b).norm() / b.norm()) < 1e-5
 
 
+@pytest.mark.parametrize("tile", [512, 128])
 @pytest.mark.parametrize("which", ["cfg1", "cfg2_block"])
-def test_tiled_adjoint_matches_direct_and_is_the_transpose(cuda, which):
+def test_tiled_adjoint_matches_direct_and_is_the_transpose(cuda, which, tile):
     from geraf.ops import _native
     from geraf.ops.RFMatchedFilter import _C as mfC
     from geraf.ops.RFMatchedFilter.tiled_matched_filter import (TilePlan, tiled_adjoint,
@@ -451,8 +457,8 @@ def test_tiled_adjoint_matches_direct_and_is_the_transpose(cuda, which):
     g_r = t(rng.standard_normal(N).astype(np.float32))
     g_i = t(rng.standard_normal(N).astype(np.float32))
     prm = mfC._params(sc.fs, sc.As_sci)
-    plan = TilePlan(v, K, prm, 1e-6)
-    assert plan.usable
+    plan = TilePlan(v, K, prm, 1e-6, tile=tile)
+    assert plan.usable and plan.tile == tile
     a_r, a_i = tiled_adjoint(g_r, g_i, tx, rx, plan, A, K, A, sc.fs, sc.As_sci, v)
     d_r, d_i = torch.empty((A, K), device=cuda), torch.empty((A, K), device=cuda)
     vt = v.t().contiguous()
@@ -465,13 +471,23 @@ def test_tiled_adjoint_matches_direct_and_is_the_transpose(cuda, which):
     f_r, f_i, _ = tiled_forward(re, im, tx, rx, v, K, sc.fs, sc.As_sci, 1e-6, plan=plan)
     lhs = float((f_r.double() * g_r.double() + f_i.double() * g_i.double()).sum())
     rhs = float((re.double() * a_r.double() + im.double() * a_i.double()).sum())
-    print(f"{which}: tiled adjoint vs direct rel-L2 {rel:.3e} ({plan.terms} terms); "
+    print(f"{which} tile {tile}: tiled adjoint vs direct rel-L2 {rel:.3e} ({plan.terms} terms); "
           f"<MF s,g>={lhs:.6e} <s,MF^T g>={rhs:.6e}", flush=True)
     assert rel <= 1e-5
     assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + abs(rhs))
+    # the forward of this plan against the direct kernel
+    o_r, o_i = torch.empty(N, device=cuda), torch.empty(N, device=cuda)
+    _native.call("geraf_mf_forward_f32", *map(_native.ptr, (re, im, tx, rx, o_r, o_i, vt)), A, K,
+                 N, _native.make_params(sc.f0, sc.As_sci, sc.fs, sc.c, sc.bias),
+                 _native.stream_of(re))
+    fwd = float((torch.complex(f_r.double(), f_i.double()) -
+                 torch.complex(o_r.double(), o_i.double())).norm() /
+                torch.complex(o_r.double(), o_i.double()).norm())
+    assert fwd <= 1e-5, fwd
 
 
-def test_tiled_mixed_plan_forward_and_adjoint(cuda):
+@pytest.mark.parametrize("tile", [512, 128])
+def test_tiled_mixed_plan_forward_and_adjoint(cuda, tile):
     """A compact block plus scattered far points: the block's tiles run the
     expansion, the scattered tiles the direct kernel, in both directions."""
     from geraf.ops import _native
@@ -485,7 +501,7 @@ def test_tiled_mixed_plan_forward_and_adjoint(cuda):
     vox = np.concatenate([sc.voxels[:8192], far])
     A, K, N = sc.A, sc.K, vox.shape[0]
     re, im, tx, rx, v = t(sc.echo.real), t(sc.echo.imag), t(sc.tx), t(sc.rx), t(vox)
-    plan = TilePlan(v, K, mfC._params(sc.fs, sc.As_sci), 1e-6)
+    plan = TilePlan(v, K, mfC._params(sc.fs, sc.As_sci), 1e-6, tile=tile)
     assert plan.usable and 0.5 < plan.tiled_fraction < 1.0, plan.tiled_fraction
     o_r, o_i, info = tiled_forward(re, im, tx, rx, v, K, sc.fs, sc.As_sci, 1e-6, plan=plan)
     ref = orc.mf_forward(sc.echo.real, sc.echo.imag, sc.tx, sc.rx, vox, **sc.params())

@@ -89,3 +89,40 @@ def test_morton_tiles_of_a_grid_are_bricks():
     t = torch.arange(3000) // 512
     dist = (pts[p].double() - inf[t, :3].double()).norm(dim=1)
     assert torch.all(dist <= inf[t, 3].double())
+
+
+def test_plan_tile_sizes_and_coverage():
+    """A dense grid at K = 512 is all 512-voxel tiles.  Scattered ray samples
+    leave voxels of large 512-voxel tiles: at K = 512 they are re-tiled in
+    128-voxel tiles (which take part of them back from the direct kernel), at
+    K = 64 they stay direct (the re-tiling would not pay).  Every voxel
+    appears exactly once, in a bucket slot or in the direct list, for each
+    tile setting."""
+    from geraf.ops.RFMatchedFilter import _C as mfC
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import TilePlan
+    from rf_workloads import tracer_scenes as ts
+
+    lin = torch.linspace(-0.05, 0.05, 32)
+    gx, gy, gz = torch.meshgrid(lin, lin, lin + 0.3, indexing="ij")
+    grid = torch.stack([gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)], 1)
+    plan = TilePlan(grid, 512, mfC._params(10e6, 701500.0), 1e-6)
+    assert {bk.tile for bk in plan.buckets} == {512} and plan.tiled_fraction == 1.0
+
+    b = ts.rays(1024, 64, 64, seed=0, uneven=(16, 64))
+    pts = b.o[:, None, :] + b.z[..., None] * b.d[:, None, :]
+    valid = torch.arange(64)[None, :] < b.n_valid[:, None].long()
+    tgt = pts[valid].contiguous()
+    r = b.radar
+    prm = mfC._params(r["adcSampleRate"], r["As_sci"])
+    plans = {t: TilePlan(tgt, 64, prm, 1e-6, tile=t) for t in (None, 512, 128)}
+    for p in plans.values():
+        seen = torch.cat([bk.idx[bk.valid] for bk in p.buckets] + [p.direct_idx])
+        assert torch.equal(torch.sort(seen).values, torch.arange(tgt.shape[0]))
+    assert {bk.tile for bk in plans[None].buckets} == {512}
+    assert plans[128].tiled_fraction > plans[512].tiled_fraction
+    prm512 = mfC._params(r["adcSampleRate"] * 8, r["As_sci"])  # the same band at K = 512
+    p512 = TilePlan(tgt, 512, prm512, 1e-6)
+    seen = torch.cat([bk.idx[bk.valid] for bk in p512.buckets] + [p512.direct_idx])
+    assert torch.equal(torch.sort(seen).values, torch.arange(tgt.shape[0]))
+    assert {bk.tile for bk in p512.buckets} == {512, 128}
+    assert p512.tiled_fraction > TilePlan(tgt, 512, prm512, 1e-6, tile=512).tiled_fraction
