@@ -283,7 +283,8 @@ def bench_large_target(dev):
     rng = np.random.default_rng(64)
     ijk = rng.integers(96, 160, (24, 3))
     idx = torch.tensor((ijk[:, 0] * 256 + ijk[:, 1]) * 256 + ijk[:, 2], device=dev)
-    d64 = lambda x: torch.tensor(np.ascontiguousarray(x), device=dev, dtype=torch.float64)  # noqa: E731
+    def d64(x):
+        return torch.tensor(np.ascontiguousarray(x), device=dev, dtype=torch.float64)
     ops = [d64(sc.echo.real), d64(sc.echo.imag), d64(sc.tx), d64(sc.rx)]
     sub = full[idx].double().t().contiguous()
     s_r = torch.empty(idx.numel(), device=dev, dtype=torch.float64)

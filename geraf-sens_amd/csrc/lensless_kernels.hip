@@ -238,6 +238,10 @@ __device__ __forceinline__ void group_sum(float* v, int groups) {
 
 // ------------------------------------------------------------------ forward
 
+// Per-wave queue of emitting samples: amplitude, carrier phase f0 tau reduced
+// to cycles, per-frequency step dfs tau in FP64 cycles, and a phasor: the
+// step exp(j 2 pi dfs tau) for the rotation synthesis, the 256-frequency
+// rotation exp(j 2 pi 256 dfs tau) for the matrix-core synthesis.
 struct FwdQueue {
   float amp[kRaysPerBlock][kQueue];
   float fr0[kRaysPerBlock][kQueue];
@@ -273,9 +277,8 @@ __device__ __forceinline__ void synth(const FwdQueue& q, int wave, int n, int gr
 // P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi); Re = P1 - P2 and
 // Im = P3 - P1 - P2 (the error stays O(eps sum |X||Y|), as with four).
 // Each lane forms one Y and one X phasor per step from FP64-reduced phases
-// (hardware sine and cosine);
-// the rows of further tiles follow by one complex rotation each (the
-// sample's exp(j 2 pi 256 dfs tau), queued with it).
+// (hardware sine and cosine); the rows of further tiles follow by one complex
+// rotation each (the sample's exp(j 2 pi 256 dfs tau), queued with it).
 // Operand lanes of that instruction: A[i][k] and B[k][j] come from lane
 // i + 16 k (resp. j + 16 k); D[4 q + v][j] is element v of lane j + 16 q.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -294,7 +297,8 @@ __device__ __forceinline__ void synth_mfma(const FwdQueue& q, int wave, int lane
       c1 = q.cyc1[wave][i];
     }
     const float2 yv = phasor_cycles_hw((float)centered_frac((double)j * c1));
-    const float2 xv = phasor_cycles_hw((float)centered_frac(fma((double)(16 * j), c1, (double)f0)));
+    const float2 xv =
+        phasor_cycles_hw((float)centered_frac(fma((double)(16 * j), c1, (double)f0)));
     float2 x = make_float2(am * xv.x, am * xv.y);
     float2 rot = make_float2(1.0f, 0.0f);
     if constexpr (MT > 1) {

@@ -384,9 +384,10 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_kernel(
 // lanes: A[i][k] and B[k][j] from lane i + 16 k (j + 16 k); D[4 q + v][j] is
 // element v of lane j + 16 q.  Each lane forms one Y and one X phasor per
 // step (phase j cyc1 and 16 j cyc1 reduced exactly in FP32 from the split
-// step, hardware sine and cosine); further tiles follow by one rotation.  After every tile the FP32 matrix sums (64 voxels) are added
-// into FP64 registers; the eight waves' FP64 sums are reduced through LDS in
-// a fixed order.
+// step, hardware sine and cosine); further tiles follow by one rotation.
+// After every tile the FP32 matrix sums (64 voxels) are added into FP64
+// registers; the eight waves' FP64 sums are reduced through LDS in a fixed
+// order.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct AdjMfmaStage {

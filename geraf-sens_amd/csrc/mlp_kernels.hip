@@ -95,7 +95,8 @@ __global__ __launch_bounds__(256) void softplus_kernel(const uint16_t* __restric
     const Vec8 a = load8(x, i, n);
     Vec8 o;
 #pragma unroll
-    for (int j = 0; j < kVec; ++j) o.v[j] = f32_bf16(softplus_terms(bf16_f32(a.v[j]), beta, threshold).y);
+    for (int j = 0; j < kVec; ++j)
+      o.v[j] = f32_bf16(softplus_terms(bf16_f32(a.v[j]), beta, threshold).y);
     store8(y, i, n, o);
   }
 }
