@@ -21,8 +21,12 @@
 // over the tile (DPP row shifts and broadcasts) times the carry of the earlier
 // tiles.  Divisions, square roots and the sigmoid's exponential use the
 // hardware approximations (about 1 ulp); the round-trip distance and the
-// carrier phase are formed in FP64.  The antenna is monostatic, so the return
-// direction is exactly the negated incident direction.
+// carrier phase are formed in FP64.  The two cdfs, their difference and their
+// slopes come from exp(-|x|) (logistic_fast): the FP32 difference of two cdfs
+// near 1 and the slope p (1 - p), as the reference forms them, lose all
+// precision in front of the surface, and over 2048 samples that costs the sdf
+// gradient 1e-3 relative.  The antenna is monostatic, so the return direction
+// is exactly the negated incident direction.
 //
 // Forward signal: emitting samples (valid, not culled, amp != 0) are appended
 // to a per-wave LDS queue as (amplitude, reduced carrier phase, per-frequency
@@ -119,7 +123,7 @@ __device__ __forceinline__ Ray load_ray(const LenslessArgs& a, int r) {
 struct Sample {
   float sdf, dist, refl, off, zz;
   V3 g;
-  float c, prev, next, pc, nc, alpha, t;
+  float c, prev, next, pc, nc, sp, sn, alpha, t;  // sp, sn: slopes of pc, nc
   bool qmask;
 };
 
@@ -156,9 +160,17 @@ __device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ra
   const float half = ic * x.dist * 0.5f;
   x.next = x.sdf + half;
   x.prev = x.sdf - half;
-  x.pc = sigmoid_fast(x.prev * y.inv_s);
-  x.nc = sigmoid_fast(x.next * y.inv_s);
-  const float q = (x.pc - x.nc + 1.0e-5f) * rcp_fast(x.pc + 1.0e-5f);
+  const Logistic lp = logistic_fast(x.prev * y.inv_s);
+  const Logistic ln = logistic_fast(x.next * y.inv_s);
+  x.pc = lp.p;
+  x.nc = ln.p;
+  x.sp = lp.d;
+  x.sn = ln.d;
+  // pc - nc from the accurate pair: the complements once pc >= 1/2 (far in
+  // front of the surface both cdfs round towards 1 and their FP32 difference
+  // would be noise of order 1e-7).
+  const float dcdf = lp.p >= 0.5f ? ln.c - lp.c : lp.p - ln.p;
+  const float q = (dcdf + 1.0e-5f) * rcp_fast(x.pc + 1.0e-5f);
   x.qmask = (q >= 0.0f) && (q <= 1.0f);
   x.alpha = fminf(fmaxf(q, 0.0f), 1.0f);
   x.t = 1.0f - x.alpha + 1.0e-7f;
@@ -693,11 +705,9 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
     const float rden = rcp_fast(x.pc + 1.0e-5f);
     const float gpc = gq * x.nc * (rden * rden);
     const float gnc = -gq * rden;
-    const float sp = x.pc * (1.0f - x.pc);
-    const float sn = x.nc * (1.0f - x.nc);
-    const float gprev = gpc * y.inv_s * sp;
-    const float gnext = gnc * y.inv_s * sn;
-    gis += gpc * x.prev * sp + gnc * x.next * sn;
+    const float gprev = gpc * y.inv_s * x.sp;
+    const float gnext = gnc * y.inv_s * x.sn;
+    gis += gpc * x.prev * x.sp + gnc * x.next * x.sn;
     const float gsdf = gprev + gnext;
     const float gic = (gnext - gprev) * x.dist * 0.5f;
     const float gc = gic * ((x.c < 1.0f ? 0.5f * (1.0f - a.anneal) : 0.0f) +

@@ -291,9 +291,19 @@ __device__ __forceinline__ float wave_affine_suffix(float m, float b, float tail
 // is a plain division, square root or exponential.
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
-// 1 / (1 + exp(-x)); saturates cleanly to 0 and 1.
-__device__ __forceinline__ float sigmoid_fast(float x) {
-  return rcp_fast(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * x));
+// The logistic function p = 1 / (1 + exp(-x)), its complement 1 - p and its
+// slope p (1 - p), each to a few ulp of its own size: with e = exp(-|x|) <= 1
+// and r = 1 / (1 + e), the pair {p, 1 - p} is {r, e r} (ordered by the sign
+// of x) and the slope is e r^2.  Forming 1 - p or p (1 - p) from a rounded p
+// instead loses all relative precision once p rounds to 1 (x > ~17).
+struct Logistic {
+  float p, c, d;  // p, 1 - p, dp/dx
+};
+__device__ __forceinline__ Logistic logistic_fast(float x) {
+  const float e = __builtin_amdgcn_exp2f(-1.44269504088896341f * fabsf(x));
+  const float r = rcp_fast(1.0f + e);
+  const float er = e * r;
+  return x >= 0.0f ? Logistic{r, er, er * r} : Logistic{er, r, er * r};
 }
 
 // Sum over the 64 lanes of a wave (butterfly, every lane gets the total).

@@ -4,7 +4,9 @@ TEST INFRASTRUCTURE ONLY (the checker of the fused lensless kernels).
 
 The composition follows the reference's torch glue line by line, in float64:
   iter_cos            rf_rendering.py:145-155 (_compute_iter_cos)
-  alpha               rf_rendering.py:157-169 (_compute_alpha_from_sdf), EPS = 1e-5
+  alpha               rf_rendering.py:157-169 (_compute_alpha_from_sdf), EPS = 1e-5;
+                      the cdf difference is taken from the complements in front
+                      of the surface (same value, no cancellation)
   transmittance       rf_rendering.py:46-52   (_compute_transmission), TRANS_EPS = 1e-7
   calibration         rf_rendering.py:750     (T - prev_cdf + new_cdf = T + offset)
   sigma               sdf_network.py:475-481  (clamp(refl * exp(light_power) * T, 1e-6) * alpha * T)
@@ -44,9 +46,18 @@ def lensless_forward(sdf, grad, dists, pts, o, d, inv_s, power, K, fs, As_sci, f
     iter_cos = -(F.relu(-true_cos * 0.5 + 0.5) * (1.0 - anneal) + F.relu(-true_cos) * anneal)
     next_sdf = sdf + iter_cos * dists * 0.5
     prev_sdf = sdf - iter_cos * dists * 0.5
-    prev_cdf = torch.sigmoid(prev_sdf * inv_s)
-    next_cdf = torch.sigmoid(next_sdf * inv_s)
-    alpha = ((prev_cdf - next_cdf + EPS) / (prev_cdf + EPS)).clip(0.0, 1.0)
+    tp = prev_sdf * inv_s
+    tn = next_sdf * inv_s
+    # prev_cdf - next_cdf, formed as (1 - next_cdf) - (1 - prev_cdf) where
+    # prev_cdf >= 1/2, and prev_cdf as 1 - (1 - prev_cdf) there: equal in exact
+    # arithmetic, but the literal difference of two cdfs within 1e-14 of 1 (and
+    # autograd's slope y (1 - y) of a cdf near 1) keeps only a few digits even
+    # in float64.
+    front = tp >= 0.0
+    cp = torch.sigmoid(-tp)
+    prev_cdf = torch.where(front, 1.0 - cp, torch.sigmoid(tp))
+    dcdf = torch.where(front, torch.sigmoid(-tn) - cp, prev_cdf - torch.sigmoid(tn))
+    alpha = ((dcdf + EPS) / (prev_cdf + EPS)).clip(0.0, 1.0)
     if n_valid is not None:
         valid = torch.arange(S)[None, :] < n_valid[:, None]
     else:

@@ -148,11 +148,13 @@ def test_backward_vs_oracle_and_adjoint_identity(cuda):
 
 
 @pytest.mark.parametrize("A,K,N", [(5, 200, 1500), (3, 300, 1100), (4, 512, 1030),
-                                   (2, 1000, 700), (2, 1030, 600)])
+                                   (2, 1000, 700), (2, 1030, 600), (1, 129, 1),
+                                   (2, 257, 513), (1, 1024, 511)])
 def test_backward_matrix_tiles(cuda, A, K, N):
     """The matrix-core adjoint (128 < K <= 1024: one, two and four 256-sample
-    tiles, partial tiles, several voxel tiles with a partial last one) and the
-    rotation adjoint beyond (K = 1030), against the FP64 oracle."""
+    tiles, partial tiles, the first K of each tile count, several voxel tiles
+    with a partial last one, a single antenna and voxel) and the rotation
+    adjoint beyond (K = 1030), against the FP64 oracle."""
     tx, rx, vox, g = _adjoint_case(cuda, A, K, N, seed=K)
     fs, As = 1.25e6 * K / 64, 701500.0
     prm = dict(f0=mf_scenes.MF_F0, slope=As, fs=fs, c=mf_scenes.SPEED_OF_LIGHT,

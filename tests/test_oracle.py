@@ -254,6 +254,54 @@ def test_lensless_transmittance_is_sequential_product():
             t *= 1.0 - a[r, s] + 1e-7
 
 
+def test_lensless_alpha_and_slope_vs_high_precision():
+    """alpha and d alpha / d sdf of the oracle against a 60-digit evaluation
+    of the reference formula, from behind the surface (sdf * inv_s = -30) to
+    far in front of it (+35), where both cdfs lie within 1e-15 of 1.  The
+    literal float64 difference of the two cdfs (and autograd's y (1 - y)
+    slope) is off by 1e-3 relative there, and fails this check."""
+    import decimal
+    from decimal import Decimal as Dec
+
+    decimal.getcontext().prec = 60
+    inv_s, dist = 64.0, 0.012
+    sdf = torch.linspace(-30.0 / inv_s, 35.0 / inv_s, 53, dtype=torch.float64)
+    S = sdf.numel()
+    o = torch.zeros(1, 3, dtype=torch.float64)
+    d = torch.tensor([[0.0, 0.0, 1.0]], dtype=torch.float64)
+    grad = torch.tensor([0.0, 0.0, -1.0], dtype=torch.float64).expand(1, S, 3).clone()
+    leaf = sdf[None].clone().requires_grad_(True)
+    pts = torch.zeros(1, S, 3, dtype=torch.float64)
+    pts[0, :, 2] = torch.linspace(1.0, 2.0, S, dtype=torch.float64)
+    dists = torch.full((1, S), dist, dtype=torch.float64)
+    _, parts = L.lensless_forward(leaf, grad, dists, pts, o, d, torch.tensor(inv_s),
+                                  torch.tensor(1.0), 1, 1.0, 0.0, 1.0, 1.0, 0.0, anneal=1.0,
+                                  return_parts=True)
+    alpha = parts["alpha"]
+    (galpha,) = torch.autograd.grad(alpha.sum(), leaf)
+    alpha = alpha.detach()
+
+    one, eps = Dec(1), Dec(L.EPS)
+
+    def sig(t):
+        return one / (one + (-t).exp())
+
+    def slope(t):
+        e = (-t).exp()
+        return e / ((one + e) * (one + e))
+
+    for s in range(S):
+        # cos = -1 and anneal = 1: iter_cos = -1, prev = sdf + dist/2, next = sdf - dist/2
+        tp = (Dec(float(sdf[s])) + Dec(dist) / 2) * Dec(inv_s)
+        tn = (Dec(float(sdf[s])) - Dec(dist) / 2) * Dec(inv_s)
+        p, n = sig(tp), sig(tn)
+        den = p + eps
+        a_ref = (p - n + eps) / den
+        g_ref = Dec(inv_s) * (slope(tp) * n / (den * den) - slope(tn) / den)
+        assert abs(float(alpha[0, s]) - float(a_ref)) <= 1e-14 * float(a_ref), s
+        assert abs(float(galpha[0, s]) - float(g_ref)) <= 1e-12 * abs(float(g_ref)), s
+
+
 @pytest.mark.parametrize("seed,uneven,offset", [(10, None, False), (11, (4, 40), True)])
 def test_lensless_oracle_gradients_vs_finite_differences(seed, uneven, offset):
     b = ts.rays(6, 40, 32, seed=seed, uneven=uneven, with_offset=offset)
