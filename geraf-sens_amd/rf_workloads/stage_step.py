@@ -10,8 +10,9 @@ One step (GeRaFStage1.loss, rf_rendering.py:721-820, in its per-ray form):
   3. the fused lensless tracer renders each ray's FMCW signal (its antenna is
      the ray origin);
   4. the matched filter backprojects all rays' signals onto the valid sample
-     points; L1 between |MF| and the target volume sampled there
-     (trilinear), plus the eikonal term (rf_rendering.py:645-650);
+     points (rays from one aperture position are one antenna, whose echo is
+     the sum of their signals); L1 between |MF| and the target volume sampled
+     there (trilinear), plus the eikonal term (rf_rendering.py:645-650);
   5. backward (matched-filter adjoint, tracer adjoint, MLP double-backward)
      and an AdamW step.
 
@@ -76,6 +77,49 @@ def torch_lensless(sdf, grad, dists, z, o, d, inv_s, power, refl, n_valid, K, fs
     out_r = (amp[..., None] * torch.cos(ph)).sum(1) / n_valid[:, None]
     out_i = (amp[..., None] * torch.sin(ph)).sum(1) / n_valid[:, None]
     return out_r, out_i
+
+
+def antenna_groups(o: torch.Tensor):
+    """Rays that leave the same aperture position are one antenna: the matched
+    filter is linear in the echo and its phase depends on the antenna position
+    only, so backprojecting the sum of their signals from that position equals
+    backprojecting each ray's signal from its own origin.  Returns the distinct
+    positions [U,3] and an index matrix [U, max rays per position] into the
+    rays, padded with R (a zero row appended to the signals), and each ray's
+    antenna [R]; the sum over a row of the matrix is taken in a fixed order
+    (deterministic, no atomics)."""
+    R = o.shape[0]
+    pos, inv = torch.unique(o, dim=0, return_inverse=True)
+    U = pos.shape[0]
+    counts = torch.bincount(inv, minlength=U)
+    order = torch.argsort(inv, stable=True)
+    start = torch.cumsum(counts, 0) - counts
+    slot = torch.arange(R, device=o.device) - start[inv[order]]
+    idx = torch.full((U, int(counts.max())), R, dtype=torch.long, device=o.device)
+    idx[inv[order], slot] = order
+    return pos.contiguous(), idx, inv
+
+
+class _MergeRays(torch.autograd.Function):
+    """Sum of each antenna's rays' signals; its adjoint hands every ray its
+    antenna's gradient row (a gather: no accumulation, unlike the autograd of
+    the padded gather, which serialises on the padding index)."""
+
+    @staticmethod
+    def forward(ctx, sig, idx, inv):
+        ctx.save_for_backward(inv)
+        ext = torch.cat([sig, sig.new_zeros(1, sig.shape[1])], 0)
+        return ext[idx].sum(1)
+
+    @staticmethod
+    def backward(ctx, g):
+        (inv,) = ctx.saved_tensors
+        return g[inv], None, None
+
+
+def merge_rays(sig: torch.Tensor, idx: torch.Tensor, inv: torch.Tensor) -> torch.Tensor:
+    """Per-antenna echo: the sum of its rays' signals [R,K] -> [U,K]."""
+    return _MergeRays.apply(sig, idx, inv)
 
 
 @dataclass
@@ -147,6 +191,11 @@ class StageStep:
         self.mf_algo = mf_algo
         self.plan = (make_plan(self.tgt, cfg.K, self.radar["adcSampleRate"], self.radar["As_sci"])
                      if mf_algo == "tiled" else None)
+        # the matched filter's antennas: the distinct ray origins (their rays'
+        # signals summed).  Scaling the sums by U / R keeps the 1/A
+        # normalisation of the per-ray form, so the prediction is unchanged.
+        self.ant_pos, self.ant_idx, self.ant_of_ray = antenna_groups(b.o)
+        self.ant_scale = self.ant_pos.shape[0] / cfg.rays
 
     def loss(self) -> torch.Tensor:
         cfg, b = self.cfg, self.b
@@ -167,12 +216,15 @@ class StageStep:
             sim_r, sim_i = torch_lensless(sdf, grad, b.dists, b.z, b.o, b.d, inv_s, power, refl,
                                           b.n_valid.float(), r["numADCSample"],
                                           r["adcSampleRate"], r["As_sci"], r["fc_start"])
+        U = self.ant_pos.shape[0]
+        ant_r = merge_rays(sim_r, self.ant_idx, self.ant_of_ray) * self.ant_scale
+        ant_i = merge_rays(sim_i, self.ant_idx, self.ant_of_ray) * self.ant_scale
         if self.mf_algo == "tiled":
-            mf_r, mf_i = matched_filter_tiled(sim_r, sim_i, b.o, b.o, self.tgt, R,
-                                              r["numADCSample"], r["adcSampleRate"], r["As_sci"],
-                                              self.tgt.shape[0], plan=self.plan)
+            mf_r, mf_i = matched_filter_tiled(ant_r, ant_i, self.ant_pos, self.ant_pos, self.tgt,
+                                              U, r["numADCSample"], r["adcSampleRate"],
+                                              r["As_sci"], self.tgt.shape[0], plan=self.plan)
         else:
-            mf_r, mf_i = matched_filter_cuda(sim_r, sim_i, b.o, b.o, self.tgt, R,
+            mf_r, mf_i = matched_filter_cuda(ant_r, ant_i, self.ant_pos, self.ant_pos, self.tgt, U,
                                              r["numADCSample"], r["adcSampleRate"], r["As_sci"],
                                              self.tgt.shape[0])
         pred = torch.hypot(mf_r, mf_i)

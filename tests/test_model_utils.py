@@ -98,3 +98,32 @@ def test_sample_volume_is_trilinear_in_xyz():
     # corners hit the corner voxels exactly
     c = mv.sample_volume(vol, np.array([[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]])).numpy()
     assert np.allclose(c, [vol[0, 0, 0], vol[-1, -1, -1]])
+
+
+@pytest.mark.parametrize("R,positions", [(8192, 4096), (300, 7), (5, 5)])
+def test_stage_antenna_groups_cover_every_ray_once(R, positions):
+    """Rays from one aperture position form one antenna; the fixed-order row
+    sums equal a scatter-add of the rays' signals, and every ray is in
+    exactly one group."""
+    from rf_workloads.stage_step import antenna_groups, merge_rays
+
+    rng = np.random.default_rng(R)
+    ap = torch.tensor(rng.uniform(-0.06, 0.06, (positions, 3)), dtype=torch.float32)
+    o = ap[torch.tensor(rng.integers(0, positions, R))]
+    pos, idx, of_ray = antenna_groups(o)
+    inv = torch.cdist(o.double(), pos.double()).argmin(1)
+    assert torch.equal(pos[inv], o) and torch.equal(of_ray, inv)
+    assert len({tuple(p) for p in pos.tolist()}) == pos.shape[0]
+    members = idx[idx < R]
+    assert torch.equal(members.sort().values, torch.arange(R))
+    for u in range(pos.shape[0]):
+        assert torch.all(inv[idx[u][idx[u] < R]] == u)
+    sig = torch.tensor(rng.standard_normal((R, 6)), requires_grad=True)
+    ref = torch.zeros(pos.shape[0], 6, dtype=sig.dtype).index_add_(0, inv, sig)
+    got = merge_rays(sig, idx, of_ray)
+    assert torch.allclose(got, ref, rtol=0, atol=1e-12)
+    # the adjoint: every ray receives its antenna's gradient row
+    g = torch.tensor(rng.standard_normal((pos.shape[0], 6)))
+    (gs,) = torch.autograd.grad(got, sig, g)
+    (gr,) = torch.autograd.grad(ref, sig, g)
+    assert torch.equal(gs, gr)

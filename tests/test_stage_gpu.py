@@ -52,6 +52,35 @@ def test_fused_step_matches_fp32_autograd_step(cuda):
     assert worst <= 2e-2
 
 
+def test_merged_antennas_give_the_per_ray_prediction(cuda):
+    """The step backprojects one echo per aperture position (the sum of its
+    rays' signals, scaled by positions / rays); by linearity that equals
+    backprojecting every ray's signal from its own origin with the 1/rays
+    normalisation."""
+    from geraf.ops.RFMatchedFilter import matched_filter_cuda
+
+    vol, axes = _target(cuda)
+    f = st.StageStep(SMALL, vol, axes, cuda, form="fused", mlp_dtype=None)
+    R, K, r = SMALL.rays, SMALL.K, f.radar
+    assert f.ant_pos.shape[0] < R
+    g = torch.Generator(device="cpu").manual_seed(3)
+    sim_r = torch.randn(R, K, generator=g).to(cuda)
+    sim_i = torch.randn(R, K, generator=g).to(cuda)
+    N = f.tgt.shape[0]
+    ref = matched_filter_cuda(sim_r, sim_i, f.b.o, f.b.o, f.tgt, R, K, r["adcSampleRate"],
+                              r["As_sci"], N)
+    U = f.ant_pos.shape[0]
+    got = matched_filter_cuda(st.merge_rays(sim_r, f.ant_idx, f.ant_of_ray) * f.ant_scale,
+                              st.merge_rays(sim_i, f.ant_idx, f.ant_of_ray) * f.ant_scale,
+                              f.ant_pos, f.ant_pos,
+                              f.tgt, U, K, r["adcSampleRate"], r["As_sci"], N)
+    ref = torch.complex(*ref).double()
+    got = torch.complex(*got).double()
+    err = float((got - ref).norm() / ref.norm())
+    print(f"merged {U} antennas vs {R} rays: rel-L2 {err:.2e}", flush=True)
+    assert err <= 1e-5
+
+
 def test_fused_bf16_step_tracks_fp32_step(cuda):
     """BF16 MLP GEMMs (FP32 accumulation, 8-bit mantissa operands) move the
     loss by about 1% and keep the full gradient direction; a few AdamW steps
