@@ -23,6 +23,8 @@
 // sums are combined in FP64 through LDS in a fixed order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <math.h>
 
 #include "abi_util.hpp"
@@ -39,10 +41,6 @@ constexpr int kHornerBlock = 32;
 #endif
 #ifndef GERAF_MF_WAVES
 #define GERAF_MF_WAVES 4
-#endif
-// Distance evaluation of the tuned forward (FP64 accurate either way).
-#ifndef GERAF_MF_DIST
-#define GERAF_MF_DIST dist3_f64_fast
 #endif
 
 // ---------------------------------------------------------------- tuned forward
@@ -94,12 +92,18 @@ __global__ __launch_bounds__(64 * WAVES) void mf_forward_horner_kernel(
     const double tx = tx_pos[3 * a + 0], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
     const double rx = rx_pos[3 * a + 0], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
     PathPhasors ph[VPL];
+    auto phasors = [&](auto mono) {
 #pragma unroll
-    for (int i = 0; i < VPL; ++i) {
-      const double d = GERAF_MF_DIST(tx, ty, tz, px[i], py[i], pz[i]) +
-                       GERAF_MF_DIST(rx, ry, rz, px[i], py[i], pz[i]);
-      ph[i] = path_phasors((d + pc.bias) * pc.inv_c, pc, -1.0f);
-    }
+      for (int i = 0; i < VPL; ++i) {
+        const double d =
+            round_trip_f64<decltype(mono)::value>(tx, ty, tz, rx, ry, rz, px[i], py[i], pz[i]);
+        ph[i] = path_phasors((d + pc.bias) * pc.inv_c, pc, -1.0f);
+      }
+    };
+    if (monostatic(tx_pos, rx_pos, a))
+      phasors(std::true_type{});
+    else
+      phasors(std::false_type{});
 
     // a*K < 2^31 (checked on the host); readfirstlane keeps the row offset,
     // and with it every echo address below, in SGPRs.

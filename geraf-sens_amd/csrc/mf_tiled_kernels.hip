@@ -36,6 +36,8 @@
 // the phasor rotated along the lane's frequencies, Kahan over the tiles).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <math.h>
 
 #include <algorithm>
@@ -199,21 +201,28 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
     const size_t ct = __builtin_amdgcn_readfirstlane((unsigned)ai) * (size_t)ntiles + t;
     const double tac = tau_c[ct];
     const float2* __restrict__ cf = coef + ct * NT;
+    auto pairs = [&](auto mono) {
 #pragma unroll
-    for (int i = 0; i < VPL; ++i) {
-      const double tau = (dist3_f64_fast(tx, ty, tz, px[i], py[i], pz[i]) +
-                          dist3_f64_fast(rx, ry, rz, px[i], py[i], pz[i]) + pc.bias) *
-                         pc.inv_c;
-      const float u = (float)((tau - tac) * invH);
-      float Pr = cf[NT - 1].x, Pi = cf[NT - 1].y;
+      for (int i = 0; i < VPL; ++i) {
+        const double tau = (round_trip_f64<decltype(mono)::value>(tx, ty, tz, rx, ry, rz, px[i],
+                                                                  py[i], pz[i]) +
+                            pc.bias) *
+                           pc.inv_c;
+        const float u = (float)((tau - tac) * invH);
+        float Pr = cf[NT - 1].x, Pi = cf[NT - 1].y;
 #pragma unroll
-      for (int n = NT - 2; n >= 0; --n) {
-        Pr = fmaf(Pr, u, cf[n].x);
-        Pi = fmaf(Pi, u, cf[n].y);
+        for (int n = NT - 2; n >= 0; --n) {
+          Pr = fmaf(Pr, u, cf[n].x);
+          Pi = fmaf(Pi, u, cf[n].y);
+        }
+        const float2 cs = phasor_cycles_hw((float)centered_frac(fcen * tau));
+        acc[i].add(cmul(make_float2(cs.x, -cs.y), make_float2(Pr, Pi)));
       }
-      const float2 cs = phasor_cycles_hw((float)centered_frac(fcen * tau));
-      acc[i].add(cmul(make_float2(cs.x, -cs.y), make_float2(Pr, Pi)));
-    }
+    };
+    if (monostatic(tx_pos, rx_pos, a))
+      pairs(std::true_type{});
+    else
+      pairs(std::false_type{});
   }
 #pragma unroll
   for (int i = 0; i < VPL; ++i) {
@@ -280,22 +289,29 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
     float Pr[NT], Pi[NT];
 #pragma unroll
     for (int n = 0; n < NT; ++n) Pr[n] = Pi[n] = 0.0f;
+    auto pairs = [&](auto mono) {
 #pragma unroll
-    for (int i = 0; i < AVPL; ++i) {
-      const double tau = (dist3_f64_fast(tx, ty, tz, px[i], py[i], pz[i]) +
-                          dist3_f64_fast(rx, ry, rz, px[i], py[i], pz[i]) + pc.bias) *
-                         pc.inv_c;
-      const float u = (float)((tau - tac) * invH);
-      const float2 p = cmul(make_float2(cr[i], ci[i]),
-                            phasor_cycles_hw((float)centered_frac(fcen * tau)));
-      float q = 1.0f;  // u^n
+      for (int i = 0; i < AVPL; ++i) {
+        const double tau = (round_trip_f64<decltype(mono)::value>(tx, ty, tz, rx, ry, rz, px[i],
+                                                                  py[i], pz[i]) +
+                            pc.bias) *
+                           pc.inv_c;
+        const float u = (float)((tau - tac) * invH);
+        const float2 p = cmul(make_float2(cr[i], ci[i]),
+                              phasor_cycles_hw((float)centered_frac(fcen * tau)));
+        float q = 1.0f;  // u^n
 #pragma unroll
-      for (int n = 0; n < NT; ++n) {
-        Pr[n] = fmaf(p.x, q, Pr[n]);
-        Pi[n] = fmaf(p.y, q, Pi[n]);
-        q *= u;
+        for (int n = 0; n < NT; ++n) {
+          Pr[n] = fmaf(p.x, q, Pr[n]);
+          Pi[n] = fmaf(p.y, q, Pi[n]);
+          q *= u;
+        }
       }
-    }
+    };
+    if (monostatic(tx_pos, rx_pos, a))
+      pairs(std::true_type{});
+    else
+      pairs(std::false_type{});
     // M'_n = j^n / n! * sum over the tile
     float2* __restrict__ out = mom + ((size_t)ai * ntiles + t) * NT;
     float inv_fact = 1.0f;

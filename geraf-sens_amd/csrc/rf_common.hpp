@@ -58,6 +58,29 @@ __device__ __forceinline__ double dist3_f64_fast(double ax, double ay, double az
   return sqrt_f64_newton(fma(dx, dx, fma(dy, dy, dz * dz)));
 }
 
+// Whether antenna a transmits and receives at one position (the three
+// coordinates equal bit for bit): then both legs of the round trip are the
+// same distance, computed once.  The loads are wave-uniform (scalar), and so
+// is the result, which lets the caller branch instead of selecting.
+__device__ __forceinline__ bool monostatic(const float* __restrict__ tx_pos,
+                                           const float* __restrict__ rx_pos, int a) {
+  const unsigned* t = reinterpret_cast<const unsigned*>(tx_pos) + 3 * a;
+  const unsigned* r = reinterpret_cast<const unsigned*>(rx_pos) + 3 * a;
+  const int same = (t[0] == r[0]) & (t[1] == r[1]) & (t[2] == r[2]);
+  return __builtin_amdgcn_readfirstlane(same) != 0;
+}
+
+// |t - p| + |r - p| (FP64, dist3_f64_fast).  With Mono the second leg is the
+// first one again: d + d, bit for bit what the two-leg sum gives when t == r.
+template <bool Mono>
+__device__ __forceinline__ double round_trip_f64(double tx, double ty, double tz, double rx,
+                                                 double ry, double rz, double px, double py,
+                                                 double pz) {
+  const double dt = dist3_f64_fast(tx, ty, tz, px, py, pz);
+  if constexpr (Mono) return dt + dt;
+  return dt + dist3_f64_fast(rx, ry, rz, px, py, pz);
+}
+
 // Fractional part of x in [-0.5, 0.5].
 __device__ __forceinline__ double centered_frac(double x) { return x - rint(x); }
 
