@@ -116,22 +116,29 @@ def linear(x: torch.Tensor, lin: nn.Linear, dtype: Optional[torch.dtype]) -> tor
     """x @ W^T + b in `dtype` (None: the parameters' dtype) for a batch of
     points x [M, in]."""
     w = lin.weight if dtype is None else lin.weight.to(dtype)
-    y = tall_matmul(x, w.t())
     if lin.bias is None:
-        return y
-    return _AddBias.apply(y, lin.bias if dtype is None else lin.bias.to(dtype))
+        return tall_matmul(x, w.t())
+    return _TallLinear.apply(x, w.t(), lin.bias if dtype is None else lin.bias.to(dtype))
 
 
-class _AddBias(torch.autograd.Function):
-    """y + b, b broadcast over the rows; db is a blocked column sum."""
+class _TallLinear(torch.autograd.Function):
+    """x @ B + b for a tall x, with the bias added in the GEMM's epilogue
+    (one pass over the output instead of a product and a separate add); the
+    backward is the tall products and a blocked column sum, differentiable to
+    any order."""
 
     @staticmethod
-    def forward(ctx, y, b):
-        return y + b
+    def forward(ctx, x, b_mat, bias):
+        ctx.save_for_backward(x, b_mat)
+        return torch.addmm(bias, x, b_mat)
 
     @staticmethod
     def backward(ctx, gy):
-        return gy, col_sum(gy) if ctx.needs_input_grad[1] else None
+        x, b_mat = ctx.saved_tensors
+        gx = tall_matmul(gy, b_mat.t()) if ctx.needs_input_grad[0] else None
+        gb_mat = tall_tn(x, gy) if ctx.needs_input_grad[1] else None
+        gbias = col_sum(gy) if ctx.needs_input_grad[2] else None
+        return gx, gb_mat, gbias
 
 
 class _SoftplusBF16(torch.autograd.Function):
