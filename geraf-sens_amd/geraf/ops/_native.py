@@ -49,6 +49,7 @@ _PARAMS = ctypes.POINTER(FmcwParams)
 # include/geraf_rf.h.
 _SIGNATURES = {
     "geraf_mf_forward_f32": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
+    "geraf_mf_forward_f32_mma": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P, _I64, _P],
     "geraf_mf_forward_plain_f32": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_forward_f64": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_backward_f32": [_P] * 7 + [_I64] * 4 + [_PARAMS, _P],
@@ -106,13 +107,16 @@ def load() -> ctypes.CDLL:
         lib.geraf_mf_tiled_workspace_bytes.restype = ctypes.c_int64
         lib.geraf_mf_tiled_workspace_bytes.argtypes = [_I64, _I64, _I64, ctypes.c_int32,
                                                        ctypes.c_int32]
+        lib.geraf_mf_forward_mma_workspace_bytes.restype = ctypes.c_int64
+        lib.geraf_mf_forward_mma_workspace_bytes.argtypes = [_I64, _I64]
         _lib = lib
     return _lib
 
 
 def exported_symbols():
     return list(_SIGNATURES) + ["geraf_rf_last_error", "geraf_rf_version",
-                                "geraf_mf_tiled_workspace_bytes"]
+                                "geraf_mf_tiled_workspace_bytes",
+                                "geraf_mf_forward_mma_workspace_bytes"]
 
 
 def version() -> str:

@@ -4,7 +4,8 @@ readings calibrate those counters (known traffic: 1 GiB read, 1 GiB written).
 
     rocprofv3 --pmc <counters> -d <dir> -o pmc -- python3 geraf-sens_amd/rf_workloads/profile_driver.py
 
-Kernels, in dispatch order: the copy; mf_forward_horner_kernel and
+Kernels, in dispatch order: the copy; mf_forward_horner_kernel,
+mf_mma_prep_kernel + mf_forward_mma_kernel (the matrix-core form) and
 mf_backward_kernel on the 64x64 MIMO / K=512 / 128^3 workload;
 lensless_forward_kernel and
 lensless_backward_kernel on 4,096 rays x 128 samples, K=256, then (with
@@ -99,6 +100,13 @@ def main() -> None:
     _native.call("geraf_mf_forward_f32", *map(_native.ptr, (re, im, tx, rx, o_r, o_i, vox)),
                  A, K, N, prm, _native.stream_of(re))
     torch.cuda.synchronize()
+    print("profile_driver: matched filter, matrix-core form, same workload", flush=True)
+    ws_bytes = int(_native.load().geraf_mf_forward_mma_workspace_bytes(A, K))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    _native.call("geraf_mf_forward_f32_mma", *map(_native.ptr, (re, im, tx, rx, o_r, o_i, vox)),
+                 A, K, N, prm, _native.ptr(ws), ws_bytes, _native.stream_of(re))
+    torch.cuda.synchronize()
+    del ws
     print("profile_driver: matched-filter adjoint, same workload", flush=True)
     a_r = torch.empty((A, K), device=dev)
     a_i = torch.empty_like(a_r)

@@ -66,6 +66,25 @@ int geraf_mf_forward_f32(const float* real, const float* imag, const float* tx_p
                          const float* voxel, int64_t A, int64_t K, int64_t N,
                          const geraf_fmcw_params* params, geraf_stream_t stream);
 
+/* Same operator (_C.MatchedFilter_forward, vision.cpp:81-84;
+ * MatchedFilter_cuda_kernel.cu:65-115) on the matrix cores, for
+ * 1 <= K <= 2048 (faster than geraf_mf_forward_f32 from K ~ 300 on).  With
+ * k = 32*alpha + beta, each antenna's sum is a complex matrix product over
+ * beta on v_mfma_f32_16x16x32_f16, every real operand split into FP16 hi + lo
+ * and each real product taken as three (FP32-accurate products, FP32
+ * accumulation); phases as in geraf_mf_forward_f32 (FP64 distances and
+ * reduction).  workspace: device memory of workspace_bytes >=
+ * geraf_mf_forward_mma_workspace_bytes(A, K), 256-byte aligned, owned by the
+ * caller (the echo in operand order); nothing is allocated.  K > 2048 is an
+ * argument error.  geraf_mf_forward_mma_workspace_bytes returns -1 for
+ * negative sizes. */
+int64_t geraf_mf_forward_mma_workspace_bytes(int64_t A, int64_t K);
+int geraf_mf_forward_f32_mma(const float* real, const float* imag, const float* tx_pos,
+                             const float* rx_pos, float* out_real, float* out_imag,
+                             const float* voxel, int64_t A, int64_t K, int64_t N,
+                             const geraf_fmcw_params* params, void* workspace,
+                             int64_t workspace_bytes, geraf_stream_t stream);
+
 /* Same operator, plain FP32 form: one thread per voxel, per-term sincosf in
  * FP32 exactly as MatchedFilter_cuda_kernel.cu:87-111 computes it. Kept as the
  * reference-formula baseline; not the production path. */

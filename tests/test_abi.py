@@ -52,6 +52,20 @@ def test_version_and_error_reporting_without_gpu():
     st = lib.geraf_mf_forward_f32(None, None, None, None, None, None, None, 4, 8, 0,
                                   ctypes.byref(prm), None)
     assert st == 0
+    # the matrix-core forward: its K limit, its workspace contract and empty work
+    assert lib.geraf_mf_forward_mma_workspace_bytes(-1, 8) == -1
+    need = lib.geraf_mf_forward_mma_workspace_bytes(3, 513)
+    assert need >= 3 * 2 * 4 * 64 * 16 + 3 * 4
+    mma = lib.geraf_mf_forward_f32_mma
+    st = mma(*([None] * 7), 2, 2049, 5, ctypes.byref(prm), None, 0, None)
+    assert st == 1 and b"2048" in lib.geraf_rf_last_error()
+    st = mma(*([ctypes.c_void_p(256)] * 7), 3, 513, 5, ctypes.byref(prm), None, need, None)
+    assert st == 1 and b"NULL" in lib.geraf_rf_last_error()
+    st = mma(*([ctypes.c_void_p(256)] * 7), 3, 513, 5, ctypes.byref(prm),
+             ctypes.c_void_p(4096), need - 1, None)
+    assert st == 1 and b"workspace too small" in lib.geraf_rf_last_error()
+    st = mma(*([None] * 7), 3, 513, 0, ctypes.byref(prm), None, 0, None)
+    assert st == 0
 
 
 def test_call_raises_runtime_error_with_message():

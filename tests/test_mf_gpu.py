@@ -37,11 +37,17 @@ def run_mf(scene_or_arrays, dev, form="f32"):
     N = v.shape[1]
     o_r = torch.empty(N, device=dev, dtype=dt)
     o_i = torch.empty(N, device=dev, dtype=dt)
-    name = {"f32": "geraf_mf_forward_f32", "plain32": "geraf_mf_forward_plain_f32",
-            "f64": "geraf_mf_forward_f64"}[form]
     params = _native.make_params(prm["f0"], prm["slope"], prm["fs"], prm["c"], prm["bias"])
-    _native.call(name, *map(_native.ptr, (re, im, t, r, o_r, o_i, v)), A, K, N, params,
-                 _native.stream_of(re))
+    ptrs = list(map(_native.ptr, (re, im, t, r, o_r, o_i, v)))
+    if form == "mma":
+        size = int(_native.load().geraf_mf_forward_mma_workspace_bytes(A, K))
+        ws = torch.empty(size, dtype=torch.uint8, device=dev)
+        _native.call("geraf_mf_forward_f32_mma", *ptrs, A, K, N, params, _native.ptr(ws), size,
+                     _native.stream_of(re))
+    else:
+        name = {"f32": "geraf_mf_forward_f32", "plain32": "geraf_mf_forward_plain_f32",
+                "f64": "geraf_mf_forward_f64"}[form]
+        _native.call(name, *ptrs, A, K, N, params, _native.stream_of(re))
     torch.cuda.synchronize()
     return o_r.double().cpu().numpy() + 1j * o_i.double().cpu().numpy()
 
@@ -91,6 +97,87 @@ def test_shapes_vs_oracle(cuda, A, K, N):
     ref = orc.mf_forward(echo.real, echo.imag, tx, rx, vox, **prm)
     out = run_mf((echo, tx, rx, vox, prm), cuda)
     assert rel_l2(out, ref) <= 3e-6 * np.sqrt(K) + 1e-6
+
+
+def _mma_case(A, K, N, mono, seed, row_scale=None):
+    rng = np.random.default_rng(seed)
+    echo = (rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K))).astype(np.complex64)
+    if row_scale is not None:
+        echo = (echo * row_scale[:, None]).astype(np.complex64)
+    tx = rng.uniform(-0.05, 0.05, (A, 3)).astype(np.float32)
+    tx[:, 2] = 0.0
+    rx = tx.copy() if mono else (tx + np.float32([0.002, 0.001, 0])).astype(np.float32)
+    vox = rng.uniform(-0.1, 0.1, (N, 3)).astype(np.float32)
+    vox[:, 2] += 0.3
+    # a 77-81 GHz sweep over the K samples, as the workloads use
+    fs = 1.25e6 * K / 64
+    prm = dict(f0=mf_scenes.MF_F0, slope=float(np.float32(701500.0)), fs=fs,
+               c=mf_scenes.SPEED_OF_LIGHT, bias=mf_scenes.PATH_BIAS)
+    return echo, tx, rx, vox, prm
+
+
+@pytest.mark.parametrize("A,K,N,mono", [(1, 257, 1, False), (3, 300, 100, True),
+                                        (5, 512, 1030, False), (2, 513, 65, True),
+                                        (2, 1000, 700, False), (1, 1024, 257, True),
+                                        (1, 2048, 300, False), (3, 2047, 129, True),
+                                        (4, 1, 64, False), (6, 100, 63, True)])
+def test_mma_forward_shapes_vs_oracle(cuda, A, K, N, mono):
+    """The matrix-core forward over one to four 512-frequency tiles, partial
+    tiles (K = 257, 513, 1000, 2047), K below one row of 32 (1, 100), ragged
+    voxel counts (partial groups of 16 and waves of 64, N = 1), single
+    antennas, monostatic and bistatic antennas, against the FP64 oracle."""
+    echo, tx, rx, vox, prm = _mma_case(A, K, N, mono, seed=K * 7 + A)
+    ref = orc.mf_forward(echo.real, echo.imag, tx, rx, vox, **prm)
+    out = run_mf((echo, tx, rx, vox, prm), cuda, "mma")
+    err = rel_l2(out, ref)
+    print(f"mma A={A} K={K} N={N} mono={mono}: rel-L2 vs FP64 {err:.2e}")
+    assert err <= TOL_TUNED_VS_F64
+
+
+def test_mma_forward_per_antenna_scaling(cuda):
+    """Echo rows 10^-30 .. 10^30 apart (the FP16 operands are scaled per
+    antenna by a power of two), a row of zeros, and a row whose entries span
+    twelve orders of magnitude: each antenna alone, and all together against
+    the FP64 oracle."""
+    A, K, N = 6, 512, 300
+    scale = np.array([1e-30, 1e30, 0.0, 1.0, 3e-4, 7e5], dtype=np.float64)
+    echo, tx, rx, vox, prm = _mma_case(A, K, N, False, seed=11, row_scale=scale)
+    rng = np.random.default_rng(12)
+    echo[3] = (echo[3] * 10.0 ** rng.uniform(-12, 0, K)).astype(np.complex64)
+    for a in range(A):
+        one = (echo[a:a + 1], tx[a:a + 1], rx[a:a + 1], vox, prm)
+        ref = orc.mf_forward(echo[a:a + 1].real, echo[a:a + 1].imag, tx[a:a + 1], rx[a:a + 1],
+                             vox, **prm)
+        out = run_mf(one, cuda, "mma")
+        if scale[a] == 0.0:
+            assert np.all(out == 0)
+            continue
+        err = rel_l2(out, ref)
+        print(f"antenna scale {scale[a]:.0e}: rel-L2 {err:.2e}")
+        assert np.all(np.isfinite(out)) and err <= TOL_TUNED_VS_F64
+    ref = orc.mf_forward(echo.real, echo.imag, tx, rx, vox, **prm)
+    assert rel_l2(run_mf((echo, tx, rx, vox, prm), cuda, "mma"), ref) <= TOL_TUNED_VS_F64
+
+
+def test_mma_forward_cfg1_and_drop_in_dispatch(cuda, cfg1):
+    """The 16x16 / K=128 / 32^3 volume through the matrix-core entry point;
+    and the drop-in op at K = 512 (it dispatches to the matrix-core form)
+    against the Horner kernel."""
+    from geraf.ops.RFMatchedFilter import matched_filter_cuda
+
+    sc, ref = cfg1
+    err = rel_l2(run_mf(sc, cuda, "mma"), ref)
+    print(f"cfg1 through the matrix-core form: rel-L2 {err:.2e}")
+    assert err <= TOL_TUNED_VS_F64
+    echo, tx, rx, vox, prm = _mma_case(8, 512, 2000, False, seed=5)
+    horner = run_mf((echo, tx, rx, vox, prm), cuda, "f32")
+    t = lambda x: torch.as_tensor(np.ascontiguousarray(x), device=cuda)  # noqa: E731
+    o_r, o_i = matched_filter_cuda(t(echo.real), t(echo.imag), t(tx), t(rx), t(vox), 8, 512,
+                                   prm["fs"], prm["slope"], 2000)
+    op = o_r.double().cpu().numpy() + 1j * o_i.double().cpu().numpy()
+    ref = orc.mf_forward(echo.real, echo.imag, tx, rx, vox, **prm)
+    assert rel_l2(op, ref) <= TOL_TUNED_VS_F64
+    assert rel_l2(op, horner) <= 2e-6 and not np.array_equal(op, horner)
 
 
 def test_empty_volume(cuda):
