@@ -196,10 +196,10 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
       par[wave][2][lane] = make_float4(w2.x, w2.y, w4.x, w4.y);
     }
     wave_sync();
-#ifndef GERAF_MMA_GUNROLL
-#define GERAF_MMA_GUNROLL 1
-#endif
-#pragma unroll GERAF_MMA_GUNROLL
+    // groups unrolled (the accumulators stay in registers), one at a time: the
+    // scheduling barrier keeps the compiler from interleaving them, which
+    // would cost registers (occupancy) and gain nothing
+#pragma unroll
     for (int G = 0; G < 4; ++G) {
       const float4 p = par[wave][0][16 * G + j];
       const float4 q1 = par[wave][1][16 * G + j];
@@ -242,6 +242,7 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
         part.y += xh.y;
       }
       acc[G].add(make_float2(part.x * up, part.y * up));
+      __builtin_amdgcn_sched_barrier(0);
     }
     wave_sync();
   }
