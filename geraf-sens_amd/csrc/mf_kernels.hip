@@ -495,6 +495,165 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mfma_kernel(
   }
 }
 
+// The same adjoint with FP16 hi/lo operands on v_mfma_f32_16x16x32_f16
+// (depth 32 voxels per instruction instead of 4): A[i][d] = X[16 m + i, v_d]
+// and B[d][j] = Y[v_d, j] with d = 8 (lane / 16) + element, i.e. lane i + 16 g
+// forms X and Y for the eight voxels 8 g .. 8 g + 7 of its 32-voxel chunk.
+// Each real operand is split into hi = fp16(x) and lo = fp16(x - hi), a real
+// product is hi*hi + hi*lo + lo*hi into one FP32 accumulator, a complex
+// product four real ones (12 instructions per 16 x 16 tile and 32 voxels).
+// X and Y are unit phasors times cg; cg is scaled per voxel tile by a power
+// of two (max |cg| in [0.5, 1)) so FP16 neither overflows nor drops small
+// gradients, and the tile's sums are scaled back before the FP64 flush.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void split8_adj(const float2* x, h8& rh, h8& rl, h8& ih, h8& il) {
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    const h2 r = __builtin_convertvector((f2v){x[e].x, x[e + 1].x}, h2);
+    const h2 i = __builtin_convertvector((f2v){x[e].y, x[e + 1].y}, h2);
+    rh[e] = r[0];
+    rh[e + 1] = r[1];
+    ih[e] = i[0];
+    ih[e + 1] = i[1];
+    rl[e] = (_Float16)(x[e].x - (float)r[0]);
+    rl[e + 1] = (_Float16)(x[e + 1].x - (float)r[1]);
+    il[e] = (_Float16)(x[e].y - (float)i[0]);
+    il[e + 1] = (_Float16)(x[e + 1].y - (float)i[1]);
+  }
+}
+
+__device__ __forceinline__ f32x4 mma3_adj(const h8& ah, const h8& al, const h8& bh,
+                                          const h8& bl, f32x4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+template <int MT>
+__global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
+    const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
+    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
+    float* __restrict__ grad_real, float* __restrict__ grad_imag,
+    const float* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc) {
+  __shared__ AdjMfmaStage st;
+  __shared__ double red[kAdjWaves][kAdjTile];
+  __shared__ float wmax[kAdjWaves];
+  const int a = blockIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = lane & 15, g = lane >> 4;
+  const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
+  const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
+  const bool mono = monostatic(tx_pos, rx_pos, a);
+
+  double sr[MT][4], si[MT][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sr[m][v] = si[m][v] = 0.0;
+
+  for (int base = 0; base < N; base += kAdjTile) {
+    float cmax;
+    {
+      const int v = base + threadIdx.x;
+      float2 cg = make_float2(0.0f, 0.0f), rot = make_float2(1.0f, 0.0f);
+      double cyc1 = 0.0;
+      if (v < N) {
+        const float gr = gout_real[v], gi = gout_imag[v];
+        const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
+        const double d = mono ? round_trip_f64<true>(tx, ty, tz, rx, ry, rz, px, py, pz)
+                              : round_trip_f64<false>(tx, ty, tz, rx, ry, rz, px, py, pz);
+        const double tau = (d + pc.bias) * pc.inv_c;
+        cyc1 = pc.dfs * tau;
+        cg = cmul(make_float2(gr, gi), phasor_cycles((float)centered_frac(pc.f0 * tau), 1.0f));
+        if (MT > 1) rot = phasor_cycles_hw((float)centered_frac(256.0 * cyc1));
+      }
+      float mx = fmaxf(fabsf(cg.x), fabsf(cg.y));
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+      if (lane == 0) wmax[wave] = mx;
+      st.cg[threadIdx.x] = cg;
+      st.rot[threadIdx.x] = rot;
+      st.cyc1[threadIdx.x] = split_cycles(cyc1);
+    }
+    __syncthreads();
+    cmax = wmax[0];
+#pragma unroll
+    for (int w = 1; w < kAdjWaves; ++w) cmax = fmaxf(cmax, wmax[w]);
+    // cg * 2^-e with 2^e > max |cg| (exact); the tile's sums are scaled back
+    int e = 0;
+    if (cmax > 0.0f && isfinite(cmax)) (void)frexpf(cmax, &e);
+    const float down = ldexpf(1.0f, -e);
+    const double up = ldexp(1.0, e);
+    f32x4 pr[MT], pi[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) pr[m] = pi[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+    for (int c = 0; c < 2; ++c) {
+      const int v0 = wave * kWave + 32 * c + 8 * g;  // this lane's eight voxels
+      float2 x[8], y[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const SplitCycles c1 = st.cyc1[v0 + q];
+        y[q] = phasor_cycles_hw(frac_multiple((float)i, c1));
+        const float2 cg = st.cg[v0 + q];
+        x[q] = cmul(make_float2(cg.x * down, cg.y * down),
+                    phasor_cycles_hw(frac_multiple((float)(16 * i), c1)));
+      }
+      h8 yrh, yrl, yih, yil;
+      split8_adj(y, yrh, yrl, yih, yil);
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        if (m > 0) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) x[q] = cmul(x[q], st.rot[v0 + q]);
+        }
+        h8 xrh, xrl, xih, xil;
+        split8_adj(x, xrh, xrl, xih, xil);
+        pr[m] = mma3_adj(xrh, xrl, yrh, yrl, pr[m]);
+        pr[m] = mma3_adj(-xih, -xil, yih, yil, pr[m]);
+        pi[m] = mma3_adj(xrh, xrl, yih, yil, pi[m]);
+        pi[m] = mma3_adj(xih, xil, yrh, yrl, pi[m]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        sr[m][v] += (double)pr[m][v] * up;
+        si[m][v] += (double)pi[m][v] * up;
+      }
+    __syncthreads();
+  }
+
+  // reduce the waves, one 256-frequency tile at a time (as the FP32 form)
+  const int tl = threadIdx.x >> 3, te = threadIdx.x & 7;
+  const int k_of_t = 64 * (tl >> 4) + 16 * (te >> 1) + (tl & 15);
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      red[wave][8 * lane + 2 * v] = sr[m][v];
+      red[wave][8 * lane + 2 * v + 1] = si[m][v];
+    }
+    __syncthreads();
+    double acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < kAdjWaves; ++w) acc += red[w][threadIdx.x];
+    const int k = 256 * m + k_of_t;
+    if (k < K) {
+      float* out = (te & 1) ? grad_imag : grad_real;
+      out[(size_t)a * K + k] = (float)(acc * inv_norm);
+    }
+    __syncthreads();
+  }
+}
+
 // Plain FP64 adjoint: one thread per (a, k), FP64 phase and sincos per voxel.
 __global__ __launch_bounds__(256) void mf_backward_f64_kernel(
     const double* __restrict__ gout_real, const double* __restrict__ gout_imag,
@@ -522,6 +681,10 @@ __global__ __launch_bounds__(256) void mf_backward_f64_kernel(
   grad_real[idx] = acc_r * inv_norm;
   grad_imag[idx] = acc_i * inv_norm;
 }
+
+#ifndef GERAF_MF_ADJ_KERNEL
+#define GERAF_MF_ADJ_KERNEL mf_backward_mma16_kernel
+#endif
 
 // Waves wanted per launch: 8 per SIMD of the 256-CU part, so that two of them
 // are ready to dual-issue most of the time.
@@ -633,7 +796,7 @@ extern "C" int geraf_mf_backward_f32(const float* grad_out_real, const float* gr
     const double inv_norm = 1.0 / (double)norm_count;
     hipStream_t st = (hipStream_t)stream;
 #define GERAF_MF_ADJ(MT_)                                                                      \
-  hipLaunchKernelGGL(mf_backward_mfma_kernel<MT_>, dim3((unsigned)A), dim3(kAdjTile), 0, st, \
+  hipLaunchKernelGGL(GERAF_MF_ADJ_KERNEL<MT_>, dim3((unsigned)A), dim3(kAdjTile), 0, st,     \
                      grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag,     \
                      voxel, (int)A, (int)K, (int)N, inv_norm, pc)
     if (K <= 256)
@@ -643,7 +806,7 @@ extern "C" int geraf_mf_backward_f32(const float* grad_out_real, const float* gr
     else
       GERAF_MF_ADJ(4);
 #undef GERAF_MF_ADJ
-    return check_launch("mf_backward_mfma_kernel");
+    return check_launch("mf_backward_mma16_kernel");
   }
   // chunks per block: a power of two dividing kAdjWaves, at most the chunk count
   const int chunks = (int)((K + kAdjChunk - 1) / kAdjChunk);
