@@ -40,8 +40,9 @@
 // Backward: pass A walks the tiles forward, keeps the transmittance carry at
 // each tile start, and queues (tau, sample) of every emitting sample; full
 // 64-sample batches evaluate h_s = Re sum_k g[r,k] exp(-j phi_sk), for
-// K > 128 as a matrix product (see h_mfma), else by two-level Horner (one
-// sample per lane, gradient row read through the scalar cache).  Pass B walks the
+// K > 128 as a matrix product with FP16 hi/lo operands (see h_mma16; the
+// FP32-operand form h_mfma stays for GERAF_LL_H16=0), else by two-level
+// Horner (one sample per lane, gradient row read through the scalar cache).  Pass B walks the
 // tiles backward, recomputes the compositing and geometry, and propagates
 // through the transmittance with a reverse affine scan
 // W_s = gT_s + t_s W_{s+1}, so dL/dalpha_i = (direct) - T_i W_{i+1} needs no
@@ -557,6 +558,75 @@ __device__ __forceinline__ void h_mfma(const double* __restrict__ btau,
   }
 }
 
+// The same h_s with FP16 hi/lo operands on v_mfma_f32_16x16x32_f16, for
+// 16 queued samples at a time.  With k = 32 a + b (b < 32),
+//   sum_k g[k] exp(-j phi_sk) = sum_a Xc[a,s] T[a,s],  T[a,s] = sum_b G[a,b] Yc[s,b],
+//   Xc = exp(-j 2 pi (f0 + 32 a dfs) tau_s),  Yc = exp(-j 2 pi b dfs tau_s):
+// the gradient row is the A operand (tile m of 16 rows of a = 512
+// frequencies; lane i + 16 g holds G[16 m + i, 8 g .. 8 g + 7], split, scaled
+// by a power of two for the ray), Yc the B operand (lane j + 16 g: sample j,
+// b = 8 g .. 8 g + 7, one hardware phasor and a rotation tree), four real
+// products of three instructions per tile; lane j + 16 q then contracts its
+// four rows of T with Xc by Horner in exp(-j 2 pi 32 dfs tau) and the four
+// lane quarters are summed with cross-lane shuffles.
+template <int MT16>
+__device__ __forceinline__ void h_mma16(const double* __restrict__ btau,
+                                        const int* __restrict__ bsid, float* __restrict__ hrow,
+                                        int cnt, int lane, const h8 (*gop)[4], float up,
+                                        const PhaseConsts& pc, float inv_n) {
+  const int j = lane & 15, g = lane >> 4;
+  for (int s0 = 0; s0 < cnt; s0 += 16) {
+    const int idx = s0 + j;
+    const bool ok = idx < cnt;
+    const double tau = ok ? btau[idx] : 0.0;
+    const SplitCycles c = split_cycles13(pc.dfs * tau);
+    const float r0 = (float)centered_frac(pc.f0 * tau);
+    const float2 w1 = cis_neg(frac_multiple(1.0f, c));
+    const float2 w2 = cmul(w1, w1), w4 = cmul(w2, w2);
+    const float2 y0 = cis_neg(frac_multiple((float)(8 * g), c));
+    float2 y[8];
+    y[0] = y0;
+    y[1] = cmul(y0, w1);
+    y[2] = cmul(y0, w2);
+    y[3] = cmul(y[1], w2);
+    y[4] = cmul(y0, w4);
+    y[5] = cmul(y[1], w4);
+    y[6] = cmul(y[2], w4);
+    y[7] = cmul(y[3], w4);
+    h8 yrh, yrl, yih, yil;
+    split8(y, yrh, yrl, yih, yil);
+    const float2 st = cis_neg(frac_multiple(32.0f, c));
+    float h = 0.0f;
+#pragma unroll
+    for (int m = 0; m < MT16; ++m) {
+      const h8& grh = gop[m][0];
+      const h8& grl = gop[m][1];
+      const h8& gih = gop[m][2];
+      const h8& gil = gop[m][3];
+      f32x4v tr = {0.0f, 0.0f, 0.0f, 0.0f}, ti = {0.0f, 0.0f, 0.0f, 0.0f};
+      tr = mma3(grh, grl, yrh, yrl, tr);
+      tr = mma3(-gih, -gil, yih, yil, tr);
+      ti = mma3(grh, grl, yih, yil, ti);
+      ti = mma3(gih, gil, yrh, yrl, ti);
+      float2 H = make_float2(tr[3], ti[3]);
+#pragma unroll
+      for (int r = 2; r >= 0; --r) {
+        const float2 t = cmul(H, st);
+        H = make_float2(t.x + tr[r], t.y + ti[r]);
+      }
+      const float2 x = cis_neg(r0 + frac_multiple((float)(32 * (16 * m + 4 * g)), c));
+      h = fmaf(x.x, H.x, fmaf(-x.y, H.y, h));
+    }
+    h += __shfl_xor(h, 16, kWave);
+    h += __shfl_xor(h, 32, kWave);
+    if (ok && g == 0) hrow[bsid[idx]] = h * up * inv_n;
+  }
+}
+
+#ifndef GERAF_LL_H16
+#define GERAF_LL_H16 1
+#endif
+
 // MT = 0: h_s by two-level Horner (one sample per lane); MT > 0: matrix-core
 // form with MT tiles of 256 frequencies (K <= 256 MT).
 template <int MT>
@@ -586,9 +656,10 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
 
   // h_s = (1/n) Re sum_k g[r,k] exp(-j phi_sk) for the first `cnt` queued
   // samples, one per lane (the gradient row is wave-uniform: scalar loads).
-  constexpr int kT = MT ? MT : 1;
+  constexpr bool kH16 = GERAF_LL_H16 && MT > 0;
+  constexpr int kT = (MT && !kH16) ? MT : 1;
   float ga_r[kT][4], ga_i[kT][4];
-  if constexpr (MT > 0) {
+  if constexpr (MT > 0 && !kH16) {
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -598,6 +669,36 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
         ga_i[m][t] = k < a.K ? ld_row(gi_row, (unsigned)k) : 0.0f;
       }
   }
+  // FP16 operands of the gradient row: 512-frequency tiles, lane i + 16 g
+  // holds g[512 m + 32 i + 8 g .. + 7], scaled by 2^-e (2^e > max |g|)
+  constexpr int kT16 = kH16 ? (MT + 1) / 2 : 1;
+  h8 gop[kT16][4];
+  float g_up = 1.0f;
+  if constexpr (kH16) {
+    float2 gv[kT16][8];
+    float mx = 0.0f;
+#pragma unroll
+    for (int m = 0; m < kT16; ++m)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = 512 * m + 32 * (lane & 15) + 8 * (lane >> 4) + e;
+        gv[m][e] = k < a.K ? make_float2(ld_row(gr_row, (unsigned)k), ld_row(gi_row, (unsigned)k))
+                           : make_float2(0.0f, 0.0f);
+        mx = fmaxf(mx, fmaxf(fabsf(gv[m][e].x), fabsf(gv[m][e].y)));
+      }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+    int ex = 0;
+    if (mx > 0.0f && isfinite(mx)) (void)frexpf(mx, &ex);
+    const float down = ldexpf(1.0f, -ex);
+    g_up = ldexpf(1.0f, ex);
+#pragma unroll
+    for (int m = 0; m < kT16; ++m) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) gv[m][e] = make_float2(gv[m][e].x * down, gv[m][e].y * down);
+      split8(gv[m], gop[m][0], gop[m][1], gop[m][2], gop[m][3]);
+    }
+  }
   auto horner_batch = [&](int cnt) {
     if constexpr (MT == 0) {
       if (lane < cnt) {
@@ -605,6 +706,8 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
         const float2 S = horner2(gr_row, gi_row, a.K, ph.w, ph.W);
         L.h[L.bsid[lane]] = cmul(ph.base, S).x * inv_n;
       }
+    } else if constexpr (kH16) {
+      h_mma16<kT16>(L.btau, L.bsid, L.h, cnt, lane, gop, g_up, pc, inv_n);
     } else {
       h_mfma<MT>(L.btau, L.bsid, L.h, cnt, lane, ga_r, ga_i, pc, inv_n);
     }

@@ -505,34 +505,6 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mfma_kernel(
 // X and Y are unit phasors times cg; cg is scaled per voxel tile by a power
 // of two (max |cg| in [0.5, 1)) so FP16 neither overflows nor drops small
 // gradients, and the tile's sums are scaled back before the FP64 flush.
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-typedef float f2v __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ void split8_adj(const float2* x, h8& rh, h8& rl, h8& ih, h8& il) {
-#pragma unroll
-  for (int e = 0; e < 8; e += 2) {
-    const h2 r = __builtin_convertvector((f2v){x[e].x, x[e + 1].x}, h2);
-    const h2 i = __builtin_convertvector((f2v){x[e].y, x[e + 1].y}, h2);
-    rh[e] = r[0];
-    rh[e + 1] = r[1];
-    ih[e] = i[0];
-    ih[e + 1] = i[1];
-    rl[e] = (_Float16)(x[e].x - (float)r[0]);
-    rl[e + 1] = (_Float16)(x[e + 1].x - (float)r[1]);
-    il[e] = (_Float16)(x[e].y - (float)i[0]);
-    il[e + 1] = (_Float16)(x[e + 1].y - (float)i[1]);
-  }
-}
-
-__device__ __forceinline__ f32x4 mma3_adj(const h8& ah, const h8& al, const h8& bh,
-                                          const h8& bl, f32x4 c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
-  return c;
-}
-
 template <int MT>
 __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
     const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
@@ -589,9 +561,9 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
     if (cmax > 0.0f && isfinite(cmax)) (void)frexpf(cmax, &e);
     const float down = ldexpf(1.0f, -e);
     const double up = ldexp(1.0, e);
-    f32x4 pr[MT], pi[MT];
+    f32x4v pr[MT], pi[MT];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) pr[m] = pi[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int m = 0; m < MT; ++m) pr[m] = pi[m] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 1
     for (int c = 0; c < 2; ++c) {
       const int v0 = wave * kWave + 32 * c + 8 * g;  // this lane's eight voxels
@@ -605,7 +577,7 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
                     phasor_cycles_hw(frac_multiple((float)(16 * i), c1)));
       }
       h8 yrh, yrl, yih, yil;
-      split8_adj(y, yrh, yrl, yih, yil);
+      split8(y, yrh, yrl, yih, yil);
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         if (m > 0) {
@@ -613,11 +585,11 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
           for (int q = 0; q < 8; ++q) x[q] = cmul(x[q], st.rot[v0 + q]);
         }
         h8 xrh, xrl, xih, xil;
-        split8_adj(x, xrh, xrl, xih, xil);
-        pr[m] = mma3_adj(xrh, xrl, yrh, yrl, pr[m]);
-        pr[m] = mma3_adj(-xih, -xil, yih, yil, pr[m]);
-        pi[m] = mma3_adj(xrh, xrl, yih, yil, pi[m]);
-        pi[m] = mma3_adj(xih, xil, yrh, yrl, pi[m]);
+        split8(x, xrh, xrl, xih, xil);
+        pr[m] = mma3(xrh, xrl, yrh, yrl, pr[m]);
+        pr[m] = mma3(-xih, -xil, yih, yil, pr[m]);
+        pi[m] = mma3(xrh, xrl, yih, yil, pi[m]);
+        pi[m] = mma3(xih, xil, yrh, yrl, pi[m]);
       }
       __builtin_amdgcn_sched_barrier(0);
     }

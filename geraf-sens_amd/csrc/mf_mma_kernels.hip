@@ -48,10 +48,6 @@
 namespace geraf {
 namespace {
 
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMmaTile = 512;    // frequencies per tile: 16 rows of 32
 constexpr int kMmaMaxTiles = 4;  // K <= 2048
@@ -64,20 +60,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A phase step split for exact multiples up to 2^11 in FP32: hi keeps 13
-// significant bits (m * hi has at most 24), lo = c - hi.
-__device__ __forceinline__ SplitCycles split_cycles13(double c) {
-  SplitCycles s;
-  s.hi = __uint_as_float(__float_as_uint((float)c) & 0xFFFFF800u);
-  s.lo = (float)(c - (double)s.hi);
-  return s;
-}
-
-// exp(-j 2 pi x) from the hardware sine and cosine, x reduced to [-0.5, 0.5].
-__device__ __forceinline__ float2 cis_neg(float x) {
-  const float r = x - rintf(x);
-  return make_float2(__builtin_amdgcn_cosf(r), -__builtin_amdgcn_sinf(r));
-}
 
 // One antenna row, tile m: lane i + 16 g holds s[a, 512 m + 32 i + 8 g + e],
 // e = 0..7, scaled, as FP16 hi and lo of the real and imaginary parts.
@@ -118,32 +100,6 @@ __global__ __launch_bounds__(64) void mf_mma_prep_kernel(const float* __restrict
   out[1 * kWave + lane] = rl;
   out[2 * kWave + lane] = ih;
   out[3 * kWave + lane] = il;
-}
-
-// hi = fp16(x) two at a time (one packed conversion), lo = fp16(x - hi) as one
-// mixed-precision FMA per value.
-__device__ __forceinline__ void split8(const float2* y, h8& rh, h8& rl, h8& ih, h8& il) {
-#pragma unroll
-  for (int e = 0; e < 8; e += 2) {
-    const h2 r = __builtin_convertvector((f2v){y[e].x, y[e + 1].x}, h2);
-    const h2 i = __builtin_convertvector((f2v){y[e].y, y[e + 1].y}, h2);
-    rh[e] = r[0];
-    rh[e + 1] = r[1];
-    ih[e] = i[0];
-    ih[e + 1] = i[1];
-    rl[e] = (_Float16)fmaf((float)r[0], -1.0f, y[e].x);
-    rl[e + 1] = (_Float16)fmaf((float)r[1], -1.0f, y[e + 1].x);
-    il[e] = (_Float16)fmaf((float)i[0], -1.0f, y[e].y);
-    il[e + 1] = (_Float16)fmaf((float)i[1], -1.0f, y[e + 1].y);
-  }
-}
-
-__device__ __forceinline__ f32x4 mma3(const h8& ah, const h8& al, const h8& bh, const h8& bl,
-                                      f32x4 c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
-  return c;
 }
 
 template <int MT>
@@ -225,7 +181,7 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
       float2 part = make_float2(0.0f, 0.0f);
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
-        f32x4 tr = {0.0f, 0.0f, 0.0f, 0.0f}, ti = {0.0f, 0.0f, 0.0f, 0.0f};
+        f32x4v tr = {0.0f, 0.0f, 0.0f, 0.0f}, ti = {0.0f, 0.0f, 0.0f, 0.0f};
         tr = mma3(rh[m], rl[m], yrh, yrl, tr);
         tr = mma3(-ih[m], -il[m], yih, yil, tr);
         ti = mma3(rh[m], rl[m], yih, yil, ti);

@@ -134,12 +134,63 @@ __device__ __forceinline__ SplitCycles split_cycles(double c) {
   s.lo = (float)(c - (double)s.hi);
   return s;
 }
+// A phase step split for exact multiples up to 2^11 in FP32: hi keeps 13
+// significant bits (m * hi has at most 24), lo = c - hi.
+__device__ __forceinline__ SplitCycles split_cycles13(double c) {
+  SplitCycles s;
+  s.hi = __uint_as_float(__float_as_uint((float)c) & 0xFFFFF800u);
+  s.lo = (float)(c - (double)s.hi);
+  return s;
+}
 // Centered fraction of m * c for an integer 0 <= m < 256: m * hi, its rint
 // and their difference are exact; only m * lo and the final add round (about
 // 3e-8 cycles, as a cast of the FP64 result to FP32 would).  |result| <= 0.5 + tiny.
 __device__ __forceinline__ float frac_multiple(float m, const SplitCycles& c) {
   const float p = m * c.hi;
   return fmaf(m, c.lo, p - rintf(p));
+}
+
+// ---------------------------------------------- FP16 hi/lo matrix operands
+// Operands of v_mfma_f32_16x16x32_f16: eight FP16 values per lane.  An FP32
+// value x is carried as hi = fp16(x) and lo = fp16(x - hi); a real product
+// is hi*hi + hi*lo + lo*hi into one FP32 accumulator (the dropped lo*lo is
+// ~2^-22 relative), i.e. the products keep FP32 operand precision.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+// hi/lo of the real and imaginary parts of eight complex values (hi two at
+// a time by one packed conversion).
+__device__ __forceinline__ void split8(const float2* y, h8& rh, h8& rl, h8& ih, h8& il) {
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    const h2 r = __builtin_convertvector((f2v){y[e].x, y[e + 1].x}, h2);
+    const h2 i = __builtin_convertvector((f2v){y[e].y, y[e + 1].y}, h2);
+    rh[e] = r[0];
+    rh[e + 1] = r[1];
+    ih[e] = i[0];
+    ih[e + 1] = i[1];
+    rl[e] = (_Float16)(y[e].x - (float)r[0]);
+    rl[e + 1] = (_Float16)(y[e + 1].x - (float)r[1]);
+    il[e] = (_Float16)(y[e].y - (float)i[0]);
+    il[e + 1] = (_Float16)(y[e + 1].y - (float)i[1]);
+  }
+}
+
+// c += A B for split operands (three instructions).
+__device__ __forceinline__ f32x4v mma3(const h8& ah, const h8& al, const h8& bh, const h8& bl,
+                                       f32x4v c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+// exp(-j 2 pi x) from the hardware sine and cosine, x reduced to [-0.5, 0.5].
+__device__ __forceinline__ float2 cis_neg(float x) {
+  const float r = x - rintf(x);
+  return make_float2(__builtin_amdgcn_cosf(r), -__builtin_amdgcn_sinf(r));
 }
 
 // Complex helpers on float2 = (re, im).
