@@ -214,13 +214,13 @@ def test_lensless_config3_subset(cuda):
 
 @pytest.mark.parametrize("R,S,K", [(96, 128, 200), (64, 96, 300), (64, 256, 512),
                                    (48, 160, 1000), (32, 80, 2048), (3, 2048, 129),
-                                   (1, 64, 258), (5, 40, 1025)])
+                                   (1, 64, 258), (5, 40, 1025), (64, 384, 256)])
 def test_lensless_matrix_tiles(cuda, R, S, K):
     """The matrix-core synthesis and its adjoint (K > 128): one to eight
     256-frequency tiles, partial tiles (K = 200, 300, 1000), the first K of
     each tile count (129, 258, 1025), rays whose emitting samples overflow
-    the 64-entry batch (S = 256, and the largest S = 2048), a single ray,
-    ragged rays with none."""
+    the 64-entry batch (S = 256, 384, and the largest S = 2048), a single
+    ray, ragged rays with none."""
     b = ts.rays(R, S, K, seed=K, uneven=(0, S))
     out, grads = _lensless_run(b, cuda)
     ref, rgrads = _lensless_oracle(b, grads["_g"])
