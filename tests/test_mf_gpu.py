@@ -291,14 +291,17 @@ def test_faster_matched_filter_equivalence(cuda):
     torch.testing.assert_close(im2.grad, im.grad[:sel], rtol=1e-5, atol=1e-7)
 
 
-def test_float64_and_half_dispatch(cuda):
+@pytest.mark.parametrize("K", [64, 512])
+def test_float64_and_half_dispatch(cuda, K):
+    """float64 runs the FP64 form; float16 is computed in FP32 (K = 512: the
+    matrix-core form) and returned as float16."""
     from geraf.ops.RFMatchedFilter import matched_filter_cuda
 
-    A, K, N = 4, 64, 100
+    A, N = 4, 100
     tx, rx, vox, _ = _adjoint_case(cuda, A, K, N, seed=7)
     rng = np.random.default_rng(1)
     s = (rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K)))
-    fs, As = 1.25e6, 701500.0
+    fs, As = 1.25e6 * K / 64, 701500.0
     prm = dict(f0=mf_scenes.MF_F0, slope=As, fs=fs, c=mf_scenes.SPEED_OF_LIGHT,
                bias=mf_scenes.PATH_BIAS)
     ref = orc.mf_forward(s.real, s.imag, tx, rx, vox, **prm)
