@@ -160,9 +160,15 @@ typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 
-// hi/lo of the real and imaginary parts of eight complex values (hi two at
-// a time by one packed conversion).
+// hi/lo of the real and imaginary parts of eight complex values: hi two at
+// a time by one packed conversion, lo = x - hi as one mixed-precision FMA
+// (FP16 hi operand, FP32 x) per value, two results per packed conversion.
+// The multiplier -1 is hidden from the compiler (an empty asm that claims to
+// modify it), which would otherwise fold the FMA into a subtraction that
+// first converts hi back to FP32.
 __device__ __forceinline__ void split8(const float2* y, h8& rh, h8& rl, h8& ih, h8& il) {
+  float negone = -1.0f;
+  asm("" : "+v"(negone));
 #pragma unroll
   for (int e = 0; e < 8; e += 2) {
     const h2 r = __builtin_convertvector((f2v){y[e].x, y[e + 1].x}, h2);
@@ -171,10 +177,10 @@ __device__ __forceinline__ void split8(const float2* y, h8& rh, h8& rl, h8& ih, 
     rh[e + 1] = r[1];
     ih[e] = i[0];
     ih[e + 1] = i[1];
-    rl[e] = (_Float16)(y[e].x - (float)r[0]);
-    rl[e + 1] = (_Float16)(y[e + 1].x - (float)r[1]);
-    il[e] = (_Float16)(y[e].y - (float)i[0]);
-    il[e + 1] = (_Float16)(y[e + 1].y - (float)i[1]);
+    rl[e] = (_Float16)fmaf((float)r[0], negone, y[e].x);
+    rl[e + 1] = (_Float16)fmaf((float)r[1], negone, y[e + 1].x);
+    il[e] = (_Float16)fmaf((float)i[0], negone, y[e].y);
+    il[e + 1] = (_Float16)fmaf((float)i[1], negone, y[e + 1].y);
   }
 }
 
