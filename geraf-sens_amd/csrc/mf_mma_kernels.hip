@@ -17,7 +17,7 @@
 // FP32 operands.  A complex product is four real ones (Re = Sr Yr - Si Yi,
 // Im = Sr Yi + Si Yr): 12 instructions per tile and 16 voxels.
 //
-// The echo operands (hi/lo of Re s and Im s, scaled per antenna by a power of
+// The echo operands (hi/lo of Re s, Im s and -Im s, scaled per antenna by a power of
 // two so that max |s| lies in [0.5, 1) and FP16 neither overflows nor loses
 // the small entries) are laid out once in operand order by
 // mf_mma_prep_kernel.  The main kernel gives each wave 64 voxels (four groups
@@ -52,7 +52,7 @@ namespace {
 constexpr int kMmaTile = 512;    // frequencies per tile: 16 rows of 32
 constexpr int kMmaMaxTiles = 4;  // K <= 2048
 constexpr int kMmaWaves = 4;     // waves per block, 64 voxels each
-constexpr int kOpsPerTile = 4;   // Re hi, Re lo, Im hi, Im lo
+constexpr int kOpsPerTile = 6;   // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -62,7 +62,8 @@ __device__ __forceinline__ void wave_sync() {
 
 
 // One antenna row, tile m: lane i + 16 g holds s[a, 512 m + 32 i + 8 g + e],
-// e = 0..7, scaled, as FP16 hi and lo of the real and imaginary parts.
+// e = 0..7, scaled, as FP16 hi and lo of the real and imaginary parts (and of
+// the negated imaginary part, the operand of Re = Sr Yr - Si Yi).
 // One wave per (antenna, tile); the scale (a power of two) is per antenna.
 __global__ __launch_bounds__(64) void mf_mma_prep_kernel(const float* __restrict__ real,
                                                           const float* __restrict__ imag,
@@ -100,6 +101,8 @@ __global__ __launch_bounds__(64) void mf_mma_prep_kernel(const float* __restrict
   out[1 * kWave + lane] = rl;
   out[2 * kWave + lane] = ih;
   out[3 * kWave + lane] = il;
+  out[4 * kWave + lane] = -ih;
+  out[5 * kWave + lane] = -il;
 }
 
 template <int MT>
@@ -125,13 +128,15 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
 
   for (int a = 0; a < A; ++a) {
     const h8* op = ops + (size_t)a * MT * kOpsPerTile * kWave + lane;
-    h8 rh[MT], rl[MT], ih[MT], il[MT];
+    h8 rh[MT], rl[MT], ih[MT], il[MT], nh[MT], nl[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
       rh[m] = op[(m * kOpsPerTile + 0) * kWave];
       rl[m] = op[(m * kOpsPerTile + 1) * kWave];
       ih[m] = op[(m * kOpsPerTile + 2) * kWave];
       il[m] = op[(m * kOpsPerTile + 3) * kWave];
+      nh[m] = op[(m * kOpsPerTile + 4) * kWave];
+      nl[m] = op[(m * kOpsPerTile + 5) * kWave];
     }
     const float up = unscale[a];
     const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
       for (int m = 0; m < MT; ++m) {
         f32x4v tr = {0.0f, 0.0f, 0.0f, 0.0f}, ti = {0.0f, 0.0f, 0.0f, 0.0f};
         tr = mma3(rh[m], rl[m], yrh, yrl, tr);
-        tr = mma3(-ih[m], -il[m], yih, yil, tr);
+        tr = mma3(nh[m], nl[m], yih, yil, tr);
         ti = mma3(rh[m], rl[m], yih, yil, ti);
         ti = mma3(ih[m], il[m], yrh, yrl, ti);
         float2 h = make_float2(tr[3], ti[3]);
