@@ -578,6 +578,7 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
       }
       h8 yrh, yrl, yih, yil;
       split8(y, yrh, yrl, yih, yil);
+      const h8 nyih = -yih, nyil = -yil;  // Re = Xr Yr + Xi (-Yi), for every tile
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         if (m > 0) {
@@ -587,7 +588,7 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
         h8 xrh, xrl, xih, xil;
         split8(x, xrh, xrl, xih, xil);
         pr[m] = mma3(xrh, xrl, yrh, yrl, pr[m]);
-        pr[m] = mma3(-xih, -xil, yih, yil, pr[m]);
+        pr[m] = mma3(xih, xil, nyih, nyil, pr[m]);
         pi[m] = mma3(xrh, xrl, yih, yil, pi[m]);
         pi[m] = mma3(xih, xil, yrh, yrl, pi[m]);
       }
