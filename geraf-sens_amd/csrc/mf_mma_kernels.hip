@@ -28,8 +28,7 @@
 // 8 g .. 8 g + 7 for lane j + 16 g) from one hardware phasor and a rotation
 // tree, splits them, issues the 12 products and contracts its four rows of T
 // with X (one phasor, Horner in the 32-frequency step).  Antennas accumulate per
-// lane with Kahan compensation; the four lanes of a voxel are summed in FP64
-// at the end.
+// lane in FP64; the four lanes of a voxel are summed at the end.
 //
 // Operand layout of the instruction (checked on the GPU with an in-bounds
 // probe): A[i][d] from lane i + 16 (d / 8), element d % 8; B[d][j] from lane
@@ -122,9 +121,11 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
   const int vo = min(v0 + lane, N - 1);
   const double px = voxel[vo], py = voxel[(size_t)N + vo], pz = voxel[2 * (size_t)N + vo];
 
-  KahanC acc[4];
+  // per-lane sums over the antennas in FP64 (one conversion and one FMA per
+  // component and antenna; more accurate than compensated FP32 and cheaper)
+  double acc_r[4], acc_i[4];
 #pragma unroll
-  for (int G = 0; G < 4; ++G) acc[G].init();
+  for (int G = 0; G < 4; ++G) acc_r[G] = acc_i[G] = 0.0;
 
   for (int a = 0; a < A; ++a) {
     const h8* op = ops + (size_t)a * MT * kOpsPerTile * kWave + lane;
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
       nh[m] = op[(m * kOpsPerTile + 4) * kWave];
       nl[m] = op[(m * kOpsPerTile + 5) * kWave];
     }
-    const float up = unscale[a];
+    const double up = (double)unscale[a];
     const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
     const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
     double d;
@@ -202,7 +203,8 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
         part.x += xh.x;
         part.y += xh.y;
       }
-      acc[G].add(make_float2(part.x * up, part.y * up));
+      acc_r[G] = fma((double)part.x, up, acc_r[G]);
+      acc_i[G] = fma((double)part.y, up, acc_i[G]);
       __builtin_amdgcn_sched_barrier(0);
     }
     wave_sync();
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
   const double inv_a = A > 0 ? 1.0 / (double)A : 0.0;
 #pragma unroll
   for (int G = 0; G < 4; ++G) {
-    double re = acc[G].re(), im = acc[G].im();
+    double re = acc_r[G], im = acc_i[G];
     re += __shfl_xor(re, 16, kWave);
     im += __shfl_xor(im, 16, kWave);
     re += __shfl_xor(re, 32, kWave);
