@@ -23,8 +23,8 @@
 // voxels per lane, the tile's coefficients of one antenna are wave-uniform
 // SGPR operands; per pair: FP64 distances and phase reduction, one phasor
 // from the hardware sine and cosine, an N_T-term real Horner with complex
-// coefficients, Kahan accumulation), adding each
-// chunk's sums into FP64 accumulators; a last kernel scales by 1/A.
+// coefficients, FP64 accumulation over the antennas), adding each chunk's
+// sums into FP64 accumulators; a last kernel scales by 1/A.
 //
 // The adjoint (MatchedFilter_cuda_kernel.cu:171-220) is the transpose of the
 // same expansion:
@@ -190,9 +190,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
     py[i] = voxel[(size_t)N + vc];
     pz[i] = voxel[2 * (size_t)N + vc];
   }
-  KahanC acc[VPL];
+  double acc_r[VPL], acc_i[VPL];  // FP64 sums over the antennas
 #pragma unroll
-  for (int i = 0; i < VPL; ++i) acc[i].init();
+  for (int i = 0; i < VPL; ++i) acc_r[i] = acc_i[i] = 0.0;
 
   for (int ai = ai_lo; ai < ai_hi; ++ai) {
     const int a = a0 + ai;
@@ -216,7 +216,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
           Pi = fmaf(Pi, u, cf[n].y);
         }
         const float2 cs = phasor_cycles_hw((float)centered_frac(fcen * tau));
-        acc[i].add(cmul(make_float2(cs.x, -cs.y), make_float2(Pr, Pi)));
+        const float2 t = cmul(make_float2(cs.x, -cs.y), make_float2(Pr, Pi));
+        acc_r[i] += (double)t.x;
+        acc_i[i] += (double)t.y;
       }
     };
     if (monostatic(tx_pos, rx_pos, a))
@@ -227,8 +229,8 @@ __global__ __launch_bounds__(64 * kEvalWaves) void mf_tile_eval_kernel(
 #pragma unroll
   for (int i = 0; i < VPL; ++i) {
     if (valid[i]) {
-      acc_re[v[i]] += acc[i].re();
-      acc_im[v[i]] += acc[i].im();
+      acc_re[v[i]] += acc_r[i];
+      acc_im[v[i]] += acc_i[i];
     }
   }
 }
