@@ -17,9 +17,9 @@
 // FP32 operands.  A complex product is four real ones (Re = Sr Yr - Si Yi,
 // Im = Sr Yi + Si Yr): 12 instructions per tile and 16 voxels.
 //
-// The echo operands (hi/lo of Re s, Im s and -Im s, scaled per antenna by a power of
-// two so that max |s| lies in [0.5, 1) and FP16 neither overflows nor loses
-// the small entries) are laid out once in operand order by
+// The echo operands (hi/lo of Re s, Im s and -Im s, scaled per antenna by a
+// power of two so that max |s| lies in [0.5, 1) and FP16 neither overflows
+// nor loses the small entries) are laid out once in operand order by
 // mf_mma_prep_kernel.  The main kernel gives each wave 64 voxels (four groups
 // of 16).  Per antenna, each lane forms its own voxel's delay in FP64 (FP64
 // distances, as the other forms) and publishes the reduced phase terms to
@@ -27,8 +27,9 @@
 // dfs tau).  Per group, each lane then forms its eight Y phasors (beta =
 // 8 g .. 8 g + 7 for lane j + 16 g) from one hardware phasor and a rotation
 // tree, splits them, issues the 12 products and contracts its four rows of T
-// with X (one phasor, Horner in the 32-frequency step).  Antennas accumulate per
-// lane in FP64; the four lanes of a voxel are summed at the end.
+// with X (one phasor, Horner in the 32-frequency step).  Antennas
+// accumulate per lane in FP64; the four lanes of a voxel are summed at the
+// end.
 //
 // Operand layout of the instruction (checked on the GPU with an in-bounds
 // probe): A[i][d] from lane i + 16 (d / 8), element d % 8; B[d][j] from lane
@@ -37,8 +38,6 @@
 // d = 8 (lane / 16) + element.
 #include <hip/hip_runtime.h>
 
-#include <type_traits>
-
 #include <math.h>
 
 #include "abi_util.hpp"
@@ -46,7 +45,6 @@
 
 namespace geraf {
 namespace {
-
 
 constexpr int kMmaTile = 512;    // frequencies per tile: 16 rows of 32
 constexpr int kMmaMaxTiles = 4;  // K <= 2048
