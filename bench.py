@@ -429,7 +429,11 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
     pair_ms = burst(lambda: (fwd(), bwd()), n)
     samples = R * S
     rate = samples / (pair_ms / 1e3)
-    # algorithmic traffic and work
+    # algorithmic traffic and work: the forward reads 28 B per sample (sdf,
+    # 3-vector gradient, reflectance, dists, z), 24 B of origin and direction
+    # per ray and writes the K complex outputs; the backward reads the same
+    # inputs and the K complex gradients and writes 20 B per sample (sdf,
+    # gradient and reflectance gradients) plus two per-ray partials
     bytes_fwd = samples * 28 + R * (24 + 8 * K) + 8
     bytes_bwd = samples * (28 + 20) + R * (24 + 8 * K + 8) + 8
     flops = 16.0 * K * samples * live  # only emitting samples carry signal work

@@ -1,6 +1,6 @@
 """The reference's matched-filter formula restated in torch-CPU FP32.
 
-TEST / BASELINE INFRASTRUCTURE ONLY: timed by bench.py as the CPU baseline
+TEST AND CPU-BASELINE INFRASTRUCTURE ONLY: timed by bench.py as the CPU baseline
 (the reference itself has no CPU path: its ops assert CUDA,
 matched_filter_cuda.py:247).  Per term it does what
 MatchedFilter_cuda_kernel.cu:87-111 does - FP32 distance, FP32 phase, one
