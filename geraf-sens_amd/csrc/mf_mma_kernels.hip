@@ -182,7 +182,7 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
       split8(y, yrh, yrl, yih, yil);
       // rows alpha = 16 m + 4 g + r of this lane: X[alpha] = X[16 m + 4 g] st^r
       // with st = w32; sum_r X T_r by Horner in st
-      float2 part = make_float2(0.0f, 0.0f);
+      float2 part;
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         f32x4v tr = {0.0f, 0.0f, 0.0f, 0.0f}, ti = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -198,8 +198,12 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
         }
         const float2 x = cis_neg(p.z + frac_multiple((float)(32 * (16 * m + 4 * g)), c));
         const float2 xh = cmul(x, h);
-        part.x += xh.x;
-        part.y += xh.y;
+        if (m == 0) {
+          part = xh;  // not 0 + xh: the compiler may not drop an add of +0
+        } else {
+          part.x += xh.x;
+          part.y += xh.y;
+        }
       }
       acc_r[G] = fma((double)part.x, up, acc_r[G]);
       acc_i[G] = fma((double)part.y, up, acc_i[G]);
