@@ -491,7 +491,7 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         "fwd_ms": f_ms, "bwd_ms": b_ms, "fwd_bwd_ms": pair_ms, "emitting_fraction": live,
         "roofline": {"hbm_gbs": gbs, "hbm_frac": gbs / 8000.0, "fp32_tflops": tfs,
                      "fp32_frac": tfs / VALU_FP32_PEAK_TFLOPS,
-                     "fp32_units": "MFMA for K > 128 (signal, h_s), VALU otherwise; the "
+                     "fp32_units": "MFMA (signal for every K, h_s for K > 128), VALU otherwise; the "
                                    "FP32 peak is 157.3 TFLOP/s on either",
                      "bytes_per_step": bytes_fwd + bytes_bwd, "flops_per_step": flops},
         "errors_vs_fp64": errs,

@@ -31,12 +31,11 @@
 // Forward signal: emitting samples (valid, not culled, amp != 0) are appended
 // to a per-wave LDS queue as (amplitude, reduced carrier phase, per-frequency
 // phase step in FP64 cycles, step phasor).  The queue is drained in batches
-// of 64 (and once at the end).  For K > 128 the drain is a complex matrix
-// product on the matrix cores (k = 16 a + b, see synth_mfma).  For K <= 128
-// lane (chunk j, group g) takes the entries i = g (mod G), seeds
-// exp(j*2*pi*(f0 + 32 j dfs) tau_i) from the FP64-reduced phase and rotates
-// it through its 32 frequencies into 32 private complex accumulators; the G
-// groups are summed with DPP at the end.
+// of 64 (and once at the end).  The drain is a complex matrix product on the
+// matrix cores (k = 16 a + b, see synth_mfma), for every K: at K = 32, 64 and
+// 128 it measured 22, 18 and 25 % faster than a rotation synthesis on the
+// vector ALUs (4,096 rays x 128 samples), although most of its 256-frequency
+// tile is idle there.
 // Backward: pass A walks the tiles forward, keeps the transmittance carry at
 // each tile start, and queues (tau, sample) of every emitting sample; full
 // 64-sample batches evaluate h_s = Re sum_k g[r,k] exp(-j phi_sk), for
@@ -60,7 +59,6 @@ namespace {
 constexpr int kRaysPerBlock = 4;
 constexpr int kTile = 64;
 constexpr int kMaxTiles = 32;  // S <= 2048
-constexpr int kSigChunk = 32;
 constexpr int kQueue = 2 * kTile;
 
 struct LenslessArgs {
@@ -225,61 +223,17 @@ __device__ __forceinline__ int lane_prefix(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-// Sums each of the N values over the lanes of every aligned block of `groups`
-// lanes (every lane of a block gets the block totals).
-template <int N>
-__device__ __forceinline__ void group_sum(float* v, int groups) {
-  if (groups >= 2)
-#pragma unroll
-    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kQuadXor1>(0.0f, v[j]);
-  if (groups >= 4)
-#pragma unroll
-    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kQuadXor2>(0.0f, v[j]);
-  if (groups >= 8)
-#pragma unroll
-    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kRowHalfMirror>(0.0f, v[j]);
-  if (groups >= 16)
-#pragma unroll
-    for (int j = 0; j < N; ++j) v[j] += dpp_mov<dpp::kRowMirror>(0.0f, v[j]);
-  if (groups >= 32)
-#pragma unroll
-    for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], 16, kWave);
-  if (groups >= 64)
-#pragma unroll
-    for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], 32, kWave);
-}
-
 // ------------------------------------------------------------------ forward
 
 // Per-wave queue of emitting samples: amplitude, carrier phase f0 tau reduced
-// to cycles, per-frequency step dfs tau in FP64 cycles, and a phasor: the
-// step exp(j 2 pi dfs tau) for the rotation synthesis, the 256-frequency
-// rotation exp(j 2 pi 256 dfs tau) for the matrix-core synthesis.
+// to cycles, per-frequency step dfs tau in FP64 cycles, and the 256-frequency
+// rotation exp(j 2 pi 256 dfs tau) (used for K > 256).
 struct FwdQueue {
   float amp[kRaysPerBlock][kQueue];
   float fr0[kRaysPerBlock][kQueue];
   float2 w[kRaysPerBlock][kQueue];
   double cyc1[kRaysPerBlock][kQueue];
 };
-
-// Adds entries [0, n) of this wave's queue to the lane's 32 frequencies.
-__device__ __forceinline__ void synth(const FwdQueue& q, int wave, int n, int grp, int groups,
-                                      int k0, float* acc_r, float* acc_i) {
-  for (int i = grp; i < n; i += groups) {
-    const float am = q.amp[wave][i];
-    const float2 w = q.w[wave][i];
-    const double c1 = q.cyc1[wave][i];
-    const float2 seed =
-        phasor_cycles((float)centered_frac(fma((double)k0, c1, (double)q.fr0[wave][i])), 1.0f);
-    float2 zc = make_float2(am * seed.x, am * seed.y);
-#pragma unroll
-    for (int j = 0; j < kSigChunk; ++j) {
-      acc_r[j] += zc.x;
-      acc_i[j] += zc.y;
-      zc = cmul(zc, w);
-    }
-  }
-}
 
 // Matrix-core synthesis.  With k = 16 a + b (b < 16) the ray's signal is a
 // complex product over its emitting samples s,
@@ -328,14 +282,12 @@ __device__ __forceinline__ void synth_mfma(const FwdQueue& q, int wave, int lane
   }
 }
 
-// MT = 0: rotation synthesis on the vector ALUs (lanes = frequency chunk x
-// sample group); MT > 0: matrix-core synthesis with MT tiles of 256
-// frequencies (K <= 256 MT).
+// Matrix-core synthesis with MT tiles of 256 frequencies (K <= 256 MT).
 template <int MT>
 __global__ __launch_bounds__(64 * kRaysPerBlock)
 __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kernel(
     LenslessArgs a, float* __restrict__ out_real, float* __restrict__ out_imag,
-    float* __restrict__ sigma_out, PhaseConsts pc, int groups, int chunks) {
+    float* __restrict__ sigma_out, PhaseConsts pc) {
   __shared__ FwdQueue q;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -343,29 +295,10 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
   if (r >= a.R) return;  // only wave-level synchronisation below
   const Ray y = load_ray(a, r);
 
-  const int grp = lane % groups;
-  const int chunk = lane / groups;
-  const bool chunk_ok = chunk < chunks;
-  const int k0 = chunk * kSigChunk;
-
-  constexpr int kAcc = MT ? 1 : kSigChunk;
-  constexpr int kTiles = MT ? MT : 1;
-  float acc_r[kAcc], acc_i[kAcc];
-  f32x4 dr[kTiles], di[kTiles], ds[kTiles];
-  if constexpr (MT == 0) {
+  f32x4 dr[MT], di[MT], ds[MT];
 #pragma unroll
-    for (int j = 0; j < kSigChunk; ++j) acc_r[j] = acc_i[j] = 0.0f;
-  } else {
-#pragma unroll
-    for (int m = 0; m < MT; ++m) dr[m] = di[m] = ds[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  }
-  auto drain = [&](int n) {
-    if constexpr (MT == 0) {
-      if (chunk_ok) synth(q, wave, n, grp, groups, k0, acc_r, acc_i);
-    } else {
-      synth_mfma<MT>(q, wave, lane, n, dr, di, ds);
-    }
-  };
+  for (int m = 0; m < MT; ++m) dr[m] = di[m] = ds[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  auto drain = [&](int n) { synth_mfma<MT>(q, wave, lane, n, dr, di, ds); };
 
   float carry = 1.0f;
   int queued = 0;
@@ -391,7 +324,6 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
       q.amp[wave][slot] = amp;
       q.fr0[wave][slot] = (float)centered_frac(pc.f0 * tau);
       q.cyc1[wave][slot] = c1;
-      if constexpr (MT == 0) q.w[wave][slot] = phasor_cycles((float)centered_frac(c1), 1.0f);
       if constexpr (MT > 1)
         q.w[wave][slot] = phasor_cycles_hw((float)centered_frac(256.0 * c1));
     }
@@ -407,14 +339,14 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
       if (lane < rest) {
         am = q.amp[wave][ndrain + lane];
         f0 = q.fr0[wave][ndrain + lane];
-        if constexpr (MT != 1) w = q.w[wave][ndrain + lane];
+        if constexpr (MT > 1) w = q.w[wave][ndrain + lane];
         c1 = q.cyc1[wave][ndrain + lane];
       }
       wave_sync();
       if (lane < rest) {
         q.amp[wave][lane] = am;
         q.fr0[wave][lane] = f0;
-        if constexpr (MT != 1) q.w[wave][lane] = w;
+        if constexpr (MT > 1) q.w[wave][lane] = w;
         q.cyc1[wave][lane] = c1;
       }
       queued = rest;
@@ -426,39 +358,24 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
   const float inv_n = y.nv > 0 ? 1.0f / (float)y.nv : 0.0f;
   float* orow = out_real + (size_t)r * a.K;
   float* irow = out_imag + (size_t)r * a.K;
-  if constexpr (MT == 0) {
-    group_sum<kSigChunk>(acc_r, groups);
-    group_sum<kSigChunk>(acc_i, groups);
-    if (chunk_ok && grp == 0) {
+  // (Xr Yr - Xi Yi, (Xr + Xi)(Yr + Yi) - Xr Yr - Xi Yi)
 #pragma unroll
-      for (int j = 0; j < kSigChunk; ++j) {
-        const int k = k0 + j;
-        if (k < a.K) {
-          st_row(orow, (unsigned)k, acc_r[j] * inv_n);
-          st_row(irow, (unsigned)k, acc_i[j] * inv_n);
-        }
-      }
-    }
-  } else {
-    // (Xr Yr - Xi Yi, (Xr + Xi)(Yr + Yi) - Xr Yr - Xi Yi)
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const f32x4 p1 = dr[m], p2 = di[m];
-      dr[m] = p1 - p2;
-      di[m] = ds[m] - p1 - p2;
-    }
-    const int kl = 64 * (lane >> 4) + (lane & 15);
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int k = 256 * m + 16 * v + kl;
-        if (k < a.K) {
-          st_row(orow, (unsigned)k, dr[m][v] * inv_n);
-          st_row(irow, (unsigned)k, di[m][v] * inv_n);
-        }
-      }
+  for (int m = 0; m < MT; ++m) {
+    const f32x4 p1 = dr[m], p2 = di[m];
+    dr[m] = p1 - p2;
+    di[m] = ds[m] - p1 - p2;
   }
+  const int kl = 64 * (lane >> 4) + (lane & 15);
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int k = 256 * m + 16 * v + kl;
+      if (k < a.K) {
+        st_row(orow, (unsigned)k, dr[m][v] * inv_n);
+        st_row(irow, (unsigned)k, di[m][v] * inv_n);
+      }
+    }
   if (sigma_out)
     for (int s = y.nv + lane; s < a.S; s += kWave) st_row(sigma_out + y.row, (unsigned)s, 0.0f);
 }
@@ -896,29 +813,22 @@ extern "C" int geraf_lensless_forward_f32(
   if (R == 0 || K == 0) return GERAF_RF_OK;
   require_ptrs({out_real, out_imag}, true);
   if (S == 0) throw ArgError("need at least one sample per ray");
-  const int chunks = (int)((K + kSigChunk - 1) / kSigChunk);
-  int groups = 1;
-  while (groups * 2 * chunks <= kWave) groups *= 2;
   const unsigned blocks = (unsigned)((R + kRaysPerBlock - 1) / kRaysPerBlock);
   const PhaseConsts pc = make_phase_consts(*params);
   const dim3 grid(blocks), block(64 * kRaysPerBlock);
   hipStream_t st = (hipStream_t)stream;
-  // K <= 128 leaves most of a 16 x 16 matrix tile idle: rotation synthesis
-  if (K <= 128)
-    hipLaunchKernelGGL(lensless_forward_kernel<0>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc, groups, chunks);
-  else if (K <= 256)
+  if (K <= 256)
     hipLaunchKernelGGL(lensless_forward_kernel<1>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc, groups, chunks);
+                       sigma_out, pc);
   else if (K <= 512)
     hipLaunchKernelGGL(lensless_forward_kernel<2>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc, groups, chunks);
+                       sigma_out, pc);
   else if (K <= 1024)
     hipLaunchKernelGGL(lensless_forward_kernel<4>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc, groups, chunks);
+                       sigma_out, pc);
   else
     hipLaunchKernelGGL(lensless_forward_kernel<8>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc, groups, chunks);
+                       sigma_out, pc);
   return check_launch("lensless_forward_kernel");
   GERAF_TRY_END
 }
@@ -949,6 +859,8 @@ extern "C" int geraf_lensless_backward_f32(
   hipLaunchKernelGGL(lensless_backward_kernel<MT_>, grid, block, lds, st, a, grad_out_real,   \
                      grad_out_imag, grad_sdf, grad_grad, grad_refl, grad_inv_s_per_ray,        \
                      grad_power_per_ray, pc)
+  // h_s: two-level Horner up to K = 128, where it measured 5-26 % faster
+  // than the FP16 matrix form
   if (K <= 128)
     GERAF_LL_BWD(0);
   else if (K <= 256)
