@@ -103,10 +103,13 @@ def MatchedFilter_forward(real, imag, tx_pos, rx_pos, output_real, output_imag, 
                      A, K, N, params, _native.stream_of(real))
 
 
-# K range of the matrix-core forward (geraf_mf_forward_f32_mma): from 257 on
-# it beats the Horner kernel (1.4x at K=384, 1.85x at 512, 2.5x at 1024);
-# the instruction handles up to 2048 samples per antenna.
-MMA_MIN_K = 257
+# K range of the matrix-core forward (geraf_mf_forward_f32_mma).  Its time is
+# flat up to K = 512 (one 512-frequency tile), the Horner kernel's grows with
+# K: at 4,096 antennas x 262,144 voxels Horner takes 9.2, 15.2 and 29.2 ms at
+# K = 64, 128 and 256, the matrix-core form 19.0-19.8 ms, so they cross near
+# K = 168 (and at 64x64 x 128^3 the matrix-core form is 2.2x faster at 512).
+# The instruction handles up to 2048 samples per antenna.
+MMA_MIN_K = 161
 MMA_MAX_K = 2048
 
 

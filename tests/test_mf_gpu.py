@@ -178,6 +178,13 @@ def test_mma_forward_cfg1_and_drop_in_dispatch(cuda, cfg1):
     ref = orc.mf_forward(echo.real, echo.imag, tx, rx, vox, **prm)
     assert rel_l2(op, ref) <= TOL_TUNED_VS_F64
     assert rel_l2(op, horner) <= 2e-6 and not np.array_equal(op, horner)
+    # K = 192 (above the op's switch to the matrix-core form at K = 161)
+    echo, tx, rx, vox, prm = _mma_case(5, 192, 900, True, seed=6)
+    o_r, o_i = matched_filter_cuda(t(echo.real), t(echo.imag), t(tx), t(rx), t(vox), 5, 192,
+                                   prm["fs"], prm["slope"], 900)
+    op = o_r.double().cpu().numpy() + 1j * o_i.double().cpu().numpy()
+    assert rel_l2(op, run_mf((echo, tx, rx, vox, prm), cuda, "mma")) == 0.0
+    assert rel_l2(op, orc.mf_forward(echo.real, echo.imag, tx, rx, vox, **prm)) <= TOL_TUNED_VS_F64
 
 
 def test_empty_volume(cuda):
