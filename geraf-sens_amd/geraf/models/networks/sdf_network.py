@@ -213,15 +213,21 @@ class SoftplusBF16(nn.Module):
 
 
 def positional_encoding(x: torch.Tensor, multires: int) -> torch.Tensor:
-    """[x, sin(2^i x), cos(2^i x) for i < multires] along the last dim."""
+    """[x, sin(2^i x), cos(2^i x) for i < multires] along the last dim, the
+    order of the reference's Embedder (sdf_network.py:19-75: include_input,
+    log-sampled bands 2^0 .. 2^(multires-1), sin then cos per band).
+
+    All bands at once: one product with the [multires, 1] band column, one
+    sine, one cosine and one concatenation, instead of two products, two
+    transcendentals and a gradient accumulation per band.  The products by
+    powers of two are exact, so the features equal the per-band form bit for
+    bit."""
     if multires <= 0:
         return x
-    feats = [x]
-    for i in range(multires):
-        f = float(2.0 ** i)
-        feats.append(torch.sin(x * f))
-        feats.append(torch.cos(x * f))
-    return torch.cat(feats, dim=-1)
+    bands = 2.0 ** torch.arange(multires, dtype=x.dtype, device=x.device)
+    xf = x.unsqueeze(-2) * bands.unsqueeze(-1)                     # [..., L, d]
+    enc = torch.stack((torch.sin(xf), torch.cos(xf)), dim=-2)     # [..., L, 2, d]
+    return torch.cat((x, enc.flatten(-3)), dim=-1)
 
 
 class SDFNetwork(nn.Module):
