@@ -280,7 +280,10 @@ class SDFNetwork(nn.Module):
                 nn.init.constant_(lin.bias, 0.0)
                 nn.init.normal_(lin.weight, 0.0, math.sqrt(2) / math.sqrt(out_dim))
 
-    def forward(self, inputs: torch.Tensor) -> torch.Tensor:
+    def forward(self, inputs: torch.Tensor, float_out: bool = True) -> torch.Tensor:
+        """[.., d_out] = (sdf, features); in FP32, or with float_out=False in
+        the compute dtype (a BF16 network then skips the cast of its whole
+        output)."""
         inputs = positional_encoding(inputs * self.scale, self.multires)
         dt = self.compute_dtype if self.compute_dtype not in (None, torch.float32) else None
         shape = inputs.shape[:-1]
@@ -295,7 +298,7 @@ class SDFNetwork(nn.Module):
             x = linear(x, lin, dt) if self.tall_products else lin(x)
             if l < self.num_layers - 2:
                 x = self.activation(x)
-        return x.float().reshape(*shape, -1)
+        return (x.float() if float_out else x).reshape(*shape, -1)
 
     def sdf(self, x: torch.Tensor) -> torch.Tensor:
         return self.forward(x)[..., :1]
@@ -310,12 +313,14 @@ class SDFNetwork(nn.Module):
         return torch.autograd.grad(y, x, torch.ones_like(y), create_graph=True,
                                    retain_graph=True, only_inputs=True)[0]
 
-    def sdf_and_gradient(self, x: torch.Tensor):
-        """(full output [.., d_out], gradient of the SDF) from one forward."""
+    def sdf_and_gradient(self, x: torch.Tensor, float_out: bool = True):
+        """(full output [.., d_out], gradient of the SDF) from one forward.
+        With float_out=False the output stays in the compute dtype; the SDF
+        column is still differentiated in FP32."""
         x.requires_grad_(True)
         with torch.enable_grad():
-            out = self.forward(x)
-            y = out[..., :1]
+            out = self.forward(x, float_out=float_out)
+            y = out[..., :1].float()
         g = torch.autograd.grad(y, x, torch.ones_like(y), create_graph=True, retain_graph=True,
                                 only_inputs=True)[0]
         return out, g
@@ -362,8 +367,9 @@ class ReflectivePowerNetwork(nn.Module):
         self.tall_products = tall_products
 
     def reflectance(self, points: torch.Tensor, features: torch.Tensor) -> torch.Tensor:
-        x = torch.cat([features, points], dim=-1)
         dt = self.compute_dtype if self.compute_dtype not in (None, torch.float32) else None
+        # features may already be in the compute dtype (no cast of them then)
+        x = torch.cat([features, points.to(features.dtype)], dim=-1)
         shape = x.shape[:-1]
         x = x.reshape(-1, x.shape[-1])
         if dt is not None:

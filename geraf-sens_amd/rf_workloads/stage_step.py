@@ -200,8 +200,10 @@ class StageStep:
     def loss(self) -> torch.Tensor:
         cfg, b = self.cfg, self.b
         R, S = cfg.rays, cfg.samples
-        out, grad = self.sdf_net.sdf_and_gradient(self.pts_n.clone())
-        sdf = out[:, 0].reshape(R, S)
+        # the features stay in the networks' compute dtype (BF16 in the fused
+        # form): the reflectance network takes them as they are
+        out, grad = self.sdf_net.sdf_and_gradient(self.pts_n.clone(), float_out=False)
+        sdf = out[:, 0].float().reshape(R, S)
         feat = out[:, 1:]
         grad = grad.reshape(R, S, 3)
         inv_s = self.var_net.inv_s().clip(1e-6, 1e6)
