@@ -174,10 +174,14 @@ def _lensless_oracle(b, g=None):
     return out.detach().numpy(), grads
 
 
-@pytest.mark.parametrize("case", ["plain", "offset_uneven", "anneal_half_K96"])
+@pytest.mark.parametrize("case", ["plain", "offset_uneven", "anneal_half_K96", "K1", "K17"])
 def test_lensless_vs_oracle(cuda, case):
     if case == "plain":
         b = ts.rays(256, 128, 256, seed=3)
+    elif case in ("K1", "K17"):
+        # one and a few frequencies: most of the synthesis tile and the h_s
+        # gradient row are padding
+        b = ts.rays(96, 80, int(case[1:]), seed=11, uneven=(8, 80))
     elif case == "offset_uneven":
         b = ts.rays(200, 96, 64, seed=5, with_offset=True, uneven=(16, 96))
     else:
