@@ -40,6 +40,9 @@
 
 #include <math.h>
 
+#include <mutex>
+#include <string>
+
 #include "abi_util.hpp"
 #include "rf_common.hpp"
 
@@ -246,25 +249,24 @@ extern "C" int64_t geraf_mf_forward_mma_workspace_bytes(int64_t A, int64_t K) {
   return (int64_t)(align256(mma_ops_bytes(A, K)) + align256((size_t)A * sizeof(float)));
 }
 
-extern "C" int geraf_mf_forward_f32_mma(const float* real, const float* imag,
-                                        const float* tx_pos, const float* rx_pos,
-                                        float* out_real, float* out_imag, const float* voxel,
-                                        int64_t A, int64_t K, int64_t N,
-                                        const geraf_fmcw_params* params, void* workspace,
-                                        int64_t workspace_bytes, geraf_stream_t stream) {
-  GERAF_TRY_BEGIN
-  check_mf_sizes(A, K, N);
-  check_params(params);
-  if (K > (int64_t)kMmaTile * kMmaMaxTiles) throw ArgError("K must be <= 2048 for this form");
-  if (N == 0) return GERAF_RF_OK;
-  require_ptrs({out_real, out_imag, voxel}, true);
+namespace geraf {
+namespace {
+
+// Enqueues the prep and main kernels with `workspace` as the echo's operand
+// buffer (at least geraf_mf_forward_mma_workspace_bytes(A, K), 256-byte
+// aligned).  Shared by the caller-workspace and the static-scratch entry
+// points.
+int mma_forward(const float* real, const float* imag, const float* tx_pos, const float* rx_pos,
+                float* out_real, float* out_imag, const float* voxel, int64_t A, int64_t K,
+                int64_t N, const geraf_fmcw_params* params, void* workspace,
+                int64_t workspace_bytes, hipStream_t stream) {
   const unsigned blocks = (unsigned)((N + 64 * kMmaWaves - 1) / (64 * kMmaWaves));
   const PhaseConsts pc = make_phase_consts(*params);
   if (A == 0 || K == 0) {
     // no antennas or no samples: the sum is empty; the kernel writes zeros
     hipLaunchKernelGGL((mf_forward_mma_kernel<1>), dim3(blocks), dim3(64 * kMmaWaves), 0,
-                       (hipStream_t)stream, nullptr, nullptr, tx_pos, rx_pos, out_real, out_imag,
-                       voxel, 0, (int)N, pc);
+                       stream, nullptr, nullptr, tx_pos, rx_pos, out_real, out_imag, voxel, 0,
+                       (int)N, pc);
     return check_launch("mf_forward_mma_kernel");
   }
   require_ptrs({real, imag, tx_pos, rx_pos, workspace}, true);
@@ -276,14 +278,13 @@ extern "C" int geraf_mf_forward_f32_mma(const float* real, const float* imag,
   h8* ops = reinterpret_cast<h8*>(workspace);
   float* unscale =
       reinterpret_cast<float*>(reinterpret_cast<char*>(workspace) + align256(mma_ops_bytes(A, K)));
-  hipLaunchKernelGGL(mf_mma_prep_kernel, dim3((unsigned)A, (unsigned)MT), dim3(kWave), 0,
-                     (hipStream_t)stream, real, imag, (int)K, MT, ops, unscale);
+  hipLaunchKernelGGL(mf_mma_prep_kernel, dim3((unsigned)A, (unsigned)MT), dim3(kWave), 0, stream,
+                     real, imag, (int)K, MT, ops, unscale);
   const int rc = check_launch("mf_mma_prep_kernel");
   if (rc != GERAF_RF_OK) return rc;
-#define GERAF_MMA_LAUNCH(T)                                                                \
-  hipLaunchKernelGGL((mf_forward_mma_kernel<T>), dim3(blocks), dim3(64 * kMmaWaves), 0,   \
-                     (hipStream_t)stream, ops, unscale, tx_pos, rx_pos, out_real, out_imag, \
-                     voxel, (int)A, (int)N, pc)
+#define GERAF_MMA_LAUNCH(T)                                                                   \
+  hipLaunchKernelGGL((mf_forward_mma_kernel<T>), dim3(blocks), dim3(64 * kMmaWaves), 0, stream, \
+                     ops, unscale, tx_pos, rx_pos, out_real, out_imag, voxel, (int)A, (int)N, pc)
   switch (MT) {
     case 1: GERAF_MMA_LAUNCH(1); break;
     case 2: GERAF_MMA_LAUNCH(2); break;
@@ -292,5 +293,84 @@ extern "C" int geraf_mf_forward_f32_mma(const float* real, const float* imag,
   }
 #undef GERAF_MMA_LAUNCH
   return check_launch("mf_forward_mma_kernel");
+}
+
+void check_mma_args(int64_t A, int64_t K, int64_t N, const geraf_fmcw_params* params) {
+  check_mf_sizes(A, K, N);
+  check_params(params);
+  if (K > (int64_t)kMmaTile * kMmaMaxTiles) throw ArgError("K must be <= 2048 for this form");
+}
+
+}  // namespace
+
+// Device-resident scratch of geraf_mf_forward_f32_mma_noalloc: the echo in
+// operand order, as the caller's workspace holds it in
+// geraf_mf_forward_f32_mma.  Each GPU that loads the library holds one copy
+// (GERAF_MF_MMA_SCRATCH_BYTES) until the process ends.  In host code the
+// array's name is not a device address: the copy of the current device is
+// found once with hipGetSymbolAddress and only that pointer is used.
+static __device__ __align__(256) unsigned char scratch[GERAF_MF_MMA_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of the current device's copy of `scratch`, looked up on the
+// first call on that device (with that device current) and kept.
+void* scratch_of_current_device() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) throw HipError(std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) throw ArgError("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      throw HipError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+}  // namespace geraf
+
+extern "C" int geraf_mf_forward_f32_mma(const float* real, const float* imag,
+                                        const float* tx_pos, const float* rx_pos,
+                                        float* out_real, float* out_imag, const float* voxel,
+                                        int64_t A, int64_t K, int64_t N,
+                                        const geraf_fmcw_params* params, void* workspace,
+                                        int64_t workspace_bytes, geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_mma_args(A, K, N, params);
+  if (N == 0) return GERAF_RF_OK;
+  require_ptrs({out_real, out_imag, voxel}, true);
+  return mma_forward(real, imag, tx_pos, rx_pos, out_real, out_imag, voxel, A, K, N, params,
+                     workspace, workspace_bytes, (hipStream_t)stream);
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_mf_forward_f32_mma_noalloc(const float* real, const float* imag,
+                                                const float* tx_pos, const float* rx_pos,
+                                                float* out_real, float* out_imag,
+                                                const float* voxel, int64_t A, int64_t K,
+                                                int64_t N, const geraf_fmcw_params* params,
+                                                geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  check_mma_args(A, K, N, params);
+  const int64_t need = geraf_mf_forward_mma_workspace_bytes(A, K);
+  if (need > (int64_t)GERAF_MF_MMA_SCRATCH_BYTES)
+    throw ArgError("A * ceil(K / 512) too large for the static scratch: the echo needs " +
+                   std::to_string(need) + " bytes, the scratch holds " +
+                   std::to_string((int64_t)GERAF_MF_MMA_SCRATCH_BYTES) +
+                   " (use geraf_mf_forward_f32_mma with a workspace)");
+  if (N == 0) return GERAF_RF_OK;
+  require_ptrs({out_real, out_imag, voxel}, true);
+  void* ws = (A == 0 || K == 0) ? nullptr : scratch_of_current_device();
+  return mma_forward(real, imag, tx_pos, rx_pos, out_real, out_imag, voxel, A, K, N, params, ws,
+                     (int64_t)GERAF_MF_MMA_SCRATCH_BYTES, (hipStream_t)stream);
   GERAF_TRY_END
 }

@@ -50,6 +50,7 @@ _PARAMS = ctypes.POINTER(FmcwParams)
 _SIGNATURES = {
     "geraf_mf_forward_f32": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_forward_f32_mma": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P, _I64, _P],
+    "geraf_mf_forward_f32_mma_noalloc": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_forward_plain_f32": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_forward_f64": [_P] * 7 + [_I64] * 3 + [_PARAMS, _P],
     "geraf_mf_backward_f32": [_P] * 7 + [_I64] * 4 + [_PARAMS, _P],

@@ -85,6 +85,25 @@ int geraf_mf_forward_f32_mma(const float* real, const float* imag, const float* 
                              const geraf_fmcw_params* params, void* workspace,
                              int64_t workspace_bytes, geraf_stream_t stream);
 
+/* Size of the device-resident scratch of geraf_mf_forward_f32_mma_noalloc
+ * (32 MiB per GPU that loads the library, held until the process ends). */
+#define GERAF_MF_MMA_SCRATCH_BYTES (32u << 20)
+
+/* geraf_mf_forward_f32_mma without a workspace argument and without any
+ * allocation: the echo's operand buffer is a static device array of
+ * GERAF_MF_MMA_SCRATCH_BYTES, so geraf_mf_forward_mma_workspace_bytes(A, K)
+ * must not exceed that (A * ceil(K / 512) <= 5,457, e.g. 4,096 antennas at
+ * K <= 512); larger sizes return GERAF_RF_EINVAL with a message.  Same
+ * kernels, same results.  The scratch is shared by every call on one GPU:
+ * calls that overlap on one GPU (different streams or host threads) race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int geraf_mf_forward_f32_mma_noalloc(const float* real, const float* imag, const float* tx_pos,
+                                     const float* rx_pos, float* out_real, float* out_imag,
+                                     const float* voxel, int64_t A, int64_t K, int64_t N,
+                                     const geraf_fmcw_params* params, geraf_stream_t stream);
+
 /* Same operator, plain FP32 form: one thread per voxel, per-term sincosf in
  * FP32 exactly as MatchedFilter_cuda_kernel.cu:87-111 computes it. Kept as the
  * reference-formula baseline; not the production path. */
