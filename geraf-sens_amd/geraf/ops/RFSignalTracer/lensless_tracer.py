@@ -46,44 +46,55 @@ def _scalar(t, device) -> torch.Tensor:
     return _c(t.reshape(1))
 
 
+def _kernel_inputs(sdf, gradients, reflectance, inv_s, power, dists, z_vals, ray_o, ray_d,
+                   trans_offset, n_valid):
+    """The kernels' operands: detached, float32 (int32 counts), contiguous."""
+    dev = sdf.device
+    return dict(sdf=_c(sdf), grad=_c(gradients), refl=_c(reflectance), dists=_c(dists),
+                z=_c(z_vals), o=_c(ray_o), d=_c(ray_d), off=_c(trans_offset),
+                nv=_c(n_valid, torch.int32), inv_s=_scalar(inv_s, dev),
+                power=_scalar(power, dev))
+
+
 class _LenslessTracerFunction(Function):
     @staticmethod
     def forward(ctx, sdf, gradients, reflectance, inv_s, power, dists, z_vals, ray_o, ray_d,
                 trans_offset, n_valid, K: int, adcSampleRate: float, fc_start: float,
                 As_sci: float, cos_anneal_ratio: float):
         R, S = sdf.shape
-        dev = sdf.device
-        ins = dict(sdf=_c(sdf), grad=_c(gradients), refl=_c(reflectance), dists=_c(dists),
-                   z=_c(z_vals), o=_c(ray_o), d=_c(ray_d), off=_c(trans_offset),
-                   nv=_c(n_valid, torch.int32), inv_s=_scalar(inv_s, dev),
-                   power=_scalar(power, dev))
-        out_r = torch.empty((R, K), dtype=torch.float32, device=dev)
+        ins = _kernel_inputs(sdf, gradients, reflectance, inv_s, power, dists, z_vals, ray_o,
+                             ray_d, trans_offset, n_valid)
+        out_r = torch.empty((R, K), dtype=torch.float32, device=sdf.device)
         out_i = torch.empty_like(out_r)
         prm = _C.tracer_params(fc_start, As_sci, adcSampleRate)
-        with torch.cuda.device(dev):
+        with torch.cuda.device(sdf.device):
             _native.call("geraf_lensless_forward_f32",
                          *map(_native.ptr, (ins["sdf"], ins["grad"], ins["refl"], ins["dists"],
                                             ins["z"], ins["o"], ins["d"], ins["off"], ins["nv"],
                                             ins["inv_s"], ins["power"], out_r, out_i, None)),
                          R, S, K, float(cos_anneal_ratio), prm, _native.stream_of(out_r))
-        ctx.ins = ins
+        # the inputs themselves are saved (autograd then refuses a backward
+        # after any of them was modified in place); the backward converts
+        # them again as the forward did
+        ctx.save_for_backward(sdf, gradients, reflectance, inv_s, power, dists, z_vals, ray_o,
+                              ray_d, trans_offset, n_valid)
         ctx.cfg = (R, S, K, float(cos_anneal_ratio), prm)
-        ctx.inv_s_shape = inv_s.shape if torch.is_tensor(inv_s) else ()
-        ctx.power_shape = power.shape if torch.is_tensor(power) else ()
-        ctx.has_refl = reflectance is not None
         return out_r, out_i
 
     @staticmethod
     @once_differentiable
     def backward(ctx, grad_real, grad_imag):
-        ins = ctx.ins
+        saved = ctx.saved_tensors
+        ins = _kernel_inputs(*saved)
         R, S, K, anneal, prm = ctx.cfg
+        inv_s, power, reflectance = saved[3], saved[4], saved[2]
         dev = ins["sdf"].device
         g_r = _c(grad_real)
         g_i = _c(grad_imag)
         g_sdf = torch.empty((R, S), dtype=torch.float32, device=dev)
         g_grad = torch.empty((R, S, 3), dtype=torch.float32, device=dev)
-        g_refl = torch.empty((R, S), dtype=torch.float32, device=dev) if ctx.has_refl else None
+        g_refl = (torch.empty((R, S), dtype=torch.float32, device=dev)
+                  if reflectance is not None else None)
         g_inv = torch.empty(R, dtype=torch.float32, device=dev)
         g_pow = torch.empty(R, dtype=torch.float32, device=dev)
         with torch.cuda.device(dev):
@@ -94,8 +105,8 @@ class _LenslessTracerFunction(Function):
                                             g_sdf, g_grad, g_refl, g_inv, g_pow)),
                          R, S, K, anneal, prm, _native.stream_of(g_sdf))
         need = ctx.needs_input_grad
-        g_inv_s = g_inv.sum().reshape(ctx.inv_s_shape) if need[3] else None
-        g_power = g_pow.sum().reshape(ctx.power_shape) if need[4] else None
+        g_inv_s = g_inv.sum().reshape(inv_s.shape) if need[3] else None
+        g_power = g_pow.sum().reshape(power.shape) if need[4] else None
         return (g_sdf if need[0] else None, g_grad if need[1] else None,
                 g_refl if (need[2] and g_refl is not None) else None, g_inv_s, g_power) + \
             (None,) * 11

@@ -39,8 +39,9 @@ def test_fused_step_matches_fp32_autograd_step(cuda):
     print(f"loss fused {lf:.6f} autograd {la:.6f}", flush=True)
     assert np.isfinite(lf) and abs(lf - la) <= 1e-3 * max(1.0, abs(la))
     # Both forms are FP32; they differ in summation order and in the
-    # tracer's phase evaluation (FP64-reduced vs torch FP32 sin/cos), so
-    # gradients agree to ~1e-3 relative per parameter tensor.
+    # tracer's phase evaluation (FP64-reduced vs torch FP32 sin/cos).  The
+    # gradients agree to ~1e-5 per parameter tensor (measured); an adjoint
+    # error of order 1e-3 fails here.
     worst = 0.0
     for x, y in zip(gf, ga):
         den = float(y.norm())
@@ -49,7 +50,7 @@ def test_fused_step_matches_fp32_autograd_step(cuda):
             continue
         worst = max(worst, float((x - y).norm()) / den)
     print(f"worst per-tensor gradient rel-L2 {worst:.3e}", flush=True)
-    assert worst <= 2e-2
+    assert worst <= 1e-3
 
 
 def test_merged_antennas_give_the_per_ray_prediction(cuda):
@@ -74,8 +75,8 @@ def test_merged_antennas_give_the_per_ray_prediction(cuda):
                               st.merge_rays(sim_i, f.ant_idx, f.ant_of_ray) * f.ant_scale,
                               f.ant_pos, f.ant_pos,
                               f.tgt, U, K, r["adcSampleRate"], r["As_sci"], N)
-    ref = torch.complex(*ref).double()
-    got = torch.complex(*got).double()
+    ref = torch.complex(*ref).to(torch.complex128)
+    got = torch.complex(*got).to(torch.complex128)
     err = float((got - ref).norm() / ref.norm())
     print(f"merged {U} antennas vs {R} rays: rel-L2 {err:.2e}", flush=True)
     assert err <= 1e-5
