@@ -72,6 +72,8 @@ _SIGNATURES = {
     "geraf_softplus_bf16": [_P] * 2 + [_I64, ctypes.c_float, ctypes.c_float, _P],
     "geraf_softplus_grad_bf16": [_P] * 3 + [_I64, ctypes.c_float, ctypes.c_float, _P],
     "geraf_softplus_grad2_bf16": [_P] * 5 + [_I64, ctypes.c_float, ctypes.c_float, _P],
+    "geraf_intersect_sphere_f32": [_P] * 4 + [_I64, ctypes.c_float, _P],
+    "geraf_calibration_offset_f32": [_P] * 5 + [_I64] * 2 + [_P],
     "geraf_mf_forward_tiled_f32": [_P] * 8 + [_I64] * 3 + [ctypes.c_int32] * 2 +
                                   [_P, _I64, _PARAMS, _P],
     "geraf_mf_backward_tiled_f32": [_P] * 8 + [_I64] * 4 + [ctypes.c_int32] * 2 +

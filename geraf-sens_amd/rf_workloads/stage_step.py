@@ -7,13 +7,18 @@ One step (GeRaFStage1.loss, rf_rendering.py:721-820, in its per-ray form):
      SDF gradient with create_graph (the normals, differentiated again);
   2. inv_s from the variance network; reflectance from the 4 x 256 signal MLP
      on [features, points]; power = exp(light_power);
-  3. the fused lensless tracer renders each ray's FMCW signal (its antenna is
+  3. the transmittance calibration (rf_rendering.py:611-626,749-750): rays
+     from the array centre through every sample hit the unit sphere; the SDF
+     there and at each ray's first sample give the offset new_cdf - prev_cdf;
+  4. the fused lensless tracer renders each ray's FMCW signal (its antenna is
      the ray origin);
-  4. the matched filter backprojects all rays' signals onto the valid sample
+  5. the matched filter backprojects all rays' signals onto the valid sample
      points (rays from one aperture position are one antenna, whose echo is
-     the sum of their signals); L1 between |MF| and the target volume sampled
-     there (trilinear), plus the eikonal term (rf_rendering.py:645-650);
-  5. backward (matched-filter adjoint, tracer adjoint, MLP double-backward)
+     the sum of their signals); the prediction |MF| against the target volume
+     sampled there (trilinear, divided by trans_power as the dataset loads it,
+     then by sub_adc), the Charbonnier loss (rf_rendering.py:686-716,814-817),
+     plus the eikonal term (rf_rendering.py:645-650);
+  6. backward (matched-filter adjoint, tracer adjoint, MLP double-backward)
      and an AdamW step.
 
 ``form="fused"`` is the MI355X path (BF16 MLPs on MFMA with split-K weight
@@ -34,6 +39,8 @@ import torch.nn.functional as F
 
 from geraf.models.networks.sdf_network import (ReflectivePowerNetwork, SDFNetwork,
                                                 SingleVarianceNetwork)
+from geraf.models.rendering.rf_stage_ops import (compute_mf_loss, matched_filter_prediction,
+                                                 transmission_offset)
 from geraf.ops.RFMatchedFilter import matched_filter_cuda
 from geraf.ops.RFMatchedFilter.tiled_matched_filter import make_plan, matched_filter_tiled
 from geraf.ops.RFSignalTracer.lensless_tracer import lensless_signal_tracer
@@ -43,11 +50,16 @@ from .mf_scenes import PATH_BIAS, SPEED_OF_LIGHT
 
 CENTER = (0.0, 0.0, 0.30)
 RADIUS = 0.3
+TRANS_POWER = 0.0158489319  # radar config trans_power: MF volumes are divided by it
+LOSS_MODE = "charbonier"    # stage configs: loss_mode, loss_scale
+LOSS_SCALE = 1.0
 
 
 def torch_lensless(sdf, grad, dists, z, o, d, inv_s, power, refl, n_valid, K, fs, As_sci, f0,
-                   anneal=1.0):
-    """The per-ray compositing + signal as plain torch ops (FP32, autograd)."""
+                   anneal=1.0, offset=None):
+    """The per-ray compositing + signal as plain torch ops (FP32, autograd);
+    ``offset`` is the calibration new_cdf - prev_cdf added to the
+    transmittance (rf_rendering.py:750)."""
     R, S = sdf.shape
     true_cos = (d[:, None, :] * grad).sum(-1)
     iter_cos = -(F.relu(-true_cos * 0.5 + 0.5) * (1.0 - anneal) + F.relu(-true_cos) * anneal)
@@ -58,6 +70,8 @@ def torch_lensless(sdf, grad, dists, z, o, d, inv_s, power, refl, n_valid, K, fs
     alpha = ((pc - nc + 1e-5) / (pc + 1e-5)).clip(0.0, 1.0)
     trans = torch.cumprod(torch.cat([torch.ones_like(alpha[:, :1]), 1.0 - alpha + 1e-7], -1),
                           -1)[:, :-1]
+    if offset is not None:
+        trans = trans + offset
     sigma = torch.clamp(refl * power * trans, min=1e-6) * (alpha * trans)
     n = F.normalize(grad, dim=-1)
     pts = o[:, None, :] + z[..., None] * d[:, None, :]
@@ -177,13 +191,15 @@ class StageStep:
         g = (self.tgt - lo) / (hi - lo) * 2.0 - 1.0
         vol = target_volume.abs().float()[None, None]                          # [1,1,X,Y,Z]
         samp = F.grid_sample(vol, g[:, [2, 1, 0]].view(1, -1, 1, 1, 3), align_corners=True)
-        self.gt = samp.view(-1)
-        self.gt = self.gt / self.gt.max().clamp(min=1e-12)
+        # the dataset divides the volume by trans_power (loading.py:51-58) and
+        # samples it at the targets; the prediction divides by sub_adc
+        self.gt = samp.view(-1) / TRANS_POWER
         if cfg.consistency_points:
             gen = torch.Generator(device="cpu").manual_seed(seed + 1)
             self.cons_pts = (torch.rand(cfg.consistency_points, 3, generator=gen) * 2 - 1) \
                 .to(device) * 0.6
-        self.radar = b.radar
+        # sub_adc: the K samples are every (512 / K)-th of the 512-sample chirp
+        self.radar = dict(b.radar, sub_adc=max(1, 512 // cfg.K))
         if mf_algo == "auto":
             mf_algo = "tiled" if form == "fused" else "direct"
         if mf_algo not in ("direct", "tiled"):
@@ -196,10 +212,25 @@ class StageStep:
         # normalisation of the per-ray form, so the prediction is unchanged.
         self.ant_pos, self.ant_idx, self.ant_of_ray = antenna_groups(b.o)
         self.ant_scale = self.ant_pos.shape[0] / cfg.rays
+        self.ant_pos_norm = ((self.ant_pos - center) / RADIUS).contiguous()
 
-    def loss(self) -> torch.Tensor:
+    def calibration_offset(self) -> torch.Tensor:
+        """trans_offset [R,S] = new_cdf - prev_cdf from the sphere hits of rays
+        leaving the centre of the (normalised) antennas (no gradient)."""
+        R, S = self.cfg.rays, self.cfg.samples
+        inv_s = self.var_net.inv_s().clip(1e-6, 1e6)
+        return transmission_offset(lambda x: self.sdf_net.sdf(x)[..., 0], inv_s,
+                                   self.pts_n.view(R, S, 3), self.ant_pos_norm)
+
+    def parts(self) -> dict:
+        """The step's forward, returning its intermediates: network outputs
+        (sdf, grad, refl, inv_s, power), the calibration offset, the per-ray
+        signals (sim_r, sim_i), the merged antenna echoes (ant_r, ant_i), the
+        matched filter at the targets (mf_r, mf_i), pred and gt, the loss
+        terms and the total ``loss``."""
         cfg, b = self.cfg, self.b
         R, S = cfg.rays, cfg.samples
+        offset = self.calibration_offset()
         # the features stay in the networks' compute dtype (BF16 in the fused
         # form): the reflectance network takes them as they are
         out, grad = self.sdf_net.sdf_and_gradient(self.pts_n.clone(), float_out=False)
@@ -212,35 +243,53 @@ class StageStep:
         r = self.radar
         if self.form == "fused":
             sim_r, sim_i = lensless_signal_tracer(sdf, grad, b.dists, b.z, b.o, b.d, inv_s, power,
-                                                  reflectance=refl, n_valid=b.n_valid,
-                                                  **r)
+                                                  reflectance=refl, trans_offset=offset,
+                                                  n_valid=b.n_valid, numADCSample=r["numADCSample"],
+                                                  adcSampleRate=r["adcSampleRate"],
+                                                  fc_start=r["fc_start"], As_sci=r["As_sci"])
         else:
             sim_r, sim_i = torch_lensless(sdf, grad, b.dists, b.z, b.o, b.d, inv_s, power, refl,
                                           b.n_valid.float(), r["numADCSample"],
-                                          r["adcSampleRate"], r["As_sci"], r["fc_start"])
+                                          r["adcSampleRate"], r["As_sci"], r["fc_start"],
+                                          offset=offset)
         U = self.ant_pos.shape[0]
         ant_r = merge_rays(sim_r, self.ant_idx, self.ant_of_ray) * self.ant_scale
         ant_i = merge_rays(sim_i, self.ant_idx, self.ant_of_ray) * self.ant_scale
-        if self.mf_algo == "tiled":
-            mf_r, mf_i = matched_filter_tiled(ant_r, ant_i, self.ant_pos, self.ant_pos, self.tgt,
-                                              U, r["numADCSample"], r["adcSampleRate"],
-                                              r["As_sci"], self.tgt.shape[0], plan=self.plan)
-        else:
-            mf_r, mf_i = matched_filter_cuda(ant_r, ant_i, self.ant_pos, self.ant_pos, self.tgt, U,
-                                             r["numADCSample"], r["adcSampleRate"], r["As_sci"],
-                                             self.tgt.shape[0])
-        pred = torch.hypot(mf_r, mf_i)
-        pred = pred / pred.detach().max().clamp(min=1e-12)
-        loss = (pred - self.gt).abs().mean()
+        keep = {}
+
+        def mf(re, im, tp, rp, pts):
+            if self.mf_algo == "tiled":
+                o = matched_filter_tiled(re, im, tp, rp, pts, U, r["numADCSample"],
+                                         r["adcSampleRate"], r["As_sci"], pts.shape[0],
+                                         plan=self.plan)
+            else:
+                o = matched_filter_cuda(re, im, tp, rp, pts, U, r["numADCSample"],
+                                        r["adcSampleRate"], r["As_sci"], pts.shape[0])
+            keep["mf"] = o
+            return o
+
+        # the targets are the valid samples already (the fused tracer's
+        # inbounds), so no mask gather
+        pred, gt = matched_filter_prediction(ant_r, ant_i, self.ant_pos, self.ant_pos, self.tgt,
+                                             self.gt, None, r, matched_filter=mf)
+        mf_loss = compute_mf_loss(pred, gt * LOSS_SCALE, LOSS_MODE).mean()
         eik = ((grad.norm(dim=-1) - 1.0) ** 2)[self.valid].mean()
-        loss = loss + 0.1 * eik
+        loss = mf_loss + 0.1 * eik
+        cons = None
         if self.cfg.consistency_points:
             # vision-SDF consistency vs an analytic box SDF (half side 0.16/0.3/2)
             h = 0.08 / RADIUS
             q = self.cons_pts.abs() - h
             box = q.clamp(min=0).norm(dim=-1) + q.max(dim=-1).values.clamp(max=0)
-            loss = loss + (self.sdf_net.sdf(self.cons_pts)[:, 0] - box).abs().mean()
-        return loss
+            cons = (self.sdf_net.sdf(self.cons_pts)[:, 0] - box).abs().mean()
+            loss = loss + cons
+        return dict(sdf=sdf, grad=grad, refl=refl, inv_s=inv_s, power=power, offset=offset,
+                    sim_r=sim_r, sim_i=sim_i, ant_r=ant_r, ant_i=ant_i, mf_r=keep["mf"][0],
+                    mf_i=keep["mf"][1], pred=pred, gt=gt, mf_loss=mf_loss, eikonal=eik,
+                    consistency=cons, loss=loss)
+
+    def loss(self) -> torch.Tensor:
+        return self.parts()["loss"]
 
     def step(self) -> float:
         self.opt.zero_grad(set_to_none=True)

@@ -279,6 +279,27 @@ int geraf_lensless_backward_f32(const float* grad_out_real, const float* grad_ou
                                 int64_t S, int64_t K, float cos_anneal_ratio,
                                 const geraf_fmcw_params* params, geraf_stream_t stream);
 
+/* Sphere hits of the transmittance calibration (GeRaFStage1.intersect_sphere,
+ * rf_rendering.py:455-495): for each of M points p [M,3] (SDF frame) and the
+ * antennas' centre c (center: device pointer to 3 floats, e.g. the mean of the
+ * normalised receiver positions), dir = (p - c)/(|p - c| + 1e-8); the ray
+ * c + t*dir meets |x| = radius at t1 <= t2; t = t1 if t1 > 1e-5, else t2 if
+ * t2 > 1e-5, else none.  valid[i] = 1 if the discriminant is >= 0 and a t was
+ * found; hit [M,3] = c + t*dir (= c where no t was found, where the reference
+ * has c + inf*dir; such hits are never used).  FP32, torch's operation order. */
+int geraf_intersect_sphere_f32(const float* center, const float* points, float* hit,
+                               uint8_t* valid, int64_t M, float radius, geraf_stream_t stream);
+
+/* Transmittance offset of the calibration (_calibrate_transmission,
+ * rf_rendering.py:611-626, used at :749-750): with prev = sigmoid(inv_s *
+ * sdf_first[r]) and new = valid[r,s] ? sigmoid(inv_s * sdf_hit[r,s]) : prev,
+ * offset[r,s] = new - prev (the fused renderer's trans_offset).  sdf_hit,
+ * valid, offset [R,S]; sdf_first [R] (the SDF at each ray's first sample);
+ * inv_s: device scalar [1]. */
+int geraf_calibration_offset_f32(const float* sdf_hit, const float* sdf_first,
+                                 const uint8_t* valid, const float* inv_s, float* offset,
+                                 int64_t R, int64_t S, geraf_stream_t stream);
+
 /* Fast matched filter: the operator of geraf_mf_forward_f32 (and of
  * _C.MatchedFilter_forward, vision.cpp:70-74; MatchedFilter_cuda_kernel.cu:65-115)
  * evaluated by per-tile expansions in the delay offset.  It has no counterpart

@@ -98,3 +98,53 @@ def masks(sdf, grad, dists, pts, o, d, inv_s, power, refl=None, offset=None, n_v
                                 0.0, refl=refl, offset=offset, n_valid=n_valid, anneal=anneal,
                                 return_parts=True)
     return parts
+
+
+# ----------------------------------------------------------- transmittance calibration
+
+SPHERE_RADIUS = 1.0  # rf_rendering.py:27
+HIT_EPS = 1e-5       # rf_rendering.py:484-490 (t1 > 1e-5 ? t1 : t2 > 1e-5 ? t2 : inf)
+DIR_EPS = 1e-8       # rf_rendering.py:466-468
+
+
+def intersect_sphere(r_pos_norm, norm_pts, radius=SPHERE_RADIUS):
+    """FP64 numpy restatement of GeRaFStage1.intersect_sphere
+    (rf_rendering.py:455-495): rays from the centre of the antennas
+    (the mean of r_pos_norm [A,3]) through every sample norm_pts [R,S,3] meet
+    the sphere |x| = radius at t1 <= t2; t is t1 if t1 > 1e-5, else t2 if
+    t2 > 1e-5, else none.  Returns (valid [1,R,S] bool, hit [1,R,S,3]); hit
+    rows of invalid samples are +-inf / nan, as the reference's c + inf * dir."""
+    import numpy as np
+
+    c = np.mean(np.asarray(r_pos_norm, dtype=np.float64), axis=0)
+    p = np.asarray(norm_pts, dtype=np.float64)
+    v = p - c
+    dirs = v / (np.linalg.norm(v, axis=-1, keepdims=True) + DIR_EPS)
+    b = 2.0 * np.sum(c * dirs, axis=-1)
+    cc = np.dot(c, c) - radius ** 2
+    disc = b * b - 4.0 * cc
+    sq = np.sqrt(np.maximum(disc, 0.0))
+    t1 = (-b - sq) / 2.0
+    t2 = (-b + sq) / 2.0
+    t = np.where(t1 > HIT_EPS, t1, np.where(t2 > HIT_EPS, t2, np.inf))
+    valid = (disc >= 0.0) & np.isfinite(t)
+    with np.errstate(invalid="ignore"):
+        hit = c + t[..., None] * dirs
+    return valid[None], hit[None]
+
+
+def transmission_offset(sdf_hit, sdf_first, valid, inv_s):
+    """FP64 numpy restatement of _calibrate_transmission (rf_rendering.py:611-626)
+    and its use at :749-750: prev_cdf = sigmoid(inv_s * sdf at each ray's
+    first sample) [R,1], new_cdf = sigmoid(inv_s * sdf at the sphere hit) where
+    the hit is valid, prev_cdf elsewhere; the calibrated transmittance
+    (T - prev_cdf) + new_cdf is T + offset with offset = new_cdf - prev_cdf
+    [R,S].  sdf_hit [R,S] (any value where not valid), sdf_first [R]."""
+    import numpy as np
+
+    sig = lambda x: 1.0 / (1.0 + np.exp(-x))  # noqa: E731
+    prev = sig(np.asarray(sdf_first, dtype=np.float64) * inv_s)[:, None]
+    with np.errstate(invalid="ignore", over="ignore"):
+        hit = sig(np.asarray(sdf_hit, dtype=np.float64) * inv_s)
+    new = np.where(np.asarray(valid, dtype=bool), hit, prev)
+    return new - prev

@@ -1,5 +1,7 @@
 """Host-side model utilities (CPU): the device ADC bank's mix order and the
 matched-filter volume format/loader."""
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -127,3 +129,27 @@ def test_stage_antenna_groups_cover_every_ray_once(R, positions):
     (gs,) = torch.autograd.grad(got, sig, g)
     (gr,) = torch.autograd.grad(ref, sig, g)
     assert torch.equal(gs, gr)
+
+
+def test_compute_mf_loss_modes_closed_form():
+    """compute_mf_loss (rf_rendering.py:196-209) on [n,1] predictions: each
+    mode against its formula written out per element, and an unknown mode
+    raises NotImplementedError."""
+    from geraf.models.rendering.rf_stage_ops import compute_mf_loss
+
+    pred = torch.tensor([[0.1], [0.9], [2.0], [0.5]], dtype=torch.float64)
+    gt = torch.tensor([[0.3], [0.2], [2.1], [0.5]], dtype=torch.float64)
+    d = [p - g for p, g in zip(pred[:, 0].tolist(), gt[:, 0].tolist())]
+    want = {
+        "l2": [x * x for x in d],
+        "l1": [abs(x) for x in d],
+        # smooth L1 with beta = 0.25: 0.5 x^2 / beta below beta, |x| - beta / 2 above
+        "smooth_l1": [0.5 * x * x / 0.25 if abs(x) < 0.25 else abs(x) - 0.125 for x in d],
+        "charbonier": [math.sqrt(x * x + 1e-3) for x in d],
+    }
+    for mode, w in want.items():
+        got = compute_mf_loss(pred, gt, mode)
+        assert got.shape == (4,)
+        np.testing.assert_allclose(got.numpy(), w, rtol=1e-15, atol=1e-15, err_msg=mode)
+    with pytest.raises(NotImplementedError):
+        compute_mf_loss(pred, gt, "sum")

@@ -402,3 +402,74 @@ def test_stage_step_torch_form_matches_oracle():
     out = (o_r + 1j * o_i).numpy()
     # points are formed in float64 here and in float32 by the oracle
     assert np.linalg.norm(out - ref.numpy()) <= 1e-4 * np.linalg.norm(ref.numpy())
+
+
+# ------------------------------------------------ transmittance calibration (sphere hits)
+
+def test_intersect_sphere_branches_closed_form():
+    """Each branch of t = t1 > 1e-5 ? t1 : t2 > 1e-5 ? t2 : inf
+    (rf_rendering.py:484-490) on rays whose roots are known: from outside the
+    unit sphere toward it (t1 = 2), from inside (t1 < 0, t2 = 1/2), missing it
+    (negative discriminant), with the sphere behind the centre (both roots
+    negative), and t1 just below the 1e-5 cut (t2 taken instead)."""
+    # centre (0, 0, -3): towards +z the roots of t^2 - 6t + 8 are 2 and 4
+    valid, hit = L.intersect_sphere(np.array([[0, 0, -3.0]]), np.array([[[0, 0, 0.0]]]))
+    scale = 3.0 / (3.0 + L.DIR_EPS)  # dir = v / (|v| + 1e-8) is not quite unit
+    b = 2.0 * -3.0 * scale
+    t1 = (-b - math.sqrt(b * b - 32.0)) / 2.0
+    assert valid[0, 0, 0] and abs(t1 - 2.0) < 1e-7
+    np.testing.assert_allclose(hit[0, 0, 0], [0, 0, -3.0 + t1 * scale], rtol=0, atol=1e-15)
+    # centre inside the sphere: t1 = -1.5 < 0, t2 = 0.5 -> the far root
+    valid, hit = L.intersect_sphere(np.array([[0, 0, 0.5]]), np.array([[[0, 0, 0.9]]]))
+    assert valid[0, 0, 0] and abs(hit[0, 0, 0, 2] - 1.0) < 1e-7
+    # misses: discriminant < 0; the reference's hit there is still c + t1 dir
+    valid, hit = L.intersect_sphere(np.array([[0, 0, -3.0]]), np.array([[[2, 0, -2.0]]]))
+    assert not valid[0, 0, 0] and np.all(np.isfinite(hit))
+    # sphere behind: roots -4 and -2 -> no t, invalid, hit not finite
+    valid, hit = L.intersect_sphere(np.array([[0, 0, -3.0]]), np.array([[[0, 0, -4.0]]]))
+    assert not valid[0, 0, 0] and not np.all(np.isfinite(hit))
+    # centre on the sphere (t1 = 0 <= 1e-5): the other root, across the sphere
+    valid, hit = L.intersect_sphere(np.array([[0, 0, -1.0]]), np.array([[[0, 0, 0.0]]]))
+    assert valid[0, 0, 0] and abs(hit[0, 0, 0, 2] - 1.0) < 1e-7
+
+
+def test_intersect_sphere_hits_lie_on_the_sphere_nearest_positive_root():
+    """Brute force: for random centres and samples, a valid hit lies on the
+    sphere, on the ray, and no smaller t > 1e-5 along the ray reaches the
+    sphere (a fine scan of |c + t dir| finds the first crossing at the hit)."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        c = rng.uniform(-2, 2, (5, 3))
+        pts = rng.uniform(-1.2, 1.2, (4, 6, 3))
+        valid, hit = L.intersect_sphere(c, pts)
+        cen = c.mean(0)
+        for r, s in zip(*np.nonzero(valid[0])):
+            h = hit[0, r, s]
+            v = pts[r, s] - cen
+            d = v / (np.linalg.norm(v) + L.DIR_EPS)
+            t = np.dot(h - cen, d) / np.dot(d, d)
+            assert t > 1e-5 and np.linalg.norm(cen + t * d - h) < 1e-9
+            # on the sphere up to the 1e-8 in the direction's normalisation
+            # (the roots take the direction as a unit vector)
+            assert abs(np.linalg.norm(h) - 1.0) <= 1e-9 + 4.0 * t * (1.0 - np.linalg.norm(d))
+            ts_ = np.linspace(1e-5, t, 2000)[:-1]
+            rad = np.linalg.norm(cen[None] + ts_[:, None] * d[None], axis=1)
+            inside0 = np.linalg.norm(cen) < 1.0
+            # no earlier crossing: the side of the sphere does not change before t
+            assert np.all((rad < 1.0) == inside0) or np.all(np.abs(rad - 1.0) < 1e-3)
+
+
+def test_transmission_offset_closed_form():
+    sig = lambda x: 1.0 / (1.0 + math.exp(-x))  # noqa: E731
+    sdf_hit = np.array([[0.1, -0.2, 5.0], [0.0, 0.3, -0.4]])
+    sdf_first = np.array([0.05, -0.1])
+    valid = np.array([[True, False, True], [False, True, True]])
+    off = L.transmission_offset(sdf_hit, sdf_first, valid, 64.0)
+    for r in range(2):
+        for s in range(3):
+            want = (sig(64 * sdf_hit[r, s]) - sig(64 * sdf_first[r])) if valid[r, s] else 0.0
+            assert abs(off[r, s] - want) < 1e-15
+    # invalid samples give exactly zero even where sdf_hit is not finite
+    off = L.transmission_offset(np.array([[np.nan, 0.1]]), np.array([0.2]),
+                                np.array([[False, True]]), 10.0)
+    assert off[0, 0] == 0.0 and np.isfinite(off[0, 1])
