@@ -86,11 +86,18 @@ def sample_volume(volume, points, mode: str = "bilinear", align_corners: bool = 
 
 def backproject(real: torch.Tensor, imag: torch.Tensor, t_pos: torch.Tensor, r_pos: torch.Tensor,
                 bound: Sequence[float], mf_res: Sequence[float], radar_cfg: dict,
-                slab_voxels: int = 1 << 22) -> torch.Tensor:
+                slab_voxels: int = 1 << 22, algo: str = "tiled",
+                tol: float = 1e-6) -> torch.Tensor:
     """|matched filter| of one frame over the grid, [Nx, Ny, Nz] float32 on the
     echo's device.  The grid is evaluated in slabs of x-planes of about
     `slab_voxels` voxels, so the voxel coordinates never occupy more than a
-    slab's worth of memory."""
+    slab's worth of memory.  ``algo``: "tiled" (the per-tile expansion form,
+    truncation bound ``tol`` of sum |s| per antenna; a grid tiles perfectly)
+    or "direct" (the drop-in op's kernels)."""
+    if algo not in ("tiled", "direct"):
+        raise ValueError(f"unknown matched-filter algorithm {algo!r}")
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import tiled_forward
+
     nx, ny, nz = grid_shape(bound, mf_res)
     dev = real.device
     ax = [torch.tensor(a, dtype=torch.float32, device=dev) for a in grid_axes(bound, mf_res)]
@@ -101,15 +108,19 @@ def backproject(real: torch.Tensor, imag: torch.Tensor, t_pos: torch.Tensor, r_p
         x1 = min(nx, x0 + planes)
         gx, gy, gz = torch.meshgrid(ax[0][x0:x1], ax[1], ax[2], indexing="ij")
         vox = torch.stack([gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)], dim=1)
-        re, im = matched_filter_cuda(real, imag, t_pos, r_pos, vox, A, K,
-                                     radar_cfg["adcSampleRate"], radar_cfg["As_sci"],
-                                     vox.shape[0])
+        if algo == "tiled":
+            re, im, _ = tiled_forward(real, imag, t_pos, r_pos, vox, K,
+                                      radar_cfg["adcSampleRate"], radar_cfg["As_sci"], tol)
+        else:
+            re, im = matched_filter_cuda(real, imag, t_pos, r_pos, vox, A, K,
+                                         radar_cfg["adcSampleRate"], radar_cfg["As_sci"],
+                                         vox.shape[0])
         out[x0:x1] = torch.hypot(re, im).reshape(x1 - x0, ny, nz)
     return out
 
 
 def produce_frame(frame_dir: str, bound: Sequence[float], mf_res: Sequence[float],
-                  radar_cfg: dict, device: str = "cuda") -> np.ndarray:
+                  radar_cfg: dict, device: str = "cuda", algo: str = "tiled") -> np.ndarray:
     """Backproject a frame folder (adcr.npy, adci.npy [M, K]; tpos.npy, rpos.npy
     [M, 3]) into its sarimage.bin and return the cube."""
     load = lambda n: torch.tensor(np.load(os.path.join(frame_dir, n)).astype(np.float32),  # noqa
@@ -118,6 +129,6 @@ def produce_frame(frame_dir: str, bound: Sequence[float], mf_res: Sequence[float
     real = real.reshape(real.shape[0], -1)
     imag = imag.reshape(imag.shape[0], -1)
     t_pos, r_pos = load("tpos.npy").reshape(-1, 3), load("rpos.npy").reshape(-1, 3)
-    vol = backproject(real, imag, t_pos, r_pos, bound, mf_res, radar_cfg)
+    vol = backproject(real, imag, t_pos, r_pos, bound, mf_res, radar_cfg, algo=algo)
     write_sarimage(os.path.join(frame_dir, "sarimage.bin"), vol)
     return vol.cpu().numpy()

@@ -417,7 +417,8 @@ def test_backproject_volume_matches_matched_filter(cuda, tmp_path):
     tx, rx = torch.tensor(sc.tx, device=cuda), torch.tensor(sc.rx, device=cuda)
     radar = dict(numADCSample=K, adcSampleRate=sc.fs, As_sci=sc.As_sci)
     bound, res = [-0.02, 0.02, -0.02, 0.02, 0.23, 0.27], [0.004, 0.004, 0.004]
-    vol = mv.backproject(re, im, tx, rx, bound, res, radar, slab_voxels=300)  # several slabs
+    vol = mv.backproject(re, im, tx, rx, bound, res, radar, slab_voxels=300,
+                         algo="direct")  # several slabs
     ax = [torch.tensor(a, dtype=torch.float32, device=cuda) for a in mv.grid_axes(bound, res)]
     g = torch.meshgrid(*ax, indexing="ij")
     vox = torch.stack([x.reshape(-1) for x in g], 1)
@@ -432,7 +433,7 @@ def test_backproject_volume_matches_matched_filter(cuda, tmp_path):
     for name, arr in (("adcr.npy", sc.echo.real), ("adci.npy", sc.echo.imag),
                       ("tpos.npy", sc.tx), ("rpos.npy", sc.rx)):
         np.save(d / name, np.asarray(arr, dtype=np.float32))
-    cube = mv.produce_frame(str(d), bound, res, radar, device=str(cuda))
+    cube = mv.produce_frame(str(d), bound, res, radar, device=str(cuda), algo="direct")
     assert np.array_equal(mv.read_sarimage(str(d / "sarimage.bin"), bound, res), cube)
     v = vol.cpu().numpy()
     assert np.linalg.norm(cube - v) / np.linalg.norm(v) < 1e-6
