@@ -124,6 +124,44 @@ def bench_tiled(dev, sc, re, im, tx, rx, sel, ref, direct_ms):
             "workload": "matched filter, 64x64 MIMO, K=512, 128^3 voxels (Morton 8^3 tiles)"}
 
 
+def plain_forms(dev, sc, re, im, tx, rx, vox, prm, stream, n_vox: int = 262144,
+                n_ant: int = 64, reps: int = 3):
+    """terms/s of the plain FP32 (per-term sincosf, the reference formula) and
+    plain FP64 GPU forms on n_vox voxels x n_ant antennas of the workload: a
+    launch that fills the part (n_vox / 256 blocks); one warm-up launch, then
+    the median of `reps` launches, each between its own events."""
+    import numpy as np
+    import torch
+
+    from geraf.ops import _native
+
+    K = sc.K
+    n_vox = min(n_vox, sc.N)
+    n_ant = min(n_ant, sc.A)
+    out = {"voxels": n_vox, "antennas": n_ant, "K": K, "launches": reps}
+    for name, dt, key in (("geraf_mf_forward_plain_f32", torch.float32, "fp32_per_term_sincos"),
+                          ("geraf_mf_forward_f64", torch.float64, "fp64")):
+        ins = [x[:n_ant].to(dt).contiguous() for x in (re, im, tx, rx)]
+        vs = vox[:, :n_vox].to(dt).contiguous()
+        o_r = torch.empty(n_vox, device=dev, dtype=dt)
+        o_i = torch.empty_like(o_r)
+        ptrs = list(map(_native.ptr, ins + [o_r, o_i, vs]))
+        call = lambda: _native.call(name, *ptrs, n_ant, K, n_vox, prm, stream)  # noqa: E731
+        call()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        out[key] = n_vox * float(n_ant) * K / (float(np.median(ms)) / 1e3)
+        out[key + "_ms"] = float(np.median(ms))
+    return out
+
+
 def mf_scenes_nearest(sc):
     from rf_workloads import mf_scenes
     return mf_scenes.nearest_voxels(sc, sc.scatterers)
@@ -180,7 +218,9 @@ def bench_adjoint(dev, sc, re, im, tx, rx, vox, g_r, g_i, prm, sel):
     log(f"adjoint: {ms:.1f} ms, {rate:.3e} terms/s; rel-L2 vs FP64 adjoint on {n} voxels "
         f"{rel:.2e}")
     return {"metric": "MF adjoint voxel*antenna*freq terms/s", "value": rate, "unit": "terms/s",
-            "ms_per_launch": ms, "kernel": "mf_backward_kernel",
+            "ms_per_launch": ms,
+            "kernel": "mf_backward_mma16_kernel<2>" if 256 < K <= 512 else
+                      "mf_backward_mma16_kernel" if 128 < K <= 1024 else "mf_backward_kernel",
             "rel_l2_vs_fp64": rel, "check_voxels": n,
             "workload": "matched-filter adjoint, 64x64 MIMO, K=512, 128^3 voxels"}
 
@@ -285,10 +325,11 @@ def bench_large_target(dev):
                                 "rel_l2_vs_direct_on_slab": rel,
                                 "speedup_vs_direct": (A * K * 256 ** 3 / rate) / full_s}
 
-    # errors against the FP64 GPU form on voxels of the central 64^3 subvolume
-    # (a full 64^3 subvolume is 4.4e15 FP64 terms at this aperture: a sample)
+    # errors against the FP64 GPU form on 256 voxels of the central 64^3
+    # subvolume (a full 64^3 subvolume is 4.4e15 FP64 terms at this aperture: a
+    # sample)
     rng = np.random.default_rng(64)
-    ijk = rng.integers(96, 160, (24, 3))
+    ijk = rng.integers(96, 160, (256, 3))
     idx = torch.tensor((ijk[:, 0] * 256 + ijk[:, 1]) * 256 + ijk[:, 2], device=dev)
     def d64(x):
         return torch.tensor(np.ascontiguousarray(x), device=dev, dtype=torch.float64)
@@ -347,6 +388,163 @@ def bench_stage2_step(dev, target_volume, axes, steps: int = 2, warmup: int = 1)
         f"MF tiled fraction {step.plan.tiled_fraction:.3f}")
     del step
     torch.cuda.empty_cache()
+    return out
+
+
+def bench_ap_tracer(dev, reps: int = 10):
+    """The [A,P] signal-tracer op (SignalTracer_forward / _backward_normal_sigma)
+    at the training shape of ap_training_case, each direction timed per launch
+    (median of `reps`), with its FP32 and HBM roofline fractions; accuracy of
+    one antenna row and 2,048 points against the FP64 oracle."""
+    import numpy as np
+    import torch
+
+    from geraf.ops import _native
+    from geraf.ops.RFSignalTracer import _C as stC
+
+    from rf_workloads.tracer_scenes import ap_training_case
+
+    c = ap_training_case(dev)
+    A, P, K, r = c["A"], c["P"], c["K"], c["radar"]
+    prm = stC.tracer_params(r["fc_start"], r["As_sci"], r["adcSampleRate"])
+    o_r = torch.empty((A, K), device=dev)
+    o_i = torch.empty_like(o_r)
+    rng = np.random.default_rng(1)
+    g_r = torch.tensor(rng.standard_normal((A, K)), dtype=torch.float32, device=dev)
+    g_i = torch.tensor(rng.standard_normal((A, K)), dtype=torch.float32, device=dev)
+    gn = torch.empty((P, 3), device=dev)
+    gsg = torch.empty((A, P), device=dev)
+    Pp = _native.ptr
+    st = _native.stream_of(o_r)
+    geo = [Pp(c["normals"]), Pp(c["sigmas"]), Pp(c["points"]), Pp(c["tx"]), Pp(c["rx"]),
+           Pp(c["inbounds"])]
+    fwd = lambda: _native.call("geraf_st_forward_f32", *geo, Pp(o_r), Pp(o_i), A, P, K,  # noqa
+                               prm, st)
+    bwd = lambda: _native.call("geraf_st_backward_f32", Pp(g_r), Pp(g_i), *geo, Pp(gn),  # noqa
+                               Pp(gsg), A, P, K, prm, st)
+
+    def per_launch(fn):
+        fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for e0, e1 in evs:
+            e0.record()
+            fn()
+            e1.record()
+        torch.cuda.synchronize()
+        return float(np.median([e0.elapsed_time(e1) for e0, e1 in evs]))
+
+    f_ms, b_ms = per_launch(fwd), per_launch(bwd)
+    terms = float(A) * P * K
+    # algorithmic work: 8 flops per (antenna, point, sample) each way; bytes:
+    # forward sigmas + points/normals/inbounds + output, backward gradient row
+    # + sigmas + points/normals/inbounds + sigma and normal gradients
+    fb = A * P * 4 + P * 25 + A * K * 8
+    bb = A * K * 8 + A * P * 8 + P * 25 + P * 12
+    res = {"workload": f"[A,P] signal tracer, A={A} antenna pairs (one bank of a 64x64 MIMO "
+                       f"aperture), P={P} points (804 rays x 64 target samples), K={K}",
+           "terms": terms}
+    for name, ms, by in (("forward", f_ms, fb), ("backward", b_ms, bb)):
+        tf = 8.0 * terms / (ms / 1e3) / 1e12
+        gbs = by / (ms / 1e3) / 1e9
+        res[name] = {"ms": ms, "terms_per_s": terms / (ms / 1e3),
+                     "roofline": {"bound": "valu", "achieved_tflops": tf,
+                                  "fp32_frac": tf / VALU_FP32_PEAK_TFLOPS, "hbm_gbs": gbs,
+                                  "hbm_frac": gbs / 8000.0, "bytes": by}}
+    res["kernels"] = {"forward": "st_forward_kernel", "backward": "st_backward_kernel"}
+    # accuracy on one antenna row / a point subset against the FP64 oracle
+    try:
+        from oracle import rf as orc
+        from rf_workloads import tracer_scenes as ts
+        kw = dict(f0=r["fc_start"], slope=r["As_sci"], fs=r["adcSampleRate"],
+                  c=ts.SPEED_OF_LIGHT, bias=ts.PATH_BIAS)
+        cpu = {k: v.cpu().numpy() for k, v in c.items() if torch.is_tensor(v)}
+        sel = slice(0, 2048)
+        fwd()
+        torch.cuda.synchronize()
+        ref = orc.st_forward(cpu["normals"], cpu["sigmas"][:1], cpu["points"], cpu["tx"][:1],
+                             cpu["rx"][:1], cpu["inbounds"], K, **kw)
+        got = (o_r[:1].double() + 1j * o_i[:1].double()).cpu().numpy()
+        res["fwd_rel_l2_vs_fp64_row0"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        bwd()
+        torch.cuda.synchronize()
+        g = (g_r.double() + 1j * g_i.double()).cpu().numpy()
+        gn_ref, gs_ref = orc.st_backward(g, cpu["normals"][sel], cpu["sigmas"][:, sel],
+                                         cpu["points"][sel], cpu["tx"], cpu["rx"],
+                                         cpu["inbounds"][sel], **kw)
+        # the op divides by its own P; the subset call divides by 2,048
+        scale = 2048.0 / P
+        res["bwd_normals_rel_l2_vs_fp64"] = float(
+            np.linalg.norm(gn[sel].double().cpu().numpy() - gn_ref * scale) /
+            np.linalg.norm(gn_ref * scale))
+        res["bwd_sigmas_rel_l2_vs_fp64"] = float(
+            np.linalg.norm(gsg[:, sel].double().cpu().numpy() - gs_ref * scale) /
+            np.linalg.norm(gs_ref * scale))
+    except Exception as e:  # the oracle is a checker; report, do not fail the bench
+        log(f"[A,P] tracer oracle check skipped: {e}")
+    log(f"[A,P] tracer A={A} P={P} K={K}: forward {f_ms:.3f} ms "
+        f"({res['forward']['roofline']['fp32_frac']:.3f} of FP32 peak), backward {b_ms:.3f} ms "
+        f"({res['backward']['roofline']['fp32_frac']:.3f}); errors "
+        f"{ {k: v for k, v in res.items() if 'rel_l2' in k} }")
+    return res
+
+
+def bench_bilinear(dev, reps: int = 20):
+    """The depth resampler (BilinearRaysample_forward / _backward) at the
+    training shape (804 rays, 32 source and 64 target samples, one sigma row)
+    and at 32,768 rays x 256 -> 512 samples, per-launch medians, with the HBM
+    fraction of its algorithmic bytes."""
+    import numpy as np
+    import torch
+
+    from geraf.ops import _native
+
+    out = {}
+    for R, S, T in ((804, 32, 64), (32768, 256, 512)):
+        rng = np.random.default_rng(R)
+        z = np.sort(rng.uniform(0.2, 0.5, (R, S)), axis=1).astype(np.float32)
+        tz = np.sort(rng.uniform(0.19, 0.51, (R, T)), axis=1).astype(np.float32)
+        nr = rng.standard_normal((R, S, 3)).astype(np.float32)
+        sg = rng.uniform(0, 1, (1, R, S)).astype(np.float32)
+        t = lambda x: torch.tensor(x, device=dev)  # noqa: E731
+        zz, tt, nn, ss = t(z), t(tz), t(nr), t(sg)
+        on = torch.empty((R, T, 3), device=dev)
+        os_ = torch.empty((1, R, T), device=dev)
+        ib = torch.empty((R, T), dtype=torch.bool, device=dev)
+        gn = torch.empty((R, S, 3), device=dev)
+        gs = torch.empty((1, R, S), device=dev)
+        P = _native.ptr
+        st = _native.stream_of(zz)
+        fwd = lambda: _native.call("geraf_bilinear_forward_f32", P(nn), P(ss), P(zz), P(tt),  # noqa
+                                   P(on), P(os_), P(ib), 1, R, S, T, st)
+        bwd = lambda: _native.call("geraf_bilinear_backward_f32", P(on), P(os_), P(zz),  # noqa
+                                   P(tt), P(gn), P(gs), 1, R, S, T, st)
+        res = {}
+        # algorithmic bytes: forward reads the sources (normals, sigma, depths)
+        # and target depths, writes normals, sigma, inbounds; the backward
+        # reads the target gradients and both depth rows, writes the source
+        # gradients
+        for name, fn, by in (("forward", fwd, R * S * 20 + R * T * 4 + R * T * 17),
+                             ("backward", bwd, R * T * 16 + R * (S + T) * 4 + R * S * 16)):
+            fn()
+            torch.cuda.synchronize()
+            evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                   for _ in range(reps)]
+            for e0, e1 in evs:
+                e0.record()
+                fn()
+                e1.record()
+            torch.cuda.synchronize()
+            ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in evs]))
+            gbs = by / (ms / 1e3) / 1e9
+            res[name] = {"ms": ms, "bytes": by, "hbm_gbs": gbs, "hbm_frac": gbs / 8000.0}
+        out[f"{R}x{S}to{T}"] = res
+        log(f"bilinear {R} rays {S}->{T}: forward {res['forward']['ms'] * 1e3:.1f} us "
+            f"({res['forward']['hbm_frac']:.3f} of HBM), backward "
+            f"{res['backward']['ms'] * 1e3:.1f} us ({res['backward']['hbm_frac']:.3f})")
+    out["workload"] = ("depth resampler: 804 rays x 32 -> 64 samples (training shape) and "
+                       "32768 rays x 256 -> 512 samples, one sigma row")
     return out
 
 
@@ -423,10 +621,23 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / n
 
+    def per_launch(fn, n):
+        """Median duration of n launches, each between its own pair of events on
+        the launch stream (what a kernel trace reports per dispatch)."""
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(n)]
+        for e0, e1 in evs:
+            e0.record()
+            fn()
+            e1.record()
+        torch.cuda.synchronize()
+        return float(np.median([e0.elapsed_time(e1) for e0, e1 in evs]))
+
     n = max(steps, 4) if samples_per_call(R, S) > (1 << 22) else max(steps, 50)
-    f_ms = burst(fwd, n)
-    b_ms = burst(bwd, n)
-    pair_ms = burst(lambda: (fwd(), bwd()), n)
+    f_ms = per_launch(fwd, n)
+    b_ms = per_launch(bwd, n)
+    pair_ms = f_ms + b_ms
+    burst_pair_ms = burst(lambda: (fwd(), bwd()), n)
     samples = R * S
     rate = samples / (pair_ms / 1e3)
     # algorithmic traffic and work: the forward reads 28 B per sample (sdf,
@@ -439,10 +650,13 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
     flops = 16.0 * K * samples * live  # only emitting samples carry signal work
     gbs = (bytes_fwd + bytes_bwd) / (pair_ms / 1e3) / 1e9
     tfs = flops / (pair_ms / 1e3) / 1e12
-    log(f"tracer {R}x{S} K={K}: fwd {f_ms:.3f} ms, bwd {b_ms:.3f} ms, pair {pair_ms:.3f} ms, {rate:.3e} samples/s, "
+    log(f"tracer {R}x{S} K={K}: fwd {f_ms:.4f} ms, bwd {b_ms:.4f} ms (per launch), pair "
+        f"{pair_ms:.4f} ms (back to back {burst_pair_ms:.4f} ms), {rate:.3e} samples/s, "
         f"{gbs:.0f} GB/s, {tfs:.1f} TFLOP/s (emitting fraction {live:.3f})")
 
-    # accuracy against the FP64 oracle on a ray subset
+    # accuracy against the FP64 oracle on a ray subset: the output gradient is
+    # nonzero only on the first n rays, so all five gradients (inv_s and
+    # power included) are the subset's
     errs = {}
     try:
         if not oracle_check:
@@ -451,8 +665,8 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         n = 256
         sub = {k: getattr(b0, k)[:n].double().clone().requires_grad_(True)
                for k in ("sdf", "grad", "refl")}
-        inv = b0.inv_s.double().clone().requires_grad_(True)
-        pw = b0.power.double().clone().requires_grad_(True)
+        inv = b0.inv_s.double().reshape(1, 1).expand(n, S).clone().requires_grad_(True)
+        pw = b0.power.double().reshape(1, 1).expand(n, S).clone().requires_grad_(True)
         r = b0.radar
         ref = L.lensless_forward(sub["sdf"], sub["grad"], b0.dists[:n].double(),
                                  L.points_f32(b0.o[:n], b0.z[:n], b0.d[:n]),
@@ -462,17 +676,26 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
                                  anneal=b0.anneal)
         gt = torch.complex(g_r[:n].double().cpu(), g_i[:n].double().cpu())
         (ref.real * gt.real + ref.imag * gt.imag).sum().backward()
+        m_r, m_i = torch.zeros_like(g_r), torch.zeros_like(g_i)
+        m_r[:n], m_i[:n] = g_r[:n], g_i[:n]
         fwd()
-        bwd()
+        _native.call("geraf_lensless_backward_f32", P(m_r), P(m_i), *ins, P(gs), P(gg),
+                     P(grf), P(gis), P(gpw), R, S, K, b.anneal, prm, stream)
         torch.cuda.synchronize()
         out = torch.complex(o_r[:n].double(), o_i[:n].double()).cpu()
+        errs["rays_checked"] = n
         errs["fwd_rel_l2"] = float((out - ref.detach()).norm() / ref.detach().norm())
         for name, mine, theirs in (("sdf", gs[:n], sub["sdf"].grad),
                                    ("gradients", gg[:n], sub["grad"].grad),
                                    ("reflectance", grf[:n], sub["refl"].grad)):
             m = mine.double().cpu()
             errs[f"grad_{name}_rel_l2"] = float((m - theirs).norm() / theirs.norm())
-        log(f"tracer errors vs FP64 oracle (256 rays): {errs}")
+        # the two scalars: error relative to the sum of |per-sample terms|
+        for name, mine, leaf in (("inv_s", gis, inv), ("power", gpw, pw)):
+            errs[f"grad_{name}_err_over_sum_abs"] = float(
+                abs(float(mine.double().sum()) - float(leaf.grad.sum())) /
+                float(leaf.grad.abs().sum()))
+        log(f"tracer errors vs FP64 oracle ({n} rays): {errs}")
     except Exception as e:  # a checker failure is reported, not fatal to the timing
         log(f"tracer oracle check skipped: {e}")
 
@@ -488,7 +711,10 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         "metric": "tracer fwd+bwd samples/s",
         "value": rate, "unit": "samples/s",
         "workload": f"fused lensless tracer, {R} rays x {S} samples, K={K}",
-        "fwd_ms": f_ms, "bwd_ms": b_ms, "fwd_bwd_ms": pair_ms, "emitting_fraction": live,
+        "fwd_ms": f_ms, "bwd_ms": b_ms, "fwd_bwd_ms": pair_ms,
+        "timing": "median per-launch kernel durations (events around each launch); "
+                  "back-to-back pair for comparison",
+        "back_to_back_pair_ms": burst_pair_ms, "emitting_fraction": live,
         "roofline": {"hbm_gbs": gbs, "hbm_frac": gbs / 8000.0, "fp32_tflops": tfs,
                      "fp32_frac": tfs / VALU_FP32_PEAK_TFLOPS,
                      "fp32_units": "MFMA (signal for every K, h_s for K > 128), VALU otherwise; the "
@@ -602,22 +828,17 @@ def main() -> int:
     nsc = len(np.unique(mf_scenes.nearest_voxels(sc, sc.scatterers)))
     peak = float(np.max(np.abs(np.abs(out[:nsc]) / np.abs(ref[:nsc]) - 1.0)))
     phase = float(np.max(np.abs(np.angle(out[:nsc] / ref[:nsc]))))
-    f64_rate = ns * float(A) * K / f64_s
-    log(f"rel-L2 vs FP64 on {ns} voxels: {rel:.3e}; peak {peak:.2e}; phase {phase:.2e} rad; "
-        f"plain FP64 {f64_rate:.3e} terms/s")
+    log(f"rel-L2 vs FP64 on {ns} voxels: {rel:.3e}; peak {peak:.2e}; phase {phase:.2e} rad "
+        f"(FP64 reference: {f64_s:.2f} s)")
 
-    # plain FP32 (reference formula) on a slab
-    slab = slice(0, 16384)
-    vs = vox[:, slab].contiguous()
-    p_r = torch.empty(vs.shape[1], device=dev)
-    p_i = torch.empty_like(p_r)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    _native.call("geraf_mf_forward_plain_f32", *map(_native.ptr, (re, im, tx, rx, p_r, p_i, vs)),
-                 A, K, vs.shape[1], prm, stream)
-    torch.cuda.synchronize()
-    p32_rate = vs.shape[1] * float(A) * K / (time.perf_counter() - t1)
-    log(f"plain FP32 {p32_rate:.3e} terms/s")
+    # the plain FP32 (reference formula, per-term sincosf) and FP64 forms on a
+    # launch that fills the part: 262,144 voxels (1,024 blocks of 256) with the
+    # first 64 antennas; terms/s does not depend on the sizes.  One warm-up
+    # launch, then the median of three, each between its own events.
+    plain = plain_forms(dev, sc, re, im, tx, rx, vox, prm, stream)
+    p32_rate, f64_rate = plain["fp32_per_term_sincos"], plain["fp64"]
+    log(f"plain FP32 {p32_rate:.3e} terms/s, plain FP64 {f64_rate:.3e} terms/s "
+        f"({plain['voxels']} voxels x {plain['antennas']} antennas x K={K})")
 
     # the FP64 GPU form pinned to the CPU FP64 oracle on a few voxels
     pin_err = None
@@ -657,7 +878,9 @@ def main() -> int:
         tracer = bench_tracer(dev, max(args.steps, 5), max(args.warmup, 1), args.cpu_seconds / 2,
                               args.no_cpu)
         tracer["stage2_size"] = bench_tracer(dev, max(args.steps, 3), 1, 0.0, True, rays=32768,
-                                             samples=256, K=512, oracle_check=False)
+                                             samples=256, K=512)
+        tracer["ap_op_training_shape"] = bench_ap_tracer(dev)
+        tracer["bilinear_resampler"] = bench_bilinear(dev)
 
     line = {
         "metric": "MF voxel*antenna*freq terms/s",
@@ -700,9 +923,8 @@ def main() -> int:
         "errors_vs_fp64": {"volume_rel_l2": rel, "peak_mag_err_max": peak,
                            "phase_err_max_rad": phase, "voxels": ns,
                            "fp64_gpu_vs_cpu_oracle_rel_l2": pin_err},
-        "plain_forms_terms_per_s": {"fp32_per_term_sincos": p32_rate, "fp64": f64_rate,
-                                    "tuned_speedup_vs_fp64": value / f64_rate,
-                                    "tuned_speedup_vs_fp32": value / p32_rate},
+        "plain_forms_terms_per_s": dict(plain, tuned_speedup_vs_fp64=value / f64_rate,
+                                        tuned_speedup_vs_fp32=value / p32_rate),
         "adjoint": adjoint,
         "tiled_expansion": tiled,
         "tracer": tracer,

@@ -18,7 +18,10 @@ not the values.  --stage2-step N does the same for the stage-2-style step
 (32,768 rays x 256 samples, K=512, consistency term; one warm-up step).
 --tiled runs the per-tile expansion matched filter and its
 adjoint on the 64x64 MIMO, K=512, 128^3 workload (twice: the second call is
-the one to read).
+the one to read).  --ap-ops runs the [A,P] signal tracer forward and backward
+at its training shape (2,048 antenna pairs, 51,456 points, K=64) and the
+depth resampler (804 rays x 32 -> 64 and 32,768 x 256 -> 512 samples), each
+three times.
 """
 import argparse
 import os
@@ -67,8 +70,12 @@ def main() -> None:
     ap.add_argument("--stage2-step", type=int, default=0)
     ap.add_argument("--form", default="fused", choices=("fused", "fp32_autograd"))
     ap.add_argument("--tiled", action="store_true")
+    ap.add_argument("--ap-ops", action="store_true")
     args = ap.parse_args()
     dev = torch.device("cuda:0")
+    if args.ap_ops:
+        ap_ops(dev)
+        return
     if args.tiled:
         tiled(dev)
         return
@@ -153,6 +160,43 @@ def stage_steps(dev, n: int, form: str, stage: int = 1) -> None:
     torch.cuda.synchronize()
     print(f"profile_driver: stage-{stage} step [{form}] "
           f"{(time.perf_counter() - t0) / n * 1e3:.1f} ms", flush=True)
+
+
+def ap_ops(dev) -> None:
+    c = ts.ap_training_case(dev)
+    A, P, K, r = c["A"], c["P"], c["K"], c["radar"]
+    print(f"profile_driver: [A,P] tracer A={A} P={P} K={K}", flush=True)
+    prm = stC.tracer_params(r["fc_start"], r["As_sci"], r["adcSampleRate"])
+    Pp = _native.ptr
+    o_r = torch.empty((A, K), device=dev)
+    o_i = torch.empty_like(o_r)
+    gn = torch.empty((P, 3), device=dev)
+    gs = torch.empty((A, P), device=dev)
+    st = _native.stream_of(o_r)
+    geo = [Pp(c[k]) for k in ("normals", "sigmas", "points", "tx", "rx", "inbounds")]
+    for _ in range(3):
+        _native.call("geraf_st_forward_f32", *geo, Pp(o_r), Pp(o_i), A, P, K, prm, st)
+        _native.call("geraf_st_backward_f32", Pp(o_r), Pp(o_i), *geo, Pp(gn), Pp(gs), A, P, K,
+                     prm, st)
+    torch.cuda.synchronize()
+    for R, S, T in ((804, 32, 64), (32768, 256, 512)):
+        print(f"profile_driver: depth resampler {R} rays {S} -> {T}", flush=True)
+        z = torch.sort(torch.rand(R, S, device=dev) * 0.3 + 0.2, dim=1).values
+        tz = torch.sort(torch.rand(R, T, device=dev) * 0.32 + 0.19, dim=1).values
+        nn_ = torch.randn(R, S, 3, device=dev)
+        sg = torch.rand(1, R, S, device=dev)
+        on = torch.empty((R, T, 3), device=dev)
+        os_ = torch.empty((1, R, T), device=dev)
+        ib = torch.empty((R, T), dtype=torch.bool, device=dev)
+        g_n = torch.empty((R, S, 3), device=dev)
+        g_s = torch.empty((1, R, S), device=dev)
+        for _ in range(3):
+            _native.call("geraf_bilinear_forward_f32", Pp(nn_), Pp(sg), Pp(z), Pp(tz), Pp(on),
+                         Pp(os_), Pp(ib), 1, R, S, T, st)
+            _native.call("geraf_bilinear_backward_f32", Pp(on), Pp(os_), Pp(z), Pp(tz), Pp(g_n),
+                         Pp(g_s), 1, R, S, T, st)
+        torch.cuda.synchronize()
+    print("profile_driver: done", flush=True)
 
 
 def trace_sizes(dev, stage2: bool) -> None:

@@ -119,3 +119,28 @@ def config3(seed: int = 0) -> RayBatch:
     sphere, noisy-sphere SDF (r = 0.5, 0.02 smooth noise), inv_s = 64,
     reflectance U(0.2, 1), power 1.0, K = 256."""
     return rays(4096, 128, 256, seed=seed)
+
+
+def ap_training_case(dev, A_total: int = 4096, bank_size: int = 2, rays: int = 804,
+                     samples: int = 64, K: int = 64, seed: int = 0):
+    """Inputs of the reference's [A,P] signal tracer at its training shape:
+    one antenna bank of a 64x64 MIMO aperture (A = A_total / bank_size =
+    2,048 pairs, tx/rx -/+1 mm), P = rays x target samples = 804 x 64 =
+    51,456 points in a 20 cm region at 30 cm (90 % in bounds), normals facing
+    the array with jitter, per-antenna sigmas U(0.1, 1), K = 64 at the stage
+    configs' 1.25 MHz / 70.15 MHz/us chirp."""
+    rng = np.random.default_rng(seed)
+    p = planar_grid(64, 1.95e-3)[::bank_size][: A_total // bank_size]
+    off = np.array([1.0e-3, 0.0, 0.0])
+    tx, rx = (p - off).astype(np.float32), (p + off).astype(np.float32)
+    A, P = tx.shape[0], rays * samples
+    pts = (rng.uniform(-0.1, 0.1, (P, 3)) + [0.0, 0.0, 0.30]).astype(np.float32)
+    n = -pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    n = (n + 0.3 * rng.standard_normal((P, 3))).astype(np.float32)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    sig = rng.uniform(0.1, 1.0, (A, P)).astype(np.float32)
+    inb = rng.uniform(0, 1, P) < 0.9
+    radar = radar_for_k(K)
+    T = lambda x, dt=None: torch.tensor(np.ascontiguousarray(x), device=dev, dtype=dt)  # noqa
+    return dict(normals=T(n), sigmas=T(sig), points=T(pts), tx=T(tx), rx=T(rx),
+                inbounds=T(inb, torch.bool), A=A, P=P, K=K, radar=radar)
