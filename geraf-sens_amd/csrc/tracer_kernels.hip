@@ -277,9 +277,29 @@ __global__ __launch_bounds__(256) void bilinear_forward_kernel(
   }
 }
 
+// First index t in [0, n) with key[t] >= v, for a non-decreasing key (LDS).
+__device__ __forceinline__ int lds_lower_bound(const int* key, int n, int v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (key[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 // Atomic-free transpose: block (ray r, antenna a).  The targets' brackets and
 // weights go to LDS; each thread then gathers, for its source samples, every
 // target that lands on them, in ascending target order (deterministic).
+// Source s receives the upper weight of the targets bracketed by (s-1, s)
+// and the lower weight of those bracketed by (s, s+1).  When the targets'
+// bracket keys do not decrease along the ray (sorted target depths, below-range
+// targets first and above-range ones last), each of the two sets is a
+// contiguous range of targets found by binary search: O(S log T + T) per ray.
+// Otherwise every source scans all targets (O(S T)).  Both visit the
+// contributing targets in ascending order, so they give the same sums.
 template <typename T>
 __global__ __launch_bounds__(64) void bilinear_backward_kernel(
     const T* __restrict__ g_n, const T* __restrict__ g_s, const T* __restrict__ z,
@@ -288,7 +308,9 @@ __global__ __launch_bounds__(64) void bilinear_backward_kernel(
   extern __shared__ unsigned char smem[];
   T* wl = reinterpret_cast<T*>(smem);
   T* wu = wl + Tn;
-  int* lo = reinterpret_cast<int*>(wu + Tn);
+  // key[t]: the bracket (t between source samples key and key + 1, in
+  // [0, S - 2]), -1 below the first source sample, S - 1 above the last
+  int* key = reinterpret_cast<int*>(wu + Tn);
   const int r = blockIdx.x;
   const int a = blockIdx.y;
   const T* zr = z + (size_t)r * S;
@@ -297,35 +319,51 @@ __global__ __launch_bounds__(64) void bilinear_backward_kernel(
     const int l = bl_lower(zr, S, tv);
     const T z0 = zr[l], z1 = zr[l + 1];
     if (tv < z0 || tv > z1) {
-      lo[t] = -2;  // contributes to no source sample
+      // contributes to no source sample; sorts before (below) or after
+      // (above) every bracket
       wl[t] = wu[t] = T(0);
+      key[t] = tv < z0 ? -1 : S - 1;
     } else {
       const T den = max(z1 - z0, static_cast<T>(1.0e-6f));
       const T u = (tv - z0) / den;
-      lo[t] = l;
+      key[t] = l;
       wu[t] = u;
       wl[t] = T(1) - u;
     }
   }
   __syncthreads();
+  int unsorted = 0;
+  for (int t = threadIdx.x; t + 1 < Tn; t += blockDim.x) unsorted |= key[t] > key[t + 1];
+  const bool ranges = __syncthreads_or(unsorted) == 0;
   const T* gs = g_s + ((size_t)a * R + r) * Tn;
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
     T acc = T(0), an0 = T(0), an1 = T(0), an2 = T(0);
-    for (int t = 0; t < Tn; ++t) {
-      const int l = lo[t];
-      T w;
-      if (l == s)
-        w = wl[t];
-      else if (l + 1 == s)
-        w = wu[t];
-      else
-        continue;
+    auto add = [&](int t, T w) {
       acc += w * gs[t];
       if (a == 0) {
         const T* gn = g_n + ((size_t)r * Tn + t) * 3;
         an0 += w * gn[0];
         an1 += w * gn[1];
         an2 += w * gn[2];
+      }
+    };
+    if (ranges) {
+      // keys s - 1 (upper weight), then keys s (lower weight): ascending t
+      const int t0 = lds_lower_bound(key, Tn, s - 1);
+      const int t1 = lds_lower_bound(key, Tn, s);
+      const int t2 = lds_lower_bound(key, Tn, s + 1);
+      if (s >= 1)
+        for (int t = t0; t < t1; ++t) add(t, wu[t]);
+      if (s <= S - 2)
+        for (int t = t1; t < t2; ++t) add(t, wl[t]);
+    } else {
+      for (int t = 0; t < Tn; ++t) {
+        const int l = key[t];
+        if (l < 0 || l > S - 2) continue;
+        if (l == s)
+          add(t, wl[t]);
+        else if (l + 1 == s)
+          add(t, wu[t]);
       }
     }
     grad_s[((size_t)a * R + r) * S + s] = acc;

@@ -303,6 +303,11 @@ class SDFNetwork(nn.Module):
     def sdf(self, x: torch.Tensor) -> torch.Tensor:
         return self.forward(x)[..., :1]
 
+    def sdf_value(self, x: torch.Tensor) -> torch.Tensor:
+        """The SDF alone, [...] in FP32: the output stays in the compute dtype
+        and only its first column is converted (no FP32 copy of the features)."""
+        return self.forward(x, float_out=False)[..., 0].float()
+
     def sdf_hidden_appearance(self, x: torch.Tensor) -> torch.Tensor:
         return self.forward(x)
 

@@ -219,7 +219,7 @@ class StageStep:
         leaving the centre of the (normalised) antennas (no gradient)."""
         R, S = self.cfg.rays, self.cfg.samples
         inv_s = self.var_net.inv_s().clip(1e-6, 1e6)
-        return transmission_offset(lambda x: self.sdf_net.sdf(x)[..., 0], inv_s,
+        return transmission_offset(self.sdf_net.sdf_value, inv_s,
                                    self.pts_n.view(R, S, 3), self.ant_pos_norm)
 
     def parts(self) -> dict:

@@ -317,3 +317,36 @@ def test_lensless_backward_refuses_inputs_modified_in_place(cuda):
     refl.mul_(2.0)
     with pytest.raises(RuntimeError, match="modified by an inplace operation"):
         (o_r.sum() + o_i.sum()).backward()
+
+
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_bilinear_backward_target_orders(cuda, order):
+    """The resampler's transpose with sorted target depths (contiguous
+    bracket ranges found by binary search; targets below and above the source
+    depths included) and with shuffled ones (the scan over all targets),
+    against the oracle; and each path is bit-reproducible."""
+    from geraf.ops import _native
+
+    rng = np.random.default_rng(31)
+    A, R, S, Tn = 2, 48, 256, 512
+    z = np.sort(rng.uniform(0.2, 0.5, (R, S)), axis=1).astype(np.float32)
+    tz = np.sort(rng.uniform(0.18, 0.52, (R, Tn)), axis=1).astype(np.float32)
+    tz[:, 100:104] = z[:, 50:51]  # exact hits on a source depth
+    if order == "shuffled":
+        tz = np.ascontiguousarray(tz[:, rng.permutation(Tn)])
+    gn = rng.standard_normal((R, Tn, 3)).astype(np.float32)
+    gs = rng.standard_normal((A, R, Tn)).astype(np.float32)
+    T = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda)  # noqa: E731
+    ins = [T(gn), T(gs), T(z), T(tz)]
+    outs = []
+    for _ in range(2):
+        o_n = torch.empty((R, S, 3), device=cuda)
+        o_s = torch.empty((A, R, S), device=cuda)
+        _native.call("geraf_bilinear_backward_f32", *map(_native.ptr, ins + [o_n, o_s]), A, R, S,
+                     Tn, _native.stream_of(o_n))
+        torch.cuda.synchronize()
+        outs.append((o_n.cpu().numpy(), o_s.cpu().numpy()))
+    rgn, rgs = orc.bilinear_backward(gn, gs, z, tz, S)
+    assert np.max(np.abs(outs[0][0] - rgn)) <= 2e-5
+    assert np.max(np.abs(outs[0][1] - rgs)) <= 2e-5
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
