@@ -1,0 +1,299 @@
+"""The SDF network's forward, its analytic gradient (the normals) and the
+hand-derived backward of both, on the library's BF16 matrix-core layer
+products (csrc/mlp_gemm_kernels.hip).
+
+Network (sdf_network.py:78-199, the configuration the RF stages use): the
+positional encoding gamma(x) (39 features), hidden layers l = 0..7 with
+z_l = W_l a_l + b_l, h_l = softplus_100(z_l), a_0 = gamma, a_4 =
+[h_3, gamma] / sqrt 2 (the skip; h_3 has 217 features), a_l = h_{l-1}
+otherwise, and the output layer [sdf, features] = W_8 h_7 + b_8.
+
+Normals, reverse mode by hand: u_7 = W_8[0], u_{l-1} = (u_l * s_l) W_l for
+l = 7..1 (s_l the softplus slope at z_l; layer 4's product splits into u_3
+and the skip's share of d sdf / d gamma), d sdf / d gamma = (u_0 * s_0) W_0 +
+the skip share, then through the encoding's Jacobian.  One product per layer.
+
+Backward, for incoming gradients (d sdf, d features, d normals):
+  * the adjoint of the normal pass (layer 0 upward): d gamma = J^T d n;
+    vbar_0 = d gamma W_0^T, and for l = 0..6 vbar_{l+1} = (vbar_l * s_l)
+    W_{l+1}^T, with dW_{l+1} += (u_{l+1} s_{l+1})^T (vbar_l s_l), dW_0 +=
+    (u_0 s_0)^T d gamma, dW_8[0] += sum (vbar_7 s_7); each layer leaves the
+    second-order term t_l = vbar_l * u_l * s'_l for the primal adjoint;
+  * the primal adjoint (layer 8 downward): zbar_8 = (d sdf, d features),
+    zbar_{l-1} = (zbar_l W_l) * s_{l-1} + t_{l-1}, dW_l += zbar_l^T a_l,
+    db_l = sum zbar_l.
+Each step is one product with its element-wise work fused in (the kernels'
+prologues and epilogues); activations and pre-activations are stored in BF16
+as the torch BF16 network stores them.  The skip's 1/sqrt 2 is folded into
+W_4.  Gradients reach the weight-norm parameters through torch autograd of
+W_l = g v / |v|.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional, Tuple
+
+import torch
+from torch.autograd import Function
+from torch.autograd.function import once_differentiable
+
+from geraf.ops import _native
+
+BF = torch.bfloat16
+BETA, THRESHOLD = 100.0, 20.0
+_PRO_NONE, _PRO_SLOPE = 0, 1
+_EPI_BIAS, _EPI_SOFTPLUS, _EPI_SOFTPLUS_TRAIN, _EPI_RELU, _EPI_GRAD_ADJ, _EPI_LAYER_ADJ = range(6)
+_BIG = 1.0e4  # a pre-activation whose softplus slope is exactly 1
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return _native.ptr(t)
+
+
+def _ld(t: Optional[torch.Tensor]) -> int:
+    return 0 if t is None else (0 if t.stride(0) == 0 else t.stride(0))
+
+
+def pack(w: torch.Tensor, transpose: bool, scale: float = 1.0) -> torch.Tensor:
+    """Operand-order BF16 copy of W [N,K] (or of W^T with transpose)."""
+    w = w.detach().float().contiguous()
+    N, K = (w.shape[1], w.shape[0]) if transpose else (w.shape[0], w.shape[1])
+    size = int(_native.load().geraf_mlp_pack_bytes(N, K))
+    if size < 0:
+        raise ValueError(f"layer of {N} x {K} is beyond the kernels' limits")
+    out = torch.empty(size, dtype=torch.uint8, device=w.device)
+    _native.call("geraf_mlp_pack_weights", _ptr(w), w.stride(0), int(transpose), float(scale),
+                 N, K, _ptr(out), _native.stream_of(w))
+    return out
+
+
+def gemm(x, w_packed, N, K, y, bias=None, x2=None, y2=None, e0=None, e1=None,
+         prologue=_PRO_NONE, epilogue=_EPI_BIAS, M=None) -> None:
+    """y = epilogue(prologue(x) W^T + bias) into the (possibly strided) y."""
+    M = x.shape[0] if M is None else M
+    _native.call("geraf_mlp_gemm_bf16", _ptr(x), _ld(x), _ptr(x2), _ld(x2), _ptr(w_packed),
+                 _ptr(bias), _ptr(y), _ld(y), _ptr(y2), _ld(y2), _ptr(e0), _ld(e0), _ptr(e1),
+                 _ld(e1), M, N, K, prologue, epilogue, BETA, THRESHOLD, _native.stream_of(y))
+
+
+class _Workspace:
+    def __init__(self):
+        self.buf = None
+
+    def get(self, device, nbytes: int) -> torch.Tensor:
+        if self.buf is None or self.buf.numel() < nbytes or self.buf.device != device:
+            self.buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        return self.buf
+
+
+_WS = _Workspace()
+
+
+def wgrad(g, x, N, K, dw, db=None, g2=None, x2=None, accumulate=False, M=None) -> None:
+    """dw[N,K] (+)= prologue(g)^T prologue(x); db (+)= column sums."""
+    M = (g.shape[0] if g.stride(0) else x.shape[0]) if M is None else M
+    nbytes = int(_native.load().geraf_mlp_wgrad_workspace_bytes(M, N, K))
+    ws = _WS.get(dw.device, nbytes)
+    _native.call("geraf_mlp_wgrad_bf16", _ptr(g), _ld(g), _ptr(g2), _ld(g2),
+                 _PRO_SLOPE if g2 is not None else _PRO_NONE, _ptr(x), _ld(x), _ptr(x2), _ld(x2),
+                 _PRO_SLOPE if x2 is not None else _PRO_NONE, _ptr(dw), dw.stride(0), _ptr(db),
+                 int(accumulate), M, N, K, BETA, THRESHOLD, _ptr(ws), ws.numel(),
+                 _native.stream_of(dw))
+
+
+# layer l: (fan-in, fan-out) of the RF stages' SDF network
+IN = [39, 256, 256, 256, 256, 256, 256, 256, 256]
+OUT = [256, 256, 256, 217, 256, 256, 256, 256, 257]
+SKIP = 4
+NL = 9
+L_BANDS = 6
+SQRT2 = math.sqrt(2.0)
+
+
+def _check_network(ws: List[torch.Tensor]) -> None:
+    for l, w in enumerate(ws):
+        if tuple(w.shape) != (OUT[l], IN[l]):
+            raise ValueError(f"layer {l}: weight {tuple(w.shape)}, the fused network needs "
+                             f"{(OUT[l], IN[l])}")
+
+
+def _packs(ws):
+    """P_l packs W_l ([out, in]; the forward and the normal-pass adjoint),
+    Q_l packs W_l^T ([in, out]; the normal pass and the primal adjoint);
+    layer 4 carries the skip's 1/sqrt 2."""
+    P, Q = [], []
+    for l, w in enumerate(ws):
+        sc = 1.0 / SQRT2 if l == SKIP else 1.0
+        P.append(pack(w, False, sc))
+        Q.append(pack(w, True, sc) if l >= 1 else None)
+    return P, Q
+
+
+def _forward(x, ws, bs, P, train: bool):
+    """Hidden layers and the output layer.  Returns (gam, z list, h list, out8)."""
+    M = x.shape[0]
+    dev = x.device
+    st = _native.stream_of(x)
+    gam = torch.empty((M, 64), dtype=BF, device=dev)
+    _native.call("geraf_mlp_positional_encoding", _ptr(x), M, L_BANDS, 1.0, _ptr(gam), 64, 0, 64,
+                 st)
+    zs, hs = [], []
+    a = gam
+    for l in range(8):
+        h = torch.empty((M, 256), dtype=BF, device=dev)
+        z = torch.empty((M, 256), dtype=BF, device=dev) if train else None
+        if l == 3:
+            # the skip input [h_3, gamma] shares h_3's buffer; z_3's last
+            # columns hold a constant whose slope is 1 (the backward passes the
+            # skip's gradients through them)
+            _native.call("geraf_mlp_positional_encoding", _ptr(x), M, L_BANDS, 1.0,
+                         _ptr(h[:, 217:]), 256, 0, 39, st)
+            if train:
+                z[:, 217:].fill_(_BIG)
+        gemm(a, P[l], OUT[l], IN[l], h, bias=bs[l], y2=z,
+             epilogue=_EPI_SOFTPLUS_TRAIN if train else _EPI_SOFTPLUS)
+        zs.append(z)
+        hs.append(h)
+        a = h
+    out8 = torch.empty((M, 264), dtype=BF, device=dev)
+    gemm(a, P[8], OUT[8], IN[8], out8, bias=bs[8])
+    return gam, zs, hs, out8
+
+
+def _normal_pass(x, ws, zs, Q):
+    """u_l for l = 0..6 (u_3's buffer also carries the skip's d sdf / d gamma
+    in columns 217..255), u_7 = W_8[0] (one broadcast row), and the normals."""
+    M = x.shape[0]
+    dev = x.device
+    u7 = torch.zeros((1, 256), dtype=BF, device=dev)
+    u7[0] = ws[8].detach()[0].to(BF)
+    u7 = u7.expand(M, 256)
+    us = [None] * 8
+    us[7] = u7
+    for l in range(7, 0, -1):
+        u = torch.empty((M, 256), dtype=BF, device=dev)
+        gemm(us[l], Q[l], IN[l], OUT[l], u, x2=zs[l], prologue=_PRO_SLOPE)
+        us[l - 1] = u
+    g0 = torch.empty((M, 40), dtype=BF, device=dev)
+    # layer 0's product with W_0^T: the packed W_0 read as [in, out]
+    gemm(us[0], pack(ws[0], True), IN[0], OUT[0], g0, x2=zs[0], prologue=_PRO_SLOPE)
+    n = torch.empty((M, 3), dtype=torch.float32, device=dev)
+    skip = us[3][:, 217:]
+    _native.call("geraf_mlp_pe_gradient", _ptr(x), M, L_BANDS, 1.0, _ptr(g0), 40, _ptr(skip), 256,
+                 _ptr(n), _native.stream_of(x))
+    return us, n
+
+
+class _FusedSDF(Function):
+    @staticmethod
+    def forward(ctx, x, with_normals: bool, *params):
+        ws, bs = list(params[0::2]), [b.detach().float().contiguous() for b in params[1::2]]
+        _check_network(ws)
+        x = x.detach().float().contiguous()
+        P, Q = _packs(ws)
+        gam, zs, hs, out8 = _forward(x, ws, bs, P, train=True)
+        sdf = out8[:, 0].float()
+        feat = out8[:, 1:257]
+        if with_normals:
+            us, n = _normal_pass(x, ws, zs, Q)
+        else:
+            us, n = None, torch.zeros((x.shape[0], 3), dtype=torch.float32, device=x.device)
+        ctx.save_for_backward(*params)
+        ctx.state = (x, gam, zs, hs, us, P, Q, with_normals)
+        ctx.mark_non_differentiable(*([] if with_normals else [n]))
+        return sdf, feat, n
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, d_sdf, d_feat, d_n):
+        params = ctx.saved_tensors
+        ws = list(params[0::2])
+        x, gam, zs, hs, us, P, Q, with_normals = ctx.state
+        ctx.state = None
+        M = x.shape[0]
+        dev = x.device
+        dws = [torch.zeros((OUT[l], IN[l]), dtype=torch.float32, device=dev) for l in range(NL)]
+        dbs = [torch.zeros(OUT[l], dtype=torch.float32, device=dev) for l in range(NL)]
+        ts = [None] * 8
+        zero_row = torch.zeros((1, 256), dtype=BF, device=dev).expand(M, 256)
+        normal_terms = with_normals and d_n is not None
+        if normal_terms:
+            dn = d_n.float().contiguous()
+            dgam = torch.empty((M, 40), dtype=BF, device=dev)
+            _native.call("geraf_mlp_pe_gradient_adjoint", _ptr(x), M, L_BANDS, 1.0, _ptr(dn),
+                         _ptr(dgam), 40, 0, 40, _native.stream_of(x))
+            # dW_0 += (u_0 s_0)^T d gamma (before u_0 is overwritten by t_0)
+            wgrad(us[0], dgam, OUT[0], IN[0], dws[0], g2=zs[0])
+            vb = torch.empty((M, 256), dtype=BF, device=dev)
+            gemm(dgam, P[0], OUT[0], IN[0], vb, y2=us[0], e0=zs[0], e1=us[0],
+                 epilogue=_EPI_GRAD_ADJ)
+            ts[0] = us[0]
+            for l in range(7):
+                nxt = l + 1
+                if l == 3:
+                    # the skip's share: vbar_3's last columns carry d gamma
+                    _native.call("geraf_mlp_pe_gradient_adjoint", _ptr(x), M, L_BANDS, 1.0,
+                                 _ptr(dn), _ptr(vb[:, 217:]), 256, 0, 39,
+                                 _native.stream_of(x))
+                u_n = us[nxt]
+                wgrad(u_n, vb, OUT[nxt], IN[nxt], dws[nxt], g2=zs[nxt], x2=zs[l],
+                      accumulate=True)
+                vb_n = torch.empty((M, 256), dtype=BF, device=dev)
+                if nxt == 7:
+                    ts[7] = torch.empty((M, 256), dtype=BF, device=dev)
+                    t_out = ts[7]
+                else:
+                    t_out = u_n
+                    ts[nxt] = u_n
+                gemm(vb, P[nxt], OUT[nxt], IN[nxt], vb_n, x2=zs[l], y2=t_out, e0=zs[nxt],
+                     e1=u_n, prologue=_PRO_SLOPE, epilogue=_EPI_GRAD_ADJ)
+                vb = vb_n
+            # dW_8[0] += sum over the points of vbar_7 s_7
+            ones = torch.ones((1, 8), dtype=BF, device=dev).expand(M, 8)
+            row = torch.zeros((1, 256), dtype=torch.float32, device=dev)
+            wgrad(ones, vb, 1, 256, row, x2=zs[7])
+            dws[8][0] += row[0]
+            del vb
+        # primal adjoint
+        zb = torch.zeros((M, 264), dtype=BF, device=dev)
+        if d_sdf is not None:
+            zb[:, 0] = d_sdf.to(BF)
+        if d_feat is not None:
+            zb[:, 1:257] = d_feat.to(BF)
+        wgrad(zb, hs[7], OUT[8], IN[8], dws[8], db=dbs[8], accumulate=True)
+        for l in range(8, 0, -1):
+            prev = l - 1
+            zb_p = torch.empty((M, 256), dtype=BF, device=dev)
+            t = ts[prev] if normal_terms else zero_row
+            gemm(zb, Q[l], IN[l], OUT[l], zb_p, e0=zs[prev], e1=t, epilogue=_EPI_LAYER_ADJ)
+            a_prev = hs[prev - 1] if prev >= 1 else gam  # layer 4: [h_3, gamma]
+            wgrad(zb_p, a_prev, OUT[prev], IN[prev], dws[prev], db=dbs[prev], accumulate=True)
+            zb = zb_p
+        dws[SKIP] *= 1.0 / SQRT2
+        grads = []
+        for l in range(NL):
+            grads += [dws[l], dbs[l]]
+        return (None, None) + tuple(grads)
+
+
+def fused_sdf(net, x: torch.Tensor, with_normals: bool = True
+              ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """(sdf [M] FP32, features [M,256] BF16, normals d sdf / d x [M,3] FP32) of
+    an SDFNetwork (the RF stages' configuration) for points x [M,3]."""
+    params = []
+    for l in range(NL):
+        lin = getattr(net, f"lin{l}")
+        params += [lin.weight, lin.bias]
+    return _FusedSDF.apply(x, with_normals, *params)
+
+
+@torch.no_grad()
+def fused_sdf_value(net, x: torch.Tensor) -> torch.Tensor:
+    """The SDF alone [M] FP32, no gradients (the calibration's evaluations)."""
+    ws = [getattr(net, f"lin{l}").weight for l in range(NL)]
+    bs = [getattr(net, f"lin{l}").bias.detach().float().contiguous() for l in range(NL)]
+    _check_network(ws)
+    x = x.detach().float().reshape(-1, 3).contiguous()
+    P = [pack(w, False, 1.0 / SQRT2 if l == SKIP else 1.0) for l, w in enumerate(ws)]
+    _, _, _, out8 = _forward(x, ws, bs, P, train=False)
+    return out8[:, 0].float()

@@ -21,8 +21,10 @@ One step (GeRaFStage1.loss, rf_rendering.py:721-820, in its per-ray form):
   6. backward (matched-filter adjoint, tracer adjoint, MLP double-backward)
      and an AdamW step.
 
-``form="fused"`` is the MI355X path (BF16 MLPs on MFMA with split-K weight
-gradients, the hand-written tracer kernels); ``form="fp32_autograd"`` is the
+``form="fused"`` is the MI355X path (the SDF network, its normals and their
+backward as the library's BF16 matrix-core products, the reflectance MLP in
+BF16 with split-K weight gradients, the hand-written tracer kernels);
+``form="fp32_autograd"`` is the
 all-FP32 autograd form of the same step: FP32 nn.Linear MLPs and the
 compositing + signal as torch FP32 ops (the reference's glue style), with the
 same matched filter.
@@ -37,6 +39,7 @@ import numpy as np
 import torch
 import torch.nn.functional as F
 
+from geraf.models.networks.fused_sdf import fused_sdf, fused_sdf_value
 from geraf.models.networks.sdf_network import (ReflectivePowerNetwork, SDFNetwork,
                                                 SingleVarianceNetwork)
 from geraf.models.rendering.rf_stage_ops import (compute_mf_loss, matched_filter_prediction,
@@ -154,7 +157,8 @@ class StageStep:
                  device: torch.device, form: str = "fused", seed: int = 0,
                  mlp_dtype: Optional[torch.dtype] = "auto", mf_algo: str = "auto"):
         """``mlp_dtype``: compute dtype of the MLP GEMMs; "auto" is BF16 for the
-        fused form and FP32 for the autograd form.  ``mf_algo``: "direct" or
+        fused form and FP32 for the autograd form.  A BF16 fused form runs the
+        SDF network on the hand-written kernels (``fused_sdf``).  ``mf_algo``: "direct" or
         "tiled" (the per-tile expansion matched filter, truncation bound 1e-6);
         "auto" is tiled for the fused form and direct for the autograd form."""
         if form not in ("fused", "fp32_autograd"):
@@ -165,6 +169,7 @@ class StageStep:
             mlp_dtype = torch.bfloat16 if form == "fused" else None
         cd = mlp_dtype
         tall = form == "fused"  # the autograd form keeps plain nn.Linear layers
+        self.hand_sdf = form == "fused" and cd == torch.bfloat16
         self.sdf_net = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6, compute_dtype=cd,
                                   tall_products=tall).to(device)
         self.var_net = SingleVarianceNetwork(0.3).to(device)
@@ -219,7 +224,9 @@ class StageStep:
         leaving the centre of the (normalised) antennas (no gradient)."""
         R, S = self.cfg.rays, self.cfg.samples
         inv_s = self.var_net.inv_s().clip(1e-6, 1e6)
-        return transmission_offset(self.sdf_net.sdf_value, inv_s,
+        sdf_fn = ((lambda x: fused_sdf_value(self.sdf_net, x)) if self.hand_sdf
+                  else self.sdf_net.sdf_value)
+        return transmission_offset(sdf_fn, inv_s,
                                    self.pts_n.view(R, S, 3), self.ant_pos_norm)
 
     def parts(self) -> dict:
@@ -233,9 +240,13 @@ class StageStep:
         offset = self.calibration_offset()
         # the features stay in the networks' compute dtype (BF16 in the fused
         # form): the reflectance network takes them as they are
-        out, grad = self.sdf_net.sdf_and_gradient(self.pts_n.clone(), float_out=False)
-        sdf = out[:, 0].float().reshape(R, S)
-        feat = out[:, 1:]
+        if self.hand_sdf:
+            sdf, feat, grad = fused_sdf(self.sdf_net, self.pts_n)
+            sdf = sdf.reshape(R, S)
+        else:
+            out, grad = self.sdf_net.sdf_and_gradient(self.pts_n.clone(), float_out=False)
+            sdf = out[:, 0].float().reshape(R, S)
+            feat = out[:, 1:]
         grad = grad.reshape(R, S, 3)
         inv_s = self.var_net.inv_s().clip(1e-6, 1e6)
         refl = self.sig_net.reflectance(self.pts_n, feat).reshape(R, S)
@@ -281,7 +292,9 @@ class StageStep:
             h = 0.08 / RADIUS
             q = self.cons_pts.abs() - h
             box = q.clamp(min=0).norm(dim=-1) + q.max(dim=-1).values.clamp(max=0)
-            cons = (self.sdf_net.sdf(self.cons_pts)[:, 0] - box).abs().mean()
+            cons_sdf = (fused_sdf(self.sdf_net, self.cons_pts, with_normals=False)[0]
+                        if self.hand_sdf else self.sdf_net.sdf(self.cons_pts)[:, 0])
+            cons = (cons_sdf - box).abs().mean()
             loss = loss + cons
         return dict(sdf=sdf, grad=grad, refl=refl, inv_s=inv_s, power=power, offset=offset,
                     sim_r=sim_r, sim_i=sim_i, ant_r=ant_r, ant_i=ant_i, mf_r=keep["mf"][0],

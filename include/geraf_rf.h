@@ -355,6 +355,74 @@ int geraf_softplus_grad2_bf16(const uint16_t* ggx, const uint16_t* g, const uint
                               uint16_t* out_g, uint16_t* out_x, int64_t n, float beta,
                               float threshold, geraf_stream_t stream);
 
+/* ---------------------------------------------------------------- SDF network products (BF16) */
+
+/* The layer products of the SDF network (sdf_network.py:78-199) on the BF16
+ * matrix cores, for the hand-written forward, normals and backward of
+ * geraf.models.networks.fused_sdf.  No counterpart in the reference's C++:
+ * they replace the network's torch layers (GEMMs and the element-wise work
+ * around them).  BF16 values are raw 16-bit patterns; accumulation is FP32.
+ *
+ * geraf_mlp_pack_weights: W [N,K] FP32 (w[n*ldw + k], or w[k*ldw + n] when
+ * transpose != 0) times scale, rounded to BF16, into the operand order the
+ * products read (geraf_mlp_pack_bytes(N, K) bytes, 16-byte aligned).
+ * N <= 272, K <= 288; geraf_mlp_pack_bytes returns -1 outside that.
+ *
+ * geraf_mlp_gemm_bf16: y[M,N] = epilogue(prologue(x)[M,K] . W^T + bias):
+ *   prologue 0: x; 1: x * s(x2) (s the softplus slope of the stored
+ *     pre-activation x2, element by element);
+ *   epilogue (z = the product + bias, bias may be NULL):
+ *     0: y = z;  1: y = softplus(z);  2: y = softplus(z), y2 = z;
+ *     3: y = relu(z);  4: y = z, y2 = z * e1 * beta * s(e0) (1 - s(e0));
+ *     5: y = z * s(e0) + e1.
+ *   Softplus(beta, threshold) as torch's; where a pre-activation is rounded
+ *   to BF16 (epilogues 1 and 2, and e0 and x2 as stored) the activation and
+ *   its slopes are taken of the rounded value.  Leading dimensions are in
+ *   elements: 0 broadcasts one row, otherwise a multiple of 8 covering the
+ *   columns rounded up to 8 (y, y2, e0, e1 also multiples of 4).  x, x2 and
+ *   wpacked 16-byte aligned, y, y2, e0, e1 8-byte aligned.
+ *
+ * geraf_mlp_wgrad_bf16: dw[N,K] (+)= prologue(g)^T prologue(x) over the M
+ * points (g [M,N], x [M,K], prologues as above), FP32, row stride lddw;
+ * db[N] (+)= the column sums of prologue(g) when db is not NULL; with
+ * accumulate != 0 the results are added to dw / db.  N, K <= 512.  The
+ * workspace (geraf_mlp_wgrad_workspace_bytes, caller-owned) holds per-block
+ * partial sums, added in a fixed order (deterministic).
+ *
+ * geraf_mlp_positional_encoding: out[p*ldo + col0 + i] = gamma(x_p * scale)_i
+ * (i < 3 + 6L: x, then per band l the 3 sines and the 3 cosines of
+ * 2^l x, the Embedder order of sdf_network.py:19-75), BF16; zeros up to
+ * pad_to columns.
+ * geraf_mlp_pe_gradient: n[p] = d sdf / d x from d sdf / d gamma = ga + gb
+ * (two BF16 [M, 3+6L] parts), FP32 [M,3].
+ * geraf_mlp_pe_gradient_adjoint: its transpose, d loss / d gamma = J dn
+ * (BF16, into out at col0, zero-padded up to pad_to). */
+int64_t geraf_mlp_pack_bytes(int64_t N, int64_t K);
+int geraf_mlp_pack_weights(const float* w, int64_t ldw, int32_t transpose, float scale,
+                           int64_t N, int64_t K, void* packed, geraf_stream_t stream);
+int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_t* x2, int64_t ldx2,
+                        const void* wpacked, const float* bias, uint16_t* y, int64_t ldy,
+                        uint16_t* y2, int64_t ldy2, const uint16_t* e0, int64_t lde0,
+                        const uint16_t* e1, int64_t lde1, int64_t M, int64_t N, int64_t K,
+                        int32_t prologue, int32_t epilogue, float beta, float threshold,
+                        geraf_stream_t stream);
+int64_t geraf_mlp_wgrad_workspace_bytes(int64_t M, int64_t N, int64_t K);
+int geraf_mlp_wgrad_bf16(const uint16_t* g, int64_t ldg, const uint16_t* g2, int64_t ldg2,
+                         int32_t prologue_g, const uint16_t* x, int64_t ldx, const uint16_t* x2,
+                         int64_t ldx2, int32_t prologue_x, float* dw, int64_t lddw, float* db,
+                         int32_t accumulate, int64_t M, int64_t N, int64_t K, float beta,
+                         float threshold, void* workspace, int64_t workspace_bytes,
+                         geraf_stream_t stream);
+int geraf_mlp_positional_encoding(const float* x, int64_t M, int32_t L, float scale,
+                                  uint16_t* out, int64_t ldo, int32_t col0, int32_t pad_to,
+                                  geraf_stream_t stream);
+int geraf_mlp_pe_gradient(const float* x, int64_t M, int32_t L, float scale, const uint16_t* ga,
+                          int64_t lda, const uint16_t* gb, int64_t ldb, float* n,
+                          geraf_stream_t stream);
+int geraf_mlp_pe_gradient_adjoint(const float* x, int64_t M, int32_t L, float scale,
+                                  const float* dn, uint16_t* out, int64_t ldo, int32_t col0,
+                                  int32_t pad_to, geraf_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

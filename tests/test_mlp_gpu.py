@@ -1,0 +1,223 @@
+"""The SDF network's BF16 matrix-core kernels on the GPU: the layer product
+(every prologue and epilogue, ragged widths, broadcast rows) and the weight
+gradient against torch on the same BF16-rounded operands; the hand-written
+network (forward, normals, backward with the double-backward terms) against
+the FP64 torch network, within the error of torch's own BF16 network."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+BETA, TH = 100.0, 20.0
+
+
+def _bf(t):
+    return t.to(torch.bfloat16)
+
+
+def _sp(z):
+    return torch.nn.functional.softplus(z, BETA, TH)
+
+
+def _slope(z):
+    return torch.where(z * BETA > TH, torch.ones_like(z), torch.sigmoid(z * BETA))
+
+
+def _curv(z):
+    s = torch.sigmoid(z * BETA)
+    return torch.where(z * BETA > TH, torch.zeros_like(z), s * (1 - s))
+
+
+def _rel(a, b):
+    a, b = a.double(), b.double()
+    return float((a - b).norm() / b.norm().clamp(min=1e-30))
+
+
+@pytest.mark.parametrize("M,N,K", [(1000, 256, 256), (77, 217, 256), (513, 256, 217),
+                                   (300, 257, 256), (65, 256, 257), (129, 256, 39),
+                                   (96, 39, 256), (31, 1, 64)])
+@pytest.mark.parametrize("pro", [0, 1])
+def test_layer_product_epilogues(cuda, M, N, K, pro):
+    from geraf.models.networks import fused_sdf as fs
+
+    g = torch.Generator(device="cpu").manual_seed(M * 7 + N + K + pro)
+    ldx = ((K + 7) // 8) * 8 + 8
+    x = _bf(torch.randn(M, ldx, generator=g) * 0.5).to(cuda)
+    x2 = _bf(torch.randn(M, ldx, generator=g) * 0.02).to(cuda)
+    w = (torch.randn(N, K, generator=g) / math.sqrt(K)).to(cuda)
+    b = (torch.randn(N, generator=g) * 0.01).to(cuda)
+    ldy = ((N + 7) // 8) * 8
+    e0 = _bf(torch.randn(M, ldy, generator=g) * 0.02).to(cuda)
+    e1 = _bf(torch.randn(M, ldy, generator=g)).to(cuda)
+    packed = fs.pack(w, False)
+    xe = x[:, :K].float()
+    if pro:
+        xe = _bf(xe * _slope(x2[:, :K].float())).float()
+    z = xe @ _bf(w).float().t() + b
+    cases = {0: _bf(z), 1: _bf(_sp(_bf(z).float())), 3: _bf(z.clamp(min=0))}
+    for epi, want in cases.items():
+        y = torch.full((M, ldy), float("nan"), dtype=torch.bfloat16, device=cuda)
+        fs.gemm(x, packed, N, K, y, bias=b, x2=x2 if pro else None, prologue=pro, epilogue=epi)
+        err = _rel(y[:, :N].float(), want.float())
+        assert err <= 8e-3, (epi, err)
+    # training softplus: y = softplus(bf16(z)), y2 = bf16(z)
+    y = torch.empty((M, ldy), dtype=torch.bfloat16, device=cuda)
+    y2 = torch.empty_like(y)
+    fs.gemm(x, packed, N, K, y, bias=b, x2=x2 if pro else None, y2=y2, prologue=pro, epilogue=2)
+    assert _rel(y2[:, :N].float(), z) <= 8e-3
+    assert _rel(y[:, :N].float(), _sp(y2[:, :N].float())) <= 4e-3
+    # adjoints (no bias): y = z, y2 = z e1 beta curv(e0); y = z s(e0) + e1
+    fs.gemm(x, packed, N, K, y, x2=x2 if pro else None, y2=y2, e0=e0, e1=e1, prologue=pro,
+            epilogue=4)
+    z0 = xe @ _bf(w).float().t()
+    assert _rel(y[:, :N].float(), z0) <= 8e-3
+    want = z0 * e1[:, :N].float() * BETA * _curv(e0[:, :N].float())
+    assert _rel(y2[:, :N].float(), want) <= 1e-2
+    fs.gemm(x, packed, N, K, y, x2=x2 if pro else None, e0=e0, e1=e1, prologue=pro, epilogue=5)
+    want = z0 * _slope(e0[:, :N].float()) + e1[:, :N].float()
+    assert _rel(y[:, :N].float(), want) <= 8e-3
+
+
+def test_layer_product_transposed_pack_and_broadcast(cuda):
+    from geraf.models.networks import fused_sdf as fs
+
+    g = torch.Generator(device="cpu").manual_seed(5)
+    M, N, K = 700, 256, 217
+    w = torch.randn(K, N, generator=g).to(cuda) / 16  # used as W^T
+    row = _bf(torch.randn(1, 224, generator=g)).to(cuda)
+    x = row.expand(M, 224)
+    x2 = _bf(torch.randn(M, 224, generator=g) * 0.03).to(cuda)
+    y = torch.empty((M, N), dtype=torch.bfloat16, device=cuda)
+    fs.gemm(x, fs.pack(w, True), N, K, y, x2=x2, prologue=1)
+    xe = _bf(row[:, :K].float() * _slope(x2[:, :K].float())).float()
+    assert _rel(y.float(), xe @ _bf(w).float()) <= 8e-3
+
+
+@pytest.mark.parametrize("M,N,K,pro", [(5000, 256, 256, (1, 1)), (777, 257, 256, (0, 0)),
+                                       (333, 217, 256, (1, 0)), (1024, 256, 39, (0, 1)),
+                                       (64, 1, 256, (0, 1)), (100000, 256, 256, (0, 0))])
+def test_weight_gradient(cuda, M, N, K, pro):
+    from geraf.models.networks import fused_sdf as fs
+
+    gen = torch.Generator(device="cpu").manual_seed(M + N + K)
+    ldg, ldx = ((N + 7) // 8) * 8, ((K + 7) // 8) * 8
+    G = _bf(torch.randn(M, ldg, generator=gen)).to(cuda)
+    X = _bf(torch.randn(M, ldx, generator=gen)).to(cuda)
+    G2 = _bf(torch.randn(M, ldg, generator=gen) * 0.02).to(cuda)
+    X2 = _bf(torch.randn(M, ldx, generator=gen) * 0.02).to(cuda)
+    dw = torch.full((N, K), 0.5, device=cuda)
+    db = torch.full((N,), 0.25, device=cuda)
+    fs.wgrad(G, X, N, K, dw, db=db, g2=G2 if pro[0] else None, x2=X2 if pro[1] else None,
+             accumulate=True)
+    Ge, Xe = G[:, :N].float(), X[:, :K].float()
+    if pro[0]:
+        Ge = _bf(Ge * _slope(G2[:, :N].float())).float()
+    if pro[1]:
+        Xe = _bf(Xe * _slope(X2[:, :K].float())).float()
+    want = Ge.double().t() @ Xe.double() + 0.5
+    # FP32 sums of BF16 products; with a prologue the operands are BF16
+    # roundings of FP32 products, which may differ by one ulp from the
+    # emulation's on a few elements (the kernel's slope uses the hardware exp)
+    tol = 5e-5 if any(pro) else 1e-5
+    assert _rel(dw, want) <= tol
+    assert _rel(db, Ge.double().sum(0) + 0.25) <= tol
+    # deterministic
+    dw2 = torch.full((N, K), 0.5, device=cuda)
+    fs.wgrad(G, X, N, K, dw2, g2=G2 if pro[0] else None, x2=X2 if pro[1] else None,
+             accumulate=True)
+    assert torch.equal(dw, dw2)
+
+
+def _nets(cuda, seed=0):
+    from geraf.models.networks.sdf_network import SDFNetwork
+
+    torch.manual_seed(seed)
+    ref = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6).to(cuda)
+    with torch.no_grad():  # move off the exact geometric init (features and skip weights nonzero)
+        for p in ref.parameters():
+            p.add_(0.02 * torch.randn_like(p))
+    bf = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6,
+                    compute_dtype=torch.bfloat16).to(cuda)
+    bf.load_state_dict(ref.state_dict())
+    return ref, bf
+
+
+def test_fused_network_vs_fp64(cuda):
+    """sdf, features, normals and the gradients of a loss through all three
+    with respect to every layer's weights and bias (as the network uses them,
+    after the weight normalisation) against the FP64 network; each error
+    within 1.5x that of torch's BF16 network (split-K products, fused
+    softplus) on the same points, i.e. within BF16 rounding."""
+    import torch.nn.utils.parametrize as parametrize
+
+    from geraf.models.networks.fused_sdf import fused_sdf
+
+    ref, bfnet = _nets(cuda)
+    M = 20000
+    g = torch.Generator(device="cpu").manual_seed(1)
+    x = (torch.rand(M, 3, generator=g) * 1.6 - 0.8).to(cuda)
+    a = torch.randn(M, generator=g).to(cuda)
+    bvec = torch.randn(M, 256, generator=g).to(cuda) * 0.05
+    c = torch.randn(M, 3, generator=g).to(cuda)
+
+    def run(kind):
+        net = ref if kind == "f64" else bfnet
+        if kind == "f64":
+            net.double()
+        with parametrize.cached():
+            layers = [(getattr(net, f"lin{l}").weight, getattr(net, f"lin{l}").bias)
+                      for l in range(9)]
+            if kind == "f64":
+                xx = x.double().clone().requires_grad_(True)
+                out = net(xx)
+                sdf, feat = out[:, 0], out[:, 1:]
+                n = torch.autograd.grad(sdf.sum(), xx, create_graph=True)[0]
+            elif kind == "torch_bf16":
+                out, n = net.sdf_and_gradient(x.clone())
+                sdf, feat = out[:, 0], out[:, 1:]
+            else:
+                sdf, feat, n = fused_sdf(net, x)
+            L = (a.double() * sdf.double()).sum() + (bvec.double() * feat.double()).sum() + \
+                (c.double() * n.double()).sum()
+            grads = torch.autograd.grad(L, [t for wb in layers for t in wb])
+        vals = (sdf.detach().double(), feat.detach().double(), n.detach().double())
+        if kind == "f64":
+            net.float()
+        return vals, [t.double() for t in grads]
+
+    v64, g64 = run("f64")
+    vt, gt = run("torch_bf16")
+    vf, gf = run("fused")
+    for i, name in enumerate(["sdf", "features", "normals"]):
+        et, ef = _rel(vt[i], v64[i]), _rel(vf[i], v64[i])
+        print(f"{name}: fused {ef:.2e}, torch BF16 {et:.2e}", flush=True)
+        assert ef <= 1.5 * et + 1e-4, name
+    worst = 0.0
+    for i, (a64, at, af) in enumerate(zip(g64, gt, gf)):
+        name = f"lin{i // 2}.{'weight' if i % 2 == 0 else 'bias'}"
+        et, ef = _rel(at, a64), _rel(af, a64)
+        worst = max(worst, ef / et)
+        print(f"grad {name}: fused {ef:.2e}, torch BF16 {et:.2e}", flush=True)
+        assert ef <= 1.5 * et + 1e-4, name
+    print(f"worst gradient error ratio fused / torch BF16: {worst:.2f}", flush=True)
+
+
+def test_fused_sdf_value_and_no_normals(cuda):
+    from geraf.models.networks.fused_sdf import fused_sdf, fused_sdf_value
+
+    ref, bfnet = _nets(cuda, seed=3)
+    x = (torch.rand(3001, 3, device=cuda) * 1.6 - 0.8)
+    s1 = fused_sdf_value(bfnet, x)
+    s2, _, n = fused_sdf(bfnet, x, with_normals=False)
+    with torch.no_grad():
+        s64 = ref.double()(x.double())[:, 0]
+    ref.float()
+    assert _rel(s1, s64) <= 1e-2 and torch.equal(s1, s2.detach())
+    assert torch.all(n == 0)
+    # no normals: the backward is the primal adjoint alone
+    bfnet.zero_grad(set_to_none=True)
+    s2.sum().backward()
+    assert all(p.grad is not None and torch.isfinite(p.grad).all() for p in bfnet.parameters())
