@@ -5,35 +5,53 @@
 // weight gradients.
 //
 // Layer products y[M,N] = x[M,K] . W^T (x: points x input features, BF16,
-// row-major; W: [N,K]).  N <= 272 and K <= 288 (the network's widths: 256
+// row-major; W: [N,K]); the outputs are written in groups of 8 columns, so
+// columns N .. 8 ceil(N / 8) - 1 of y (and y2) receive unspecified values.  N <= 272 and K <= 288 (the network's widths: 256
 // hidden, 257 outputs, 39 encoded inputs).  The weights are small and the
 // batch of points is tall, so the kernel is weight-stationary: a workgroup
 // (8 waves, one per CU) copies the whole packed W into LDS once and then
-// streams tiles of 32 points from HBM straight into matrix operands.  On
+// streams tiles of 16 or 32 points from HBM straight into matrix operands,
+// the next tile's loads in flight while the current one is multiplied.  On
 // v_mfma_f32_16x16x32_bf16 the weights are the A operand (16 output
 // features x 32 inputs) and the points the B operand (32 inputs x 16
 // points), so a lane's B fragment is 8 consecutive input features of one
-// point (one 16-byte load) and its accumulator holds 4 consecutive output
-// features of one point (one 8-byte store).  Operand layout (the same as
+// point (one 16-byte load).  The pack permutes W's rows so that the
+// accumulators of output blocks 2m and 2m + 1 hold, in lane (j, q), the 8
+// consecutive features 32 m + 8 q .. + 7 of point j (one 16-byte store; a
+// wave's store writes 64 contiguous bytes of each of 16 rows).  Operand layout (the same as
 // the FP16 form used by the matched filter, checked on the GPU): A[i][d] is
 // element d % 8 of lane i + 16 (d / 8), B[d][j] element d % 8 of lane
 // j + 16 (d / 8), D[4 q + r][j] element r of lane j + 16 q.
 //
-// Prologues (on x): none, or x * s(x2) with s the softplus slope of a stored
-// pre-activation x2 (the gradient pass and its adjoint).  Epilogues (on
-// z = acc + bias): z; softplus(z); softplus(z) with z kept (training);
-// relu(z); z and z * e1 * beta * curv(e0) (the adjoint of the gradient pass,
-// its second-order term); z * s(e0) + e1 (the adjoint of a hidden layer).
-// Softplus follows torch.nn.Softplus(beta, threshold): y = z if beta z >
-// threshold else log1p(exp(beta z)) / beta, slope s = sigmoid(beta z) (1
-// above the threshold), curvature s (1 - s) = e / (1 + e)^2 (0 above).
+// Softplus slope codes.  Every use of a stored pre-activation z in the
+// network's backward is through the softplus slope s = sigmoid(beta z) or
+// its curvature s (1 - s), so the forward stores instead the code
+// q = copysign(m, t), m = sigmoid(-|t|) in (0, 1/2], t = beta z, computed
+// from the FP32 accumulator and kept as FP16 (11 significant bits; m's
+// range stays clear of FP16's, subnormals included, wherever s or s (1 - s)
+// is not negligible).  Decoding is exact arithmetic on q: s = 1 - m for
+// q >= +0, s = m for q <= -0 (the sign bit; -0 is t -> -inf), s (1 - s) =
+// m (1 - m).  Above torch's threshold (t > 20) m < 2.1e-9 rounds to +0:
+// slope 1, curvature 0, as torch.nn.Softplus's linear branch has.
+//
+// Prologues (on x): none, or x * s(q) (the gradient pass and its adjoint).
+// Epilogues (on z = acc + bias): z; softplus(z); softplus(z) with the code
+// of z (training); relu(z); z and z * e1 * beta * s(1-s) (the adjoint of
+// the gradient pass, its second-order term); z * s(e0) + e1 (the adjoint of
+// a hidden layer).  Softplus follows torch.nn.Softplus(beta, threshold): y
+// = z if beta z > threshold else log1p(exp(beta z)) / beta, evaluated as
+// (max(t, 0) + log1p(exp(-|t|))) / beta with the hardware exp2 / log2 /
+// reciprocal (log1p by its series below 2^-6).
 //
 // Weight gradients dW[N,K] = G^T X (G: [M,N], X: [M,K], both tall) with the
 // same optional prologue on either side: split over the points, each
-// workgroup stages 32-point tiles of G and X transposed in LDS and keeps a
-// 256 x 256 block of dW in registers; the partial blocks go to a workspace
-// and a second kernel sums them in a fixed order (deterministic).  The bias
-// gradient (column sums of G) comes from one more product against ones.
+// workgroup stages steps of 32 or 64 points of G and X transposed in LDS
+// (4-point register transposes, 8-byte LDS stores), keeps a 256 x 256 block
+// of dW (plus one 16-row block for N up to 272) in registers and has the
+// next step's loads in flight while it multiplies; the partial blocks go to
+// a workspace and a second kernel sums them in a fixed order
+// (deterministic).  The bias gradient (column sums of G) comes from one
+// more product against ones.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -52,8 +70,9 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGemmWaves = 8;
-constexpr int kMaxNB = 17;   // output feature blocks of 16 (N <= 272)
-constexpr int kPts = 32;     // points per wave tile (two blocks of 16)
+constexpr int kMaxN = 272;   // output features
+constexpr int kMaxK = 288;   // input features
+constexpr int kNbChunk = 8;  // output blocks of 16 per accumulator pass
 
 enum Pro { PRO_NONE = 0, PRO_MUL_SLOPE = 1 };
 enum Epi {
@@ -65,35 +84,60 @@ enum Epi {
   EPI_LAYER_ADJ = 5
 };
 
+// ------------------------------------------------------------------ element-wise math
+
 __device__ __forceinline__ float bf(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// round to nearest even (v_cvt_pk_bf16_f32; NaN stays NaN)
 __device__ __forceinline__ uint16_t to_bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
-struct SpTerms {
-  float y, s, c;  // softplus, slope, curvature / beta
+__device__ __forceinline__ float hf(uint16_t h) {
+  return (float)__builtin_bit_cast(_Float16, h);
+}
+
+__device__ __forceinline__ uint16_t to_hf(float f) {
+  return __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+constexpr float kLog2e = 1.44269504088896341f;
+constexpr float kLn2 = 0.693147180559945309f;
+
+// s from the code q: 1/2 + copysign(1/2 - |q|, q)
+__device__ __forceinline__ float code_slope(uint16_t q) {
+  const float f = hf(q);
+  return 0.5f + copysignf(0.5f - fabsf(f), f);
+}
+
+// s (1 - s) = m (1 - m) from the code q
+__device__ __forceinline__ float code_curv(uint16_t q) {
+  const float m = fabsf(hf(q));
+  return fmaf(-m, m, m);
+}
+
+struct Softplus {
+  float y, q;
 };
 
-__device__ __forceinline__ SpTerms sp_terms(float z, float beta, float threshold) {
+// Above torch's threshold (t > 20) the log1p term is below 2.1e-9, so the
+// linear branch (y = z) and this form agree to within an FP32 ulp of z; the
+// select is left out (it would cost a branch per element).
+template <bool CODE>
+__device__ __forceinline__ Softplus softplus_of(float z, float beta, float inv_beta) {
   const float t = z * beta;
-  if (t > threshold) return SpTerms{z, 1.0f, 0.0f};
-  const float e = __expf(t);
-  const float r = 1.0f / (1.0f + e);
+  const float e = __builtin_amdgcn_exp2f(-fabsf(t) * kLog2e);  // exp(-|t|) in (0, 1]
   const float u = 1.0f + e;
-  const float l1p = (u == 1.0f) ? e : __logf(u) * (e / (u - 1.0f));
-  return SpTerms{l1p / beta, e * r, e * r * r};
+  // log1p(e): e itself below 2^-12 (relative error < 2^-13), else log(1 + e)
+  // (the rounding of 1 + e costs at most 2^-24 / e relative)
+  const float l = e < 0.000244140625f ? e : __builtin_amdgcn_logf(u) * kLn2;
+  Softplus r;
+  r.y = (fmaxf(t, 0.0f) + l) * inv_beta;
+  r.q = CODE ? copysignf(e * __builtin_amdgcn_rcpf(u), t) : 0.0f;
+  return r;
 }
 
-__device__ __forceinline__ float sp_slope(float z, float beta, float threshold) {
-  const float t = z * beta;
-  if (t > threshold) return 1.0f;
-  const float e = __expf(t);
-  return e / (1.0f + e);
-}
+// ------------------------------------------------------------------ layer products
 
 struct GemmArgs {
   const uint16_t* x;
@@ -106,136 +150,206 @@ struct GemmArgs {
   const uint16_t* e1;
   int64_t ldx, ldx2, ldy, ldy2, lde0, lde1;
   int M, N, K, nbN, ksK;
-  float beta, threshold;
+  float beta, inv_beta;
 };
 
-// The B fragment of points p (this lane's point) for input features
-// k0 .. k0 + 7: a 16-byte load, zeros past K or M.
+__device__ __forceinline__ u16x8 zero8() { return u16x8{0, 0, 0, 0, 0, 0, 0, 0}; }
+
+// Raw inputs of the B fragments of one point (its row pointers, already at
+// this lane's 8-feature group), step ks of 32 input features: 16-byte loads
+// of x (and of the code for the prologue).  Steps below ksK - 1 lie inside
+// K; the last step loads only where it starts inside K (rows hold K rounded
+// up to 8) and zeros its features past K; steps past ksK are zero (their W
+// fragments are zero in LDS).
 template <int PRO>
-__device__ __forceinline__ bf8 load_frag(const GemmArgs& a, int p, int k0) {
-  u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (p < a.M && k0 < a.K) {
-    v = *reinterpret_cast<const u16x8*>(a.x + (size_t)p * a.ldx + k0);
-    if (PRO == PRO_MUL_SLOPE) {
-      const u16x8 w = *reinterpret_cast<const u16x8*>(a.x2 + (size_t)p * a.ldx2 + k0);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = to_bf(bf(v[e]) * sp_slope(bf(w[e]), a.beta, a.threshold));
+__device__ __forceinline__ void load_step(const GemmArgs& a, const uint16_t* xp,
+                                          const uint16_t* qp, int ks, int g, u16x8& v, u16x8& q) {
+  v = zero8();
+  if (PRO == PRO_MUL_SLOPE) q = zero8();
+  if (ks < a.ksK - 1) {
+    v = *reinterpret_cast<const u16x8*>(xp + 32 * ks);
+    if (PRO == PRO_MUL_SLOPE) q = *reinterpret_cast<const u16x8*>(qp + 32 * ks);
+  } else if (ks == a.ksK - 1) {
+    if (32 * ks + 8 * g < a.K) {
+      v = *reinterpret_cast<const u16x8*>(xp + 32 * ks);
+      if (PRO == PRO_MUL_SLOPE) q = *reinterpret_cast<const u16x8*>(qp + 32 * ks);
     }
-    if (k0 + 8 > a.K) {
+  }
+}
+
+template <int PRO>
+__device__ __forceinline__ bf8 make_frag(const GemmArgs& a, int ks, int g, u16x8 v, u16x8 q) {
+  if (PRO == PRO_MUL_SLOPE) {
 #pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (k0 + e >= a.K) v[e] = 0;
-    }
+    for (int e = 0; e < 8; ++e) v[e] = to_bf(bf(v[e]) * code_slope(q[e]));
+  }
+  if (ks == a.ksK - 1 && 32 * a.ksK != a.K) {
+    const int k0 = 32 * ks + 8 * g;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (k0 + e >= a.K) v[e] = 0;
   }
   return __builtin_bit_cast(bf8, v);
 }
 
-template <int KS, int PRO, int EPI>
+constexpr bool adj_epi(int epi) { return epi == EPI_GRAD_ADJ || epi == EPI_LAYER_ADJ; }
+
+// 8 consecutive 16-bit values at column f0 of row p (f0 a multiple of 8 and
+// below N; rows hold N rounded up to 8, so the access stays in the row)
+__device__ __forceinline__ u16x8 load8(const uint16_t* base, int64_t ld, int p, int f0) {
+  return *reinterpret_cast<const u16x8*>(base + (size_t)p * ld + f0);
+}
+
+__device__ __forceinline__ void store8(uint16_t* base, int64_t ld, int p, int f0, u16x8 v) {
+  *reinterpret_cast<u16x8*>(base + (size_t)p * ld + f0) = v;
+}
+
+// Output feature of row i of MFMA block nb in the packed W: blocks 2m and
+// 2m + 1 share features 32 m .. 32 m + 31, row 4 q + r of block 2m + h
+// holding feature 32 m + 8 q + 4 h + r.
+__host__ __device__ constexpr int packed_feature(int nb, int i) {
+  return 32 * (nb >> 1) + 8 * (i >> 2) + 4 * (nb & 1) + (i & 3);
+}
+
+// Wave tile: 16 PB points; the workgroup's waves stride over the tiles.
+template <int KS, int PB, int PRO, int EPI>
 __global__ __launch_bounds__(64 * kGemmWaves) void mlp_gemm_kernel(GemmArgs a) {
+  constexpr int kTile = 16 * PB;
+  constexpr bool kAdj = adj_epi(EPI);
+  // W in LDS as [nbN][KS][64] fragments, zero past ksK (so every product
+  // runs KS steps without a guard), then the bias
   extern __shared__ bf8 wl[];
-  const int nw = a.nbN * a.ksK * kWave;
-  for (int i = threadIdx.x; i < nw; i += blockDim.x) wl[i] = a.wpack[i];
+  float* bias_l = reinterpret_cast<float*>(wl + a.nbN * KS * kWave);
+  const int nw = a.nbN * KS * kWave;
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+    const int l = i & (kWave - 1), ks = (i / kWave) % KS, nb = i / (kWave * KS);
+    wl[i] = ks < a.ksK ? a.wpack[(nb * a.ksK + ks) * kWave + l] : __builtin_bit_cast(bf8, zero8());
+  }
+  for (int i = threadIdx.x; i < 16 * (a.nbN + 1); i += blockDim.x)
+    bias_l[i] = (a.bias != nullptr && i < a.N) ? a.bias[i] : 0.0f;
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
-  const int tiles = (a.M + kPts - 1) / kPts;
-  for (int t = blockIdx.x * kGemmWaves + wave; t < tiles; t += gridDim.x * kGemmWaves) {
-    const int p0 = t * kPts;
-    bf8 xf[2][KS];
+  const int tiles = (a.M + kTile - 1) / kTile;
+  const int stride = gridDim.x * kGemmWaves;
+  int t = blockIdx.x * kGemmWaves + wave;
+  u16x8 xr[PB][KS], qr[PB][KS];
+  // rows past M load row M - 1 (their results are not stored)
+  auto load_tile = [&](int tt) {
 #pragma unroll
-    for (int pb = 0; pb < 2; ++pb)
+    for (int pb = 0; pb < PB; ++pb) {
+      const int p = min(tt * kTile + 16 * pb + j, a.M - 1);
+      const uint16_t* xp = a.x + (size_t)p * a.ldx + 8 * g;
+      const uint16_t* qp = PRO == PRO_MUL_SLOPE ? a.x2 + (size_t)p * a.ldx2 + 8 * g : nullptr;
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) xf[pb][ks] = load_frag<PRO>(a, p0 + 16 * pb + j, 32 * ks + 8 * g);
-    f32x4v acc[2][kMaxNB];
-#pragma unroll
-    for (int nb = 0; nb < kMaxNB; ++nb) {
-      acc[0][nb] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
-      acc[1][nb] = acc[0][nb];
+      for (int ks = 0; ks < KS; ++ks) load_step<PRO>(a, xp, qp, ks, g, xr[pb][ks], qr[pb][ks]);
     }
+  };
+  // The next tile's inputs are loaded while this one is multiplied, except
+  // with an adjoint epilogue: its operands take those registers (they are
+  // loaded per chunk of output blocks, ahead of the chunk's products).
+  constexpr bool kPrefetch = !kAdj;
+  if (kPrefetch && t < tiles) load_tile(t);
+  for (; t < tiles; t += stride) {
+    const int p0 = t * kTile;
+    const bool full_tile = p0 + kTile <= a.M;
+    if constexpr (!kPrefetch) load_tile(t);
+    bf8 xf[PB][KS];
 #pragma unroll
-    for (int nb = 0; nb < kMaxNB; ++nb) {
-      if (nb < a.nbN) {
+    for (int pb = 0; pb < PB; ++pb)
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-          const bf8 w = wl[(nb * a.ksK + ks) * kWave + lane];
-          acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xf[0][ks], acc[0][nb], 0, 0, 0);
-          acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xf[1][ks], acc[1][nb], 0, 0, 0);
+      for (int ks = 0; ks < KS; ++ks) xf[pb][ks] = make_frag<PRO>(a, ks, g, xr[pb][ks], qr[pb][ks]);
+    if (kPrefetch && t + stride < tiles) load_tile(t + stride);
+    // chunks of output blocks: a rolled loop (one chunk's code in the
+    // instruction cache); arrays inside are indexed by unrolled inner loops
+#pragma unroll 1
+    for (int ci = 0; ci * kNbChunk < a.nbN; ++ci) {
+      // every block of the chunk inside N and every point inside M: no guards
+      const bool fast = full_tile && 16 * (ci + 1) * kNbChunk <= a.N;
+      // the chunk's epilogue operands (8 features per lane and block pair),
+      // in flight during its products
+      constexpr int kPairs = kNbChunk / 2;
+      u16x8 ea[PB][kAdj ? kPairs : 1], eb[PB][kAdj ? kPairs : 1];
+      if constexpr (kAdj) {
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) {
+          const int p = min(p0 + 16 * pb + j, a.M - 1);
+#pragma unroll
+          for (int pm = 0; pm < kPairs; ++pm) {
+            const int f0 = 16 * ci * kNbChunk + 32 * pm + 8 * g;
+            if (fast || f0 < a.N) {
+              ea[pb][pm] = load8(a.e0, a.lde0, p, f0);
+              eb[pb][pm] = load8(a.e1, a.lde1, p, f0);
+            } else {
+              ea[pb][pm] = zero8();
+              eb[pb][pm] = zero8();
+            }
+          }
         }
       }
-    }
-    // epilogue: lane (j, g) holds features 16 nb + 4 g + r of point p0 + 16 pb + j
+      f32x4v acc[PB][kNbChunk];
 #pragma unroll
-    for (int pb = 0; pb < 2; ++pb) {
-      const int p = p0 + 16 * pb + j;
-      if (p >= a.M) continue;
+      for (int c = 0; c < kNbChunk; ++c)
 #pragma unroll
-      for (int nb = 0; nb < kMaxNB; ++nb) {
-        if (nb >= a.nbN) continue;
-        const int f0 = 16 * nb + 4 * g;
-        if (f0 >= a.N) continue;
-        const bool full = f0 + 4 <= a.N;
-        float z[4];
+        for (int pb = 0; pb < PB; ++pb) acc[pb][c] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+      // blocks past nbN (a partial last chunk) multiply the last block's
+      // weights again; the epilogue stores none of them
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          z[r] = acc[pb][nb][r] + ((a.bias && f0 + r < a.N) ? a.bias[f0 + r] : 0.0f);
-        u16x4 o = {0, 0, 0, 0}, o2 = {0, 0, 0, 0};
-        u16x4 ea = {0, 0, 0, 0}, eb = {0, 0, 0, 0};
-        if (EPI == EPI_GRAD_ADJ || EPI == EPI_LAYER_ADJ) {
-          const uint16_t* pa = a.e0 + (size_t)p * a.lde0 + f0;
-          const uint16_t* pbp = a.e1 + (size_t)p * a.lde1 + f0;
-          if (full) {
-            ea = *reinterpret_cast<const u16x4*>(pa);
-            eb = *reinterpret_cast<const u16x4*>(pbp);
-          } else {
+      for (int c = 0; c < kNbChunk; ++c) {
+        const bf8* wrow = wl + min(ci * kNbChunk + c, a.nbN - 1) * (KS * kWave) + lane;
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (f0 + r < a.N) {
-                ea[r] = pa[r];
-                eb[r] = pbp[r];
-              }
+        for (int ks = 0; ks < KS; ++ks) {
+          const bf8 w = wrow[ks * kWave];
+#pragma unroll
+          for (int pb = 0; pb < PB; ++pb)
+            acc[pb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xf[pb][ks], acc[pb][c], 0, 0, 0);
+        }
+      }
+      // lane (j, g) holds features f0 .. f0 + 7 of point p0 + 16 pb + j in
+      // blocks 2 pm (the first 4) and 2 pm + 1 of the chunk
+      auto emit = [&](int pb, int pm) {
+        const int p = p0 + 16 * pb + j;
+        const int f0 = 16 * ci * kNbChunk + 32 * pm + 8 * g;
+        const f32x4v b0 = *reinterpret_cast<const f32x4v*>(bias_l + f0);
+        const f32x4v b1 = *reinterpret_cast<const f32x4v*>(bias_l + f0 + 4);
+        u16x8 o, o2;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float z = (e < 4 ? acc[pb][2 * pm][e] + b0[e] : acc[pb][2 * pm + 1][e - 4] + b1[e - 4]);
+          if constexpr (EPI == EPI_BIAS) {
+            o[e] = to_bf(z);
+          } else if constexpr (EPI == EPI_SOFTPLUS) {
+            o[e] = to_bf(softplus_of<false>(z, a.beta, a.inv_beta).y);
+          } else if constexpr (EPI == EPI_SOFTPLUS_TRAIN) {
+            const Softplus sp = softplus_of<true>(z, a.beta, a.inv_beta);
+            o[e] = to_bf(sp.y);
+            o2[e] = to_hf(sp.q);
+          } else if constexpr (EPI == EPI_RELU) {
+            o[e] = to_bf(fmaxf(z, 0.0f));
+          } else if constexpr (EPI == EPI_GRAD_ADJ) {
+            // y = vbar (the product), y2 = vbar * u * beta * s (1 - s)
+            o[e] = to_bf(z);
+            o2[e] = to_bf(z * bf(eb[pb][pm][e]) * (a.beta * code_curv(ea[pb][pm][e])));
+          } else {  // EPI_LAYER_ADJ: zbar = hbar * s + t
+            o[e] = to_bf(z * code_slope(ea[pb][pm][e]) + bf(eb[pb][pm][e]));
           }
         }
+        store8(a.y, a.ldy, p, f0, o);
+        if constexpr (EPI == EPI_SOFTPLUS_TRAIN || EPI == EPI_GRAD_ADJ) store8(a.y2, a.ldy2, p, f0, o2);
+      };
+      if (fast) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (EPI == EPI_BIAS) {
-            o[r] = to_bf(z[r]);
-          } else if (EPI == EPI_SOFTPLUS) {
-            o[r] = to_bf(sp_terms(bf(to_bf(z[r])), a.beta, a.threshold).y);
-          } else if (EPI == EPI_SOFTPLUS_TRAIN) {
-            o2[r] = to_bf(z[r]);
-            // the activation of the stored (rounded) pre-activation, as the
-            // backward sees it
-            o[r] = to_bf(sp_terms(bf(o2[r]), a.beta, a.threshold).y);
-          } else if (EPI == EPI_RELU) {
-            o[r] = to_bf(fmaxf(z[r], 0.0f));
-          } else if (EPI == EPI_GRAD_ADJ) {
-            // y = vbar (the product), y2 = vbar * u * beta * curv(z_l)
-            o[r] = to_bf(z[r]);
-            const SpTerms st = sp_terms(bf(ea[r]), a.beta, a.threshold);
-            o2[r] = to_bf(z[r] * bf(eb[r]) * a.beta * st.c);
-          } else {  // EPI_LAYER_ADJ: zbar = hbar * s(z) + t
-            const float s = sp_slope(bf(ea[r]), a.beta, a.threshold);
-            o[r] = to_bf(fmaf(z[r], s, bf(eb[r])));
-          }
-        }
-        uint16_t* py = a.y + (size_t)p * a.ldy + f0;
-        if (full) {
-          *reinterpret_cast<u16x4*>(py) = o;
-        } else {
+        for (int pb = 0; pb < PB; ++pb)
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (f0 + r < a.N) py[r] = o[r];
-        }
-        if (EPI == EPI_SOFTPLUS_TRAIN || EPI == EPI_GRAD_ADJ) {
-          uint16_t* py2 = a.y2 + (size_t)p * a.ldy2 + f0;
-          if (full) {
-            *reinterpret_cast<u16x4*>(py2) = o2;
-          } else {
+          for (int pm = 0; pm < kPairs; ++pm) emit(pb, pm);
+      } else {
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (f0 + r < a.N) py2[r] = o2[r];
-          }
+        for (int pb = 0; pb < PB; ++pb) {
+          if (p0 + 16 * pb + j >= a.M) continue;
+#pragma unroll
+          for (int pm = 0; pm < kPairs; ++pm)
+            if (16 * ci * kNbChunk + 32 * pm + 8 * g < a.N) emit(pb, pm);
         }
       }
     }
@@ -243,8 +357,8 @@ __global__ __launch_bounds__(64 * kGemmWaves) void mlp_gemm_kernel(GemmArgs a) {
 }
 
 // W [N,K] (FP32, w[n * ldw + k], or w[k * ldw + n] with transpose) times
-// `scale` -> the A operands in LDS order: [nb][ks][lane] of 8 BF16, zero
-// padded to 16 nb x 32 ks.
+// `scale` -> the A operands in LDS order: [nb][ks][lane] of 8 BF16, rows in
+// packed_feature order, zero padded to 16 nb x 32 ks.
 __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__ w, int64_t ldw,
                                                        int transpose, float scale, int N, int K,
                                                        int ksK, u16x8* __restrict__ out,
@@ -254,7 +368,7 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__
   const int lane = idx & 63;
   const int ks = (idx >> 6) % ksK;
   const int nb = (idx >> 6) / ksK;
-  const int n = 16 * nb + (lane & 15);
+  const int n = packed_feature(nb, lane & 15);
   const int k0 = 32 * ks + 8 * (lane >> 4);
   u16x8 v;
 #pragma unroll
@@ -270,58 +384,80 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__
 // ------------------------------------------------------------------ weight gradients
 
 constexpr int kWgWaves = 8;
-constexpr int kWgTile = 256;        // dW block per workgroup: 256 x 256
-constexpr int kWgStride = 40;       // LDS row stride of the transposed tiles (32 points + pad)
+constexpr int kWgTile = 256;          // dW block per workgroup: 256 (+ 16) x 256
+constexpr int kWgRows = kWgTile + 16;  // rows of a partial block (N up to 272)
+constexpr int kWgCols = kWgTile + 1;   // columns of a partial block (the last: the bias)
+constexpr int kWgMaxBlocks = 256;      // point chunks (partial blocks) per launch
 
 struct WgradArgs {
   const uint16_t* g;
   const uint16_t* g2;
   const uint16_t* x;
   const uint16_t* x2;
-  float* partial;  // [gridDim.x][N_t][K_t + 1] for this (n, k) super-tile
+  float* partial;  // [gridDim.x][kWgRows][kWgCols] for this (n, k) super-tile
   int64_t ldg, ldg2, ldx, ldx2;
   int M, N, K;
   int with_bias;
   int pts_per_block;
-  float beta, threshold;
 };
 
-// A 32-point tile of a [M, C] tensor (columns c0 .. c0 + 255), transposed into
-// LDS: t[c][p] (row stride kWgStride), with the optional slope prologue.
-template <int PRO>
-__device__ __forceinline__ void stage_tile(const uint16_t* src, int64_t ld, const uint16_t* src2,
-                                           int64_t ld2, int M, int C, int c0, int p0,
-                                           uint16_t* t, float beta, float threshold) {
-  // 32 points x 32 column groups of 8 = 1024 loads of 16 bytes, two per thread
-  for (int q = threadIdx.x; q < kPts * (kWgTile / 8); q += blockDim.x) {
-    const int p = q / (kWgTile / 8);
-    const int cg = q % (kWgTile / 8);
-    const int c = c0 + 8 * cg;
-    u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (p0 + p < M && c < C) {
-      v = *reinterpret_cast<const u16x8*>(src + (size_t)(p0 + p) * ld + c);
-      if (PRO == PRO_MUL_SLOPE) {
-        const u16x8 w = *reinterpret_cast<const u16x8*>(src2 + (size_t)(p0 + p) * ld2 + c);
+// One staging task: 4 consecutive points x 8 consecutive columns of a
+// [M, C] tensor (16-byte loads; zeros past pend or C).
+template <bool PRO>
+__device__ __forceinline__ void load_task(const uint16_t* src, int64_t ld, const uint16_t* src2,
+                                          int64_t ld2, int pend, int C, int p, int c, u16x8 (&r)[4],
+                                          u16x8 (&q)[4]) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = to_bf(bf(v[e]) * sp_slope(bf(w[e]), beta, threshold));
-      }
-      if (c + 8 > C) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-          if (c + e >= C) v[e] = 0;
-      }
+  for (int i = 0; i < 4; ++i) {
+    r[i] = zero8();
+    if (PRO) q[i] = zero8();
+    if (p + i < pend && c < C) {
+      r[i] = *reinterpret_cast<const u16x8*>(src + (size_t)(p + i) * ld + c);
+      if (PRO) q[i] = *reinterpret_cast<const u16x8*>(src2 + (size_t)(p + i) * ld2 + c);
     }
+  }
+}
+
+// The task's prologue, then its transpose into t[column][point] (row stride S
+// halfwords): one 8-byte LDS store per column.
+template <bool PRO, int S>
+__device__ __forceinline__ void store_task(uint16_t* t, int C, int c, int col, int pt,
+                                           u16x8 (&r)[4], const u16x8 (&q)[4]) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) t[(8 * cg + e) * kWgStride + p] = v[e];
+  for (int i = 0; i < 4; ++i) {
+    if (PRO) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) r[i][e] = to_bf(bf(r[i][e]) * code_slope(q[i][e]));
+    }
+    if (c + 8 > C) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (c + e >= C) r[i][e] = 0;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const u16x4 v = {r[0][e], r[1][e], r[2][e], r[3][e]};
+    *reinterpret_cast<u16x4*>(t + (col + e) * S + pt) = v;
   }
 }
 
 // Workgroup: 8 waves in 4 (n) x 2 (k); wave tile 64 (n) x 128 (k) of a
-// 256 x 256 block of dW (the host offsets g and x to the block's columns).
-template <int PRO_G, int PRO_X>
+// 256 x 256 block of dW (the host offsets g and x to the block's columns);
+// with EXTRA, rows 256 .. 271 as well (each wave 16 x 32 of them).  STEP
+// points are staged per synchronisation: 64 for plain operands (each thread
+// loads one task of G and one of X), 32 with a prologue or EXTRA (waves 0-3
+// load G, waves 4-7 X; with EXTRA 16 threads of wave 4 also the extra rows).
+template <int PRO_G, int PRO_X, int STEP, bool EXTRA>
 __global__ __launch_bounds__(64 * kWgWaves) void mlp_wgrad_kernel(WgradArgs a) {
-  __shared__ uint16_t gt[kWgTile * kWgStride];
-  __shared__ uint16_t xt[kWgTile * kWgStride];
+  static_assert(STEP == 32 || STEP == 64, "staging step");
+  static_assert(STEP == 32 || (PRO_G == PRO_NONE && PRO_X == PRO_NONE && !EXTRA),
+                "64-point steps stage plain operands only");
+  constexpr int S = STEP + 8;  // LDS row stride (halfwords)
+  constexpr int kRowsG = EXTRA ? kWgRows : kWgTile;
+  extern __shared__ uint16_t wg_lds[];
+  uint16_t* gt = wg_lds;               // [kRowsG][S]: G^T
+  uint16_t* xt = wg_lds + kRowsG * S;  // [256][S]: X^T
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -329,70 +465,133 @@ __global__ __launch_bounds__(64 * kWgWaves) void mlp_wgrad_kernel(WgradArgs a) {
   const int pbeg = blockIdx.x * a.pts_per_block;
   const int pend = min(a.M, pbeg + a.pts_per_block);
   const bool bias_col = a.with_bias && wk == 0;
+  // staging tasks of this thread
+  const bool g_task = STEP == 64 || wave < 4;
+  int tp, tc;  // task's first point (in the step) and first column
+  if (STEP == 64) {
+    tp = 4 * (lane & 15);
+    tc = 8 * (4 * wave + (lane >> 4));
+  } else {
+    tp = 4 * (lane & 7);
+    tc = 8 * (8 * (wave & 3) + (lane >> 3));
+  }
+  const bool x_extra = EXTRA && wave == 4 && lane < 16;  // rows 256 .. 271 of G^T
+  const int xp = 4 * (lane & 7), xc = kWgTile + 8 * (lane >> 3);
+  u16x8 r0[4], q0[4], r1[4], q1[4];
+  auto load_step = [&](int p0) {
+    if (STEP == 64) {
+      load_task<false>(a.g, a.ldg, nullptr, 0, pend, a.N, p0 + tp, tc, r0, q0);
+      load_task<false>(a.x, a.ldx, nullptr, 0, pend, a.K, p0 + tp, tc, r1, q1);
+    } else {
+      if (g_task)
+        load_task<PRO_G == PRO_MUL_SLOPE>(a.g, a.ldg, a.g2, a.ldg2, pend, a.N, p0 + tp, tc, r0, q0);
+      else
+        load_task<PRO_X == PRO_MUL_SLOPE>(a.x, a.ldx, a.x2, a.ldx2, pend, a.K, p0 + tp, tc, r0, q0);
+      if (EXTRA && x_extra) load_task<false>(a.g, a.ldg, nullptr, 0, pend, a.N, p0 + xp, xc, r1, q1);
+    }
+  };
   f32x4v acc[4][8];
   f32x4v bacc[4];
+  f32x4v accx[2], baccx;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     bacc[i] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) acc[i][kk] = bacc[i];
   }
+  accx[0] = accx[1] = baccx = bacc[0];
   u16x8 ones_u;
 #pragma unroll
   for (int e = 0; e < 8; ++e) ones_u[e] = 0x3f80;  // 1.0 in BF16
   const bf8 ones = __builtin_bit_cast(bf8, ones_u);
-  for (int p0 = pbeg; p0 < pend; p0 += kPts) {
+  if (pbeg < pend) load_step(pbeg);
+  for (int p0 = pbeg; p0 < pend; p0 += STEP) {
+    __syncthreads();  // the previous step's products are done with the tiles
+    if (STEP == 64) {
+      store_task<false, S>(gt, a.N, tc, tc, tp, r0, q0);
+      store_task<false, S>(xt, a.K, tc, tc, tp, r1, q1);
+    } else {
+      if (g_task)
+        store_task<PRO_G == PRO_MUL_SLOPE, S>(gt, a.N, tc, tc, tp, r0, q0);
+      else
+        store_task<PRO_X == PRO_MUL_SLOPE, S>(xt, a.K, tc, tc, tp, r0, q0);
+      if (EXTRA && x_extra) store_task<false, S>(gt, a.N, xc, xc, xp, r1, q1);
+    }
     __syncthreads();
-    stage_tile<PRO_G>(a.g, a.ldg, a.g2, a.ldg2, pend, a.N, 0, p0, gt, a.beta, a.threshold);
-    stage_tile<PRO_X>(a.x, a.ldx, a.x2, a.ldx2, pend, a.K, 0, p0, xt, a.beta, a.threshold);
-    __syncthreads();
-    // A = G^T rows (n), depth = the 32 points; B = X (points x k)
-    bf8 af[4], bfr[8];
+    if (p0 + STEP < pend) load_step(p0 + STEP);
+    // A = G^T rows (n), depth = 32 points; B = X (points x k)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      af[i] = *reinterpret_cast<const bf8*>(gt + (64 * wn + 16 * i + j) * kWgStride + 8 * g);
+    for (int ks = 0; ks < STEP / 32; ++ks) {
+      bf8 af[4], bfr[8];
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk)
-      bfr[kk] = *reinterpret_cast<const bf8*>(xt + (128 * wk + 16 * kk + j) * kWgStride + 8 * g);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < 4; ++i)
+        af[i] = *reinterpret_cast<const bf8*>(gt + (64 * wn + 16 * i + j) * S + 32 * ks + 8 * g);
 #pragma unroll
       for (int kk = 0; kk < 8; ++kk)
-        acc[i][kk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[kk], acc[i][kk], 0, 0, 0);
-      if (bias_col)
-        bacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], ones, bacc[i], 0, 0, 0);
+        bfr[kk] =
+            *reinterpret_cast<const bf8*>(xt + (128 * wk + 16 * kk + j) * S + 32 * ks + 8 * g);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk)
+          acc[i][kk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[kk], acc[i][kk], 0, 0, 0);
+        if (bias_col)
+          bacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], ones, bacc[i], 0, 0, 0);
+      }
+      if (EXTRA) {
+        const bf8 ax = *reinterpret_cast<const bf8*>(gt + (kWgTile + j) * S + 32 * ks + 8 * g);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const bf8 bx = *reinterpret_cast<const bf8*>(
+              xt + (128 * wk + 16 * (2 * wn + h) + j) * S + 32 * ks + 8 * g);
+          accx[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax, bx, accx[h], 0, 0, 0);
+        }
+        if (a.with_bias && wave == 4)
+          baccx = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax, ones, baccx, 0, 0, 0);
+      }
     }
   }
-  // partial block: [N_t = 256][K_t + 1 = 257] (the last column: the bias gradient)
-  float* out = a.partial + (size_t)blockIdx.x * kWgTile * (kWgTile + 1);
+  // partial block [kWgRows][kWgCols] (the last column: the bias gradient)
+  float* out = a.partial + (size_t)blockIdx.x * kWgRows * kWgCols;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n = 64 * wn + 16 * i + 4 * g + r;
 #pragma unroll
-      for (int kk = 0; kk < 8; ++kk) out[(size_t)n * (kWgTile + 1) + 128 * wk + 16 * kk + j] =
-          acc[i][kk][r];
-      if (bias_col && j == 0) out[(size_t)n * (kWgTile + 1) + kWgTile] = bacc[i][r];
+      for (int kk = 0; kk < 8; ++kk)
+        out[(size_t)n * kWgCols + 128 * wk + 16 * kk + j] = acc[i][kk][r];
+      if (bias_col && j == 0) out[(size_t)n * kWgCols + kWgTile] = bacc[i][r];
+    }
+  }
+  if (EXTRA) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = kWgTile + 4 * g + r;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        out[(size_t)n * kWgCols + 128 * wk + 16 * (2 * wn + h) + j] = accx[h][r];
+      if (a.with_bias && wave == 4 && j == 0) out[(size_t)n * kWgCols + kWgTile] = baccx[r];
     }
   }
 }
 
 // dW[n][k] (+)= sum over the partial blocks in order; db[n] likewise.
 __global__ __launch_bounds__(256) void mlp_wgrad_reduce_kernel(const float* __restrict__ partial,
-                                                               int blocks, int n0, int k0, int N,
-                                                               int K, float* __restrict__ dw,
-                                                               int64_t lddw, float* __restrict__ db,
+                                                               int blocks, int rows, int n0,
+                                                               int k0, int N, int K,
+                                                               float* __restrict__ dw,
+                                                               int64_t lddw,
+                                                               float* __restrict__ db,
                                                                int accumulate) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int cols = kWgTile + 1;
-  if (idx >= kWgTile * cols) return;
-  const int n = idx / cols, c = idx % cols;
+  if (idx >= rows * kWgCols) return;
+  const int n = idx / kWgCols, c = idx % kWgCols;
   const bool is_bias = c == kWgTile;
   if (n0 + n >= N) return;
   if (is_bias ? (db == nullptr || k0 != 0) : (k0 + c >= K)) return;
   double s = 0.0;
-  for (int b = 0; b < blocks; ++b) s += (double)partial[(size_t)b * kWgTile * cols + idx];
+  for (int b = 0; b < blocks; ++b) s += (double)partial[(size_t)b * kWgRows * kWgCols + idx];
   if (is_bias) {
     db[n0 + n] = (float)(accumulate ? (double)db[n0 + n] + s : s);
   } else {
@@ -402,52 +601,86 @@ __global__ __launch_bounds__(256) void mlp_wgrad_reduce_kernel(const float* __re
 }
 
 // ------------------------------------------------------------------ positional encoding
+//
+// A workgroup takes 256 points (128 for the gradient): each thread forms its
+// point's row in LDS, then the workgroup moves the rows between LDS and
+// HBM with consecutive threads on consecutive columns (coalesced 2-byte
+// accesses, any column offset and stride).
+
+constexpr int kPePts = 256;
+constexpr int kPeGradPts = 128;  // the gradient's FP32 rows
+constexpr int kPeMaxCols = 64;
+constexpr int kPeStride = kPeMaxCols + 1;
 
 // gamma(x) = [x, sin(2^l x), cos(2^l x) for l < L] (x scaled), BF16, into
 // out[p * ldo + c0 + i] for i < 3 + 6 L (the band order of sdf_network.py's
 // Embedder: per band the 3 sines, then the 3 cosines); columns up to
 // pad_to are zero-filled.
-__global__ __launch_bounds__(256) void pe_kernel(const float* __restrict__ x, int M, int L,
-                                                 float scale, uint16_t* __restrict__ out,
-                                                 int64_t ldo, int c0, int pad_to) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M) return;
-  float v[3];
+__global__ __launch_bounds__(kPePts) void pe_kernel(const float* __restrict__ x, int M, int L,
+                                                    float scale, uint16_t* __restrict__ out,
+                                                    int64_t ldo, int c0, int pad_to) {
+  __shared__ uint16_t rows[kPePts * kPeStride];
+  const int pbase = blockIdx.x * kPePts;
+  const int p = pbase + threadIdx.x;
+  uint16_t* row = rows + threadIdx.x * kPeStride;
+  if (p < M) {
+    float v[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) v[d] = x[3 * (size_t)p + d] * scale;
-  uint16_t* o = out + (size_t)p * ldo + c0;
-  for (int d = 0; d < 3; ++d) o[d] = to_bf(v[d]);
-  for (int l = 0; l < L; ++l) {
-    const float f = (float)(1 << l);
-    for (int d = 0; d < 3; ++d) {
-      const float t = v[d] * f;
-      o[3 + 6 * l + d] = to_bf(sinf(t));
-      o[3 + 6 * l + 3 + d] = to_bf(cosf(t));
+    for (int d = 0; d < 3; ++d) v[d] = x[3 * (size_t)p + d] * scale;
+    for (int d = 0; d < 3; ++d) row[d] = to_bf(v[d]);
+    for (int l = 0; l < L; ++l) {
+      const float f = (float)(1 << l);
+      for (int d = 0; d < 3; ++d) {
+        float sn, cs;
+        sincosf(v[d] * f, &sn, &cs);
+        row[3 + 6 * l + d] = to_bf(sn);
+        row[3 + 6 * l + 3 + d] = to_bf(cs);
+      }
     }
+    for (int c = 3 + 6 * L; c < pad_to; ++c) row[c] = 0;
   }
-  for (int c = 3 + 6 * L; c < pad_to; ++c) o[c] = 0;
+  __syncthreads();
+  const int np = min(kPePts, M - pbase);
+  for (int i = threadIdx.x; i < np * pad_to; i += kPePts) {
+    const int q = i / pad_to, c = i - q * pad_to;
+    out[(size_t)(pbase + q) * ldo + c0 + c] = rows[q * kPeStride + c];
+  }
 }
 
 // n = d sdf / d x from d sdf / d gamma (the sum of two BF16 parts, [M, 3+6L]
 // each at their own stride): n_d = scale * (g[d] + sum_l 2^l (cos(2^l x_d)
 // g[3+6l+d] - sin(2^l x_d) g[3+6l+3+d])), FP32.
-__global__ __launch_bounds__(256) void pe_grad_kernel(const float* __restrict__ x, int M, int L,
-                                                      float scale, const uint16_t* __restrict__ ga,
-                                                      int64_t lda, const uint16_t* __restrict__ gb,
-                                                      int64_t ldb, float* __restrict__ n) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kPeGradPts) void pe_grad_kernel(const float* __restrict__ x, int M, int L,
+                                                         float scale,
+                                                         const uint16_t* __restrict__ ga,
+                                                         int64_t lda,
+                                                         const uint16_t* __restrict__ gb,
+                                                         int64_t ldb, float* __restrict__ n) {
+  __shared__ float rows[kPeGradPts * kPeStride];
+  const int pbase = blockIdx.x * kPeGradPts;
+  const int np = min(kPeGradPts, M - pbase);
+  const int nc = 3 + 6 * L;
+  for (int i = threadIdx.x; i < np * nc; i += kPeGradPts) {
+    const int q = i / nc, c = i - q * nc;
+    rows[q * kPeStride + c] = bf(ga[(size_t)(pbase + q) * lda + c]);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < np * nc; i += kPeGradPts) {
+    const int q = i / nc, c = i - q * nc;
+    rows[q * kPeStride + c] += bf(gb[(size_t)(pbase + q) * ldb + c]);
+  }
+  __syncthreads();
+  const int p = pbase + threadIdx.x;
   if (p >= M) return;
-  const uint16_t* a = ga + (size_t)p * lda;
-  const uint16_t* b = gb + (size_t)p * ldb;
+  const float* row = rows + threadIdx.x * kPeStride;
   for (int d = 0; d < 3; ++d) {
     const float v = x[3 * (size_t)p + d] * scale;
-    float s = bf(a[d]) + bf(b[d]);
+    float s = row[d];
     for (int l = 0; l < L; ++l) {
       const float f = (float)(1 << l);
-      const float t = v * f;
-      const float gs = bf(a[3 + 6 * l + d]) + bf(b[3 + 6 * l + d]);
-      const float gc = bf(a[3 + 6 * l + 3 + d]) + bf(b[3 + 6 * l + 3 + d]);
-      s += f * (cosf(t) * gs - sinf(t) * gc);
+      float sn, cs;
+      sincosf(v * f, &sn, &cs);
+      s += f * (cs * row[3 + 6 * l + d] - sn * row[3 + 6 * l + 3 + d]);
     }
     n[3 * (size_t)p + d] = s * scale;
   }
@@ -456,27 +689,39 @@ __global__ __launch_bounds__(256) void pe_grad_kernel(const float* __restrict__ 
 // The adjoint of pe_grad for d loss / d n: d loss / d gamma = J dn with J the
 // encoding's Jacobian (scaled), BF16, into out[p * ldo + c0 + i], zero-padded
 // up to pad_to columns.
-__global__ __launch_bounds__(256) void pe_grad_adj_kernel(const float* __restrict__ x, int M,
-                                                          int L, float scale,
-                                                          const float* __restrict__ dn,
-                                                          uint16_t* __restrict__ out,
-                                                          int64_t ldo, int c0, int pad_to) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M) return;
-  uint16_t* o = out + (size_t)p * ldo + c0;
-  for (int d = 0; d < 3; ++d) {
-    const float v = x[3 * (size_t)p + d] * scale;
-    const float gd = dn[3 * (size_t)p + d] * scale;
-    o[d] = to_bf(gd);
-    for (int l = 0; l < L; ++l) {
-      const float f = (float)(1 << l);
-      const float t = v * f;
-      o[3 + 6 * l + d] = to_bf(f * cosf(t) * gd);
-      o[3 + 6 * l + 3 + d] = to_bf(-f * sinf(t) * gd);
+__global__ __launch_bounds__(kPePts) void pe_grad_adj_kernel(const float* __restrict__ x, int M,
+                                                             int L, float scale,
+                                                             const float* __restrict__ dn,
+                                                             uint16_t* __restrict__ out,
+                                                             int64_t ldo, int c0, int pad_to) {
+  __shared__ uint16_t rows[kPePts * kPeStride];
+  const int pbase = blockIdx.x * kPePts;
+  const int p = pbase + threadIdx.x;
+  uint16_t* row = rows + threadIdx.x * kPeStride;
+  if (p < M) {
+    for (int d = 0; d < 3; ++d) {
+      const float v = x[3 * (size_t)p + d] * scale;
+      const float gd = dn[3 * (size_t)p + d] * scale;
+      row[d] = to_bf(gd);
+      for (int l = 0; l < L; ++l) {
+        const float f = (float)(1 << l);
+        float sn, cs;
+        sincosf(v * f, &sn, &cs);
+        row[3 + 6 * l + d] = to_bf(f * cs * gd);
+        row[3 + 6 * l + 3 + d] = to_bf(-f * sn * gd);
+      }
     }
+    for (int c = 3 + 6 * L; c < pad_to; ++c) row[c] = 0;
   }
-  for (int c = 3 + 6 * L; c < pad_to; ++c) o[c] = 0;
+  __syncthreads();
+  const int np = min(kPePts, M - pbase);
+  for (int i = threadIdx.x; i < np * pad_to; i += kPePts) {
+    const int q = i / pad_to, c = i - q * pad_to;
+    out[(size_t)(pbase + q) * ldo + c0 + c] = rows[q * kPeStride + c];
+  }
 }
+
+// ------------------------------------------------------------------ host side
 
 // ld == 0 broadcasts one row to every point; otherwise rows are 16-byte
 // aligned and hold the columns rounded up to 8 (the fragments' width).
@@ -499,9 +744,18 @@ void allow_lds(F* kernel, size_t bytes) {
 }
 
 int ks_of(int64_t K) { return (int)((K + 31) / 32); }
-int nb_of(int64_t N) { return (int)((N + 15) / 16); }
+// output blocks of 16: pairs of blocks cover 32 features (packed_feature)
+int nb_of(int64_t N) { return (int)(2 * ((N + 31) / 32)); }
 
-int gemm_grid(int64_t M) {
+// LDS of the layer product: W padded to the kernel's KS steps, and the bias
+size_t gemm_lds_bytes(int64_t N, int64_t K) {
+  const int ksK = ks_of(K);
+  const int ks_kernel = ksK <= 2 ? 2 : (ksK <= 8 ? 8 : 9);
+  return (size_t)nb_of(N) * ks_kernel * kWave * 16 + (size_t)16 * (nb_of(N) + 1) * sizeof(float);
+}
+constexpr size_t kLdsBytes = 160 * 1024;
+
+int cu_count() {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
     int v = 0;
@@ -509,20 +763,28 @@ int gemm_grid(int64_t M) {
         v > 0)
       cus = v;
   }
-  const int64_t tiles = (M + kPts - 1) / kPts;
-  const int64_t want = (tiles + kGemmWaves - 1) / kGemmWaves;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(want, cus));
+  return cus;
 }
+
+// Points per wave tile: 16 where the prologue's codes or the epilogue's
+// operands take the registers, 32 otherwise (twice the reuse of each
+// weight fragment read from LDS).
+constexpr int gemm_pb(int pro, int epi) { return (pro == PRO_MUL_SLOPE || adj_epi(epi)) ? 1 : 2; }
 
 template <int KS, int PRO>
 int launch_gemm_epi(int epi, const GemmArgs& a, size_t lds, hipStream_t st) {
-  const dim3 grid(gemm_grid(a.M)), block(64 * kGemmWaves);
+  const dim3 block(64 * kGemmWaves);
   switch (epi) {
-#define GERAF_EPI(E)                                                               \
-  case E:                                                                          \
-    allow_lds(mlp_gemm_kernel<KS, PRO, E>, lds);                                   \
-    hipLaunchKernelGGL((mlp_gemm_kernel<KS, PRO, E>), grid, block, lds, st, a);    \
-    break;
+#define GERAF_EPI(E)                                                                        \
+  case E: {                                                                                 \
+    constexpr int PB = gemm_pb(PRO, E);                                                     \
+    const int64_t tiles = (a.M + 16 * PB - 1) / (16 * PB);                                  \
+    const dim3 grid((unsigned)std::max<int64_t>(                                            \
+        1, std::min<int64_t>((tiles + kGemmWaves - 1) / kGemmWaves, cu_count())));          \
+    allow_lds(mlp_gemm_kernel<KS, PB, PRO, E>, lds);                                        \
+    hipLaunchKernelGGL((mlp_gemm_kernel<KS, PB, PRO, E>), grid, block, lds, st, a);         \
+    break;                                                                                  \
+  }
     GERAF_EPI(EPI_BIAS)
     GERAF_EPI(EPI_SOFTPLUS)
     GERAF_EPI(EPI_SOFTPLUS_TRAIN)
@@ -543,20 +805,31 @@ int launch_gemm(int epi, const GemmArgs& a, size_t lds, hipStream_t st) {
   return launch_gemm_epi<9, PRO>(epi, a, lds, st);
 }
 
+template <int PG, int PX, int STEP, bool EXTRA>
+int launch_wgrad(const WgradArgs& b, int blocks, hipStream_t st) {
+  constexpr int S = STEP + 8;
+  const size_t lds = (size_t)((EXTRA ? kWgRows : kWgTile) + kWgTile) * S * sizeof(uint16_t);
+  allow_lds(mlp_wgrad_kernel<PG, PX, STEP, EXTRA>, lds);
+  hipLaunchKernelGGL((mlp_wgrad_kernel<PG, PX, STEP, EXTRA>), dim3((unsigned)blocks),
+                     dim3(64 * kWgWaves), lds, st, b);
+  return check_launch("mlp_wgrad_kernel");
+}
+
 }  // namespace
 }  // namespace geraf
 
 using namespace geraf;
 
 extern "C" int64_t geraf_mlp_pack_bytes(int64_t N, int64_t K) {
-  if (N <= 0 || K <= 0 || N > 16 * kMaxNB || K > 32 * 9) return -1;
+  if (N <= 0 || K <= 0 || N > kMaxN || K > kMaxK || gemm_lds_bytes(N, K) > kLdsBytes) return -1;
   return (int64_t)nb_of(N) * ks_of(K) * kWave * 16;
 }
 
 extern "C" int geraf_mlp_pack_weights(const float* w, int64_t ldw, int32_t transpose, float scale,
                                       int64_t N, int64_t K, void* packed, geraf_stream_t stream) {
   GERAF_TRY_BEGIN
-  if (geraf_mlp_pack_bytes(N, K) < 0) throw ArgError("N must be in [1, 272] and K in [1, 288]");
+  if (geraf_mlp_pack_bytes(N, K) < 0)
+    throw ArgError("N must be in [1, 272] and K in [1, 288], N > 256 with K <= 256");
   require_ptrs({w, packed}, true);
   if ((reinterpret_cast<uintptr_t>(packed) & 15u) != 0) throw ArgError("packed must be 16-byte aligned");
   const int total = nb_of(N) * ks_of(K) * kWave;
@@ -576,13 +849,16 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
                                    float threshold, geraf_stream_t stream) {
   GERAF_TRY_BEGIN
   if (M < 0 || M > INT32_MAX) throw ArgError("M out of range");
-  if (geraf_mlp_pack_bytes(N, K) < 0) throw ArgError("N must be in [1, 272] and K in [1, 288]");
+  if (geraf_mlp_pack_bytes(N, K) < 0)
+    throw ArgError("N must be in [1, 272] and K in [1, 288], N > 256 with K <= 256");
   if (M == 0) return GERAF_RF_OK;
   if (prologue != PRO_NONE && prologue != PRO_MUL_SLOPE) throw ArgError("unknown prologue");
+  if (epilogue < EPI_BIAS || epilogue > EPI_LAYER_ADJ) throw ArgError("unknown epilogue");
+  if (!(beta > 0.0f)) throw ArgError("beta must be positive");
   require_ptrs({x, wpacked, y}, true);
   check_ld(ldx, K, "x");
   check_ld(ldy, N, "y");
-  if (ldy % 4 != 0) throw ArgError("y: leading dimension must be a multiple of 4");
+  if (ldy == 0) throw ArgError("y: leading dimension must not be 0");
   if (prologue == PRO_MUL_SLOPE) {
     require_ptrs({x2}, true);
     check_ld(ldx2, K, "x2");
@@ -590,21 +866,19 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
   if (epilogue == EPI_SOFTPLUS_TRAIN || epilogue == EPI_GRAD_ADJ) {
     require_ptrs({y2}, true);
     check_ld(ldy2, N, "y2");
-    if (ldy2 % 4 != 0) throw ArgError("y2: leading dimension must be a multiple of 4");
+    if (ldy2 == 0) throw ArgError("y2: leading dimension must not be 0");
   }
   if (epilogue == EPI_GRAD_ADJ || epilogue == EPI_LAYER_ADJ) {
     require_ptrs({e0, e1}, true);
     check_ld(lde0, N, "e0");
     check_ld(lde1, N, "e1");
-    if (lde0 % 4 != 0 || lde1 % 4 != 0)
-      throw ArgError("e0/e1: leading dimensions must be multiples of 4");
   }
   const uintptr_t align = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(x2) |
                           reinterpret_cast<uintptr_t>(wpacked);
-  const uintptr_t align8 = reinterpret_cast<uintptr_t>(y) | reinterpret_cast<uintptr_t>(y2) |
+  const uintptr_t align2 = reinterpret_cast<uintptr_t>(y) | reinterpret_cast<uintptr_t>(y2) |
                            reinterpret_cast<uintptr_t>(e0) | reinterpret_cast<uintptr_t>(e1);
-  if ((align & 15u) != 0 || (align8 & 7u) != 0)
-    throw ArgError("x, x2, wpacked must be 16-byte and y, y2, e0, e1 8-byte aligned");
+  if (((align | align2) & 15u) != 0)
+    throw ArgError("x, x2, wpacked, y, y2, e0, e1 must be 16-byte aligned");
   GemmArgs a;
   a.x = x;
   a.x2 = x2;
@@ -626,8 +900,11 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
   a.nbN = nb_of(N);
   a.ksK = ks_of(K);
   a.beta = beta;
-  a.threshold = threshold;
-  const size_t lds = (size_t)a.nbN * a.ksK * kWave * sizeof(bf8);
+  a.inv_beta = 1.0f / beta;
+  // the softplus form has no linear branch: it equals torch's above
+  // thresholds where exp(-threshold) is below BF16's and the code's resolution
+  if (!(threshold >= 15.0f)) throw ArgError("threshold must be >= 15 (torch's default is 20)");
+  const size_t lds = gemm_lds_bytes(N, K);
   const hipStream_t st = (hipStream_t)stream;
   if (prologue == PRO_MUL_SLOPE) return launch_gemm<PRO_MUL_SLOPE>(epilogue, a, lds, st);
   return launch_gemm<PRO_NONE>(epilogue, a, lds, st);
@@ -636,10 +913,7 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
 
 extern "C" int64_t geraf_mlp_wgrad_workspace_bytes(int64_t M, int64_t N, int64_t K) {
   if (M < 0 || N <= 0 || K <= 0) return -1;
-  (void)M;
-  (void)N;
-  (void)K;
-  return (int64_t)256 * kWgTile * (kWgTile + 1) * (int64_t)sizeof(float);
+  return (int64_t)kWgMaxBlocks * kWgRows * kWgCols * (int64_t)sizeof(float);
 }
 
 extern "C" int geraf_mlp_wgrad_bf16(const uint16_t* g, int64_t ldg, const uint16_t* g2,
@@ -650,8 +924,12 @@ extern "C" int geraf_mlp_wgrad_bf16(const uint16_t* g, int64_t ldg, const uint16
                                     float beta, float threshold, void* workspace,
                                     int64_t workspace_bytes, geraf_stream_t stream) {
   GERAF_TRY_BEGIN
+  (void)beta;
+  (void)threshold;
   if (M < 0 || M > INT32_MAX || N <= 0 || K <= 0 || N > 512 || K > 512)
     throw ArgError("sizes out of range (N, K in [1, 512])");
+  if (prologue_g != PRO_NONE && prologue_g != PRO_MUL_SLOPE) throw ArgError("unknown prologue_g");
+  if (prologue_x != PRO_NONE && prologue_x != PRO_MUL_SLOPE) throw ArgError("unknown prologue_x");
   require_ptrs({dw, workspace}, true);
   if (workspace_bytes < geraf_mlp_wgrad_workspace_bytes(M, N, K))
     throw ArgError("workspace too small (see geraf_mlp_wgrad_workspace_bytes)");
@@ -673,68 +951,56 @@ extern "C" int geraf_mlp_wgrad_bf16(const uint16_t* g, int64_t ldg, const uint16
   const uintptr_t align = reinterpret_cast<uintptr_t>(g) | reinterpret_cast<uintptr_t>(g2) |
                           reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(x2);
   if ((align & 15u) != 0) throw ArgError("g, g2, x, x2 must be 16-byte aligned");
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        v > 0)
-      cus = v;
-  }
-  const int nsup = (int)((N + kWgTile - 1) / kWgTile), ksup = (int)((K + kWgTile - 1) / kWgTile);
-  // point chunks: a multiple of 32 points each, at most 256 chunks (the workspace)
+  // row super-tiles: one of up to 272 rows, else blocks of 256
+  const int nsup = N <= kWgRows ? 1 : (int)((N + kWgTile - 1) / kWgTile);
+  const int ksup = (int)((K + kWgTile - 1) / kWgTile);
+  // point chunks: a multiple of 64 points each, at most kWgMaxBlocks chunks
+  const int cus = cu_count();
   int64_t per = (M + cus - 1) / std::max(1, cus);
-  per = std::max<int64_t>(kPts, ((per + kPts - 1) / kPts) * kPts);
-  const int blocks = M == 0 ? 1 : (int)std::min<int64_t>(256, (M + per - 1) / per);
-  if ((int64_t)blocks * per < M) per = ((M + blocks - 1) / blocks + kPts - 1) / kPts * kPts;
+  per = std::max<int64_t>(64, ((per + 63) / 64) * 64);
+  if ((M + per - 1) / per > kWgMaxBlocks) per = ((M + kWgMaxBlocks - 1) / kWgMaxBlocks + 63) / 64 * 64;
+  const int blocks = M == 0 ? 1 : (int)((M + per - 1) / per);
   float* partial = reinterpret_cast<float*>(workspace);
   for (int ns = 0; ns < nsup; ++ns) {
+    const int64_t n0 = (int64_t)ns * kWgTile;
+    const int rows = (int)std::min<int64_t>(nsup == 1 ? N : kWgTile, N - n0);
     for (int kz = 0; kz < ksup; ++kz) {
-      WgradArgs a;
-      a.g = g;
-      a.g2 = g2;
-      a.x = x;
-      a.x2 = x2;
-      a.partial = partial;
-      a.ldg = ldg;
-      a.ldg2 = ldg2;
-      a.ldx = ldx;
-      a.ldx2 = ldx2;
-      a.M = (int)M;
-      a.N = (int)N;
-      a.K = (int)K;
-      a.with_bias = db != nullptr;
-      a.pts_per_block = (int)per;
-      a.beta = beta;
-      a.threshold = threshold;
-      // one super-tile at a time: blockIdx.y / z select it
-      const dim3 grid((unsigned)blocks, 1, 1);
-      WgradArgs b = a;
-      // shift the super-tile origin into the pointers' column offsets
-      b.g = g ? g + (size_t)ns * kWgTile : nullptr;
-      b.g2 = g2 ? g2 + (size_t)ns * kWgTile : nullptr;
+      WgradArgs b;
+      // the super-tile's origin as column offsets of the operands
+      b.g = g ? g + (size_t)n0 : nullptr;
+      b.g2 = g2 ? g2 + (size_t)n0 : nullptr;
       b.x = x ? x + (size_t)kz * kWgTile : nullptr;
       b.x2 = x2 ? x2 + (size_t)kz * kWgTile : nullptr;
-      b.N = (int)std::min<int64_t>(kWgTile, N - (int64_t)ns * kWgTile);
+      b.partial = partial;
+      b.ldg = ldg;
+      b.ldg2 = ldg2;
+      b.ldx = ldx;
+      b.ldx2 = ldx2;
+      b.M = (int)M;
+      b.N = rows;
       b.K = (int)std::min<int64_t>(kWgTile, K - (int64_t)kz * kWgTile);
       b.with_bias = (db != nullptr && kz == 0);
+      b.pts_per_block = (int)per;
       if (M > 0) {
-#define GERAF_WG(PG, PX)                                                                     \
-  hipLaunchKernelGGL((mlp_wgrad_kernel<PG, PX>), grid, dim3(64 * kWgWaves), 0, st, b)
-        if (prologue_g == PRO_MUL_SLOPE && prologue_x == PRO_MUL_SLOPE)
-          GERAF_WG(PRO_MUL_SLOPE, PRO_MUL_SLOPE);
-        else if (prologue_g == PRO_MUL_SLOPE)
-          GERAF_WG(PRO_MUL_SLOPE, PRO_NONE);
-        else if (prologue_x == PRO_MUL_SLOPE)
-          GERAF_WG(PRO_NONE, PRO_MUL_SLOPE);
-        else
-          GERAF_WG(PRO_NONE, PRO_NONE);
-#undef GERAF_WG
-        const int rc = check_launch("mlp_wgrad_kernel");
+        const bool pg = prologue_g == PRO_MUL_SLOPE, px = prologue_x == PRO_MUL_SLOPE;
+        int rc;
+        if (rows > kWgTile) {
+          if (pg || px) throw ArgError("a prologue needs N <= 256 (or N > 272)");
+          rc = launch_wgrad<PRO_NONE, PRO_NONE, 32, true>(b, blocks, st);
+        } else if (pg && px) {
+          rc = launch_wgrad<PRO_MUL_SLOPE, PRO_MUL_SLOPE, 32, false>(b, blocks, st);
+        } else if (pg) {
+          rc = launch_wgrad<PRO_MUL_SLOPE, PRO_NONE, 32, false>(b, blocks, st);
+        } else if (px) {
+          rc = launch_wgrad<PRO_NONE, PRO_MUL_SLOPE, 32, false>(b, blocks, st);
+        } else {
+          rc = launch_wgrad<PRO_NONE, PRO_NONE, 64, false>(b, blocks, st);
+        }
         if (rc != GERAF_RF_OK) return rc;
       }
-      const int cells = kWgTile * (kWgTile + 1);
+      const int cells = rows * kWgCols;
       hipLaunchKernelGGL(mlp_wgrad_reduce_kernel, dim3((cells + 255) / 256), dim3(256), 0, st,
-                         partial, M == 0 ? 0 : blocks, ns * kWgTile, kz * kWgTile, (int)N,
+                         partial, M == 0 ? 0 : blocks, rows, (int)n0, kz * kWgTile, (int)N,
                          (int)K, dw, lddw, (kz == 0) ? db : nullptr, (int)accumulate);
       const int rc = check_launch("mlp_wgrad_reduce_kernel");
       if (rc != GERAF_RF_OK) return rc;
@@ -748,12 +1014,13 @@ extern "C" int geraf_mlp_positional_encoding(const float* x, int64_t M, int32_t 
                                              uint16_t* out, int64_t ldo, int32_t col0,
                                              int32_t pad_to, geraf_stream_t stream) {
   GERAF_TRY_BEGIN
-  if (M < 0 || M > INT32_MAX || L < 0 || L > 16) throw ArgError("sizes out of range");
-  if (col0 < 0 || pad_to < 3 + 6 * L || col0 + pad_to > ldo)
-    throw ArgError("columns out of range");
+  if (M < 0 || M > INT32_MAX || L < 0 || 3 + 6 * L > kPeMaxCols)
+    throw ArgError("sizes out of range (L in [0, 10])");
+  if (col0 < 0 || pad_to < 3 + 6 * L || pad_to > kPeMaxCols || col0 + pad_to > ldo)
+    throw ArgError("columns out of range (pad_to in [3 + 6 L, 64])");
   if (M == 0) return GERAF_RF_OK;
   require_ptrs({x, out}, true);
-  hipLaunchKernelGGL(pe_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(pe_kernel, dim3((unsigned)((M + kPePts - 1) / kPePts)), dim3(kPePts), 0,
                      (hipStream_t)stream, x, (int)M, (int)L, scale, out, ldo, (int)col0,
                      (int)pad_to);
   return check_launch("pe_kernel");
@@ -764,12 +1031,13 @@ extern "C" int geraf_mlp_pe_gradient(const float* x, int64_t M, int32_t L, float
                                      const uint16_t* ga, int64_t lda, const uint16_t* gb,
                                      int64_t ldb, float* n, geraf_stream_t stream) {
   GERAF_TRY_BEGIN
-  if (M < 0 || M > INT32_MAX || L < 0 || L > 16) throw ArgError("sizes out of range");
+  if (M < 0 || M > INT32_MAX || L < 0 || 3 + 6 * L > kPeMaxCols)
+    throw ArgError("sizes out of range (L in [0, 10])");
   if (lda < 3 + 6 * L || ldb < 3 + 6 * L) throw ArgError("leading dimensions too small");
   if (M == 0) return GERAF_RF_OK;
   require_ptrs({x, ga, gb, n}, true);
-  hipLaunchKernelGGL(pe_grad_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, x, (int)M, (int)L, scale, ga, lda, gb, ldb, n);
+  hipLaunchKernelGGL(pe_grad_kernel, dim3((unsigned)((M + kPeGradPts - 1) / kPeGradPts)),
+                     dim3(kPeGradPts), 0, (hipStream_t)stream, x, (int)M, (int)L, scale, ga, lda, gb, ldb, n);
   return check_launch("pe_grad_kernel");
   GERAF_TRY_END
 }
@@ -779,14 +1047,15 @@ extern "C" int geraf_mlp_pe_gradient_adjoint(const float* x, int64_t M, int32_t 
                                              int32_t col0, int32_t pad_to,
                                              geraf_stream_t stream) {
   GERAF_TRY_BEGIN
-  if (M < 0 || M > INT32_MAX || L < 0 || L > 16) throw ArgError("sizes out of range");
-  if (col0 < 0 || pad_to < 3 + 6 * L || col0 + pad_to > ldo)
-    throw ArgError("columns out of range");
+  if (M < 0 || M > INT32_MAX || L < 0 || 3 + 6 * L > kPeMaxCols)
+    throw ArgError("sizes out of range (L in [0, 10])");
+  if (col0 < 0 || pad_to < 3 + 6 * L || pad_to > kPeMaxCols || col0 + pad_to > ldo)
+    throw ArgError("columns out of range (pad_to in [3 + 6 L, 64])");
   if (M == 0) return GERAF_RF_OK;
   require_ptrs({x, dn, out}, true);
-  hipLaunchKernelGGL(pe_grad_adj_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, x, (int)M, (int)L, scale, dn, out, ldo, (int)col0,
-                     (int)pad_to);
+  hipLaunchKernelGGL(pe_grad_adj_kernel, dim3((unsigned)((M + kPePts - 1) / kPePts)),
+                     dim3(kPePts), 0, (hipStream_t)stream, x, (int)M, (int)L, scale, dn, out,
+                     ldo, (int)col0, (int)pad_to);
   return check_launch("pe_grad_adj_kernel");
   GERAF_TRY_END
 }

@@ -23,10 +23,13 @@ Backward, for incoming gradients (d sdf, d features, d normals):
     zbar_{l-1} = (zbar_l W_l) * s_{l-1} + t_{l-1}, dW_l += zbar_l^T a_l,
     db_l = sum zbar_l.
 Each step is one product with its element-wise work fused in (the kernels'
-prologues and epilogues); activations and pre-activations are stored in BF16
-as the torch BF16 network stores them.  The skip's 1/sqrt 2 is folded into
-W_4.  Gradients reach the weight-norm parameters through torch autograd of
-W_l = g v / |v|.
+prologues and epilogues).  Activations are stored in BF16 as the torch BF16
+network stores them; in place of each pre-activation z_l the forward stores
+its softplus slope code q_l = copysign(sigmoid(-|beta z_l|), beta z_l) in
+FP16, computed from the FP32 accumulator (the backward needs z_l only
+through the slope s_l and the curvature s_l (1 - s_l), both exact functions
+of q_l).  The skip's 1/sqrt 2 is folded into W_4.  Gradients reach the
+weight-norm parameters through torch autograd of W_l = g v / |v|.
 """
 from __future__ import annotations
 
@@ -40,10 +43,11 @@ from torch.autograd.function import once_differentiable
 from geraf.ops import _native
 
 BF = torch.bfloat16
+CODE = torch.float16  # softplus slope codes
 BETA, THRESHOLD = 100.0, 20.0
 _PRO_NONE, _PRO_SLOPE = 0, 1
 _EPI_BIAS, _EPI_SOFTPLUS, _EPI_SOFTPLUS_TRAIN, _EPI_RELU, _EPI_GRAD_ADJ, _EPI_LAYER_ADJ = range(6)
-_BIG = 1.0e4  # a pre-activation whose softplus slope is exactly 1
+_SLOPE_ONE = 0.0  # the slope code of slope 1, curvature 0
 
 
 def _ptr(t: Optional[torch.Tensor]):
@@ -130,37 +134,39 @@ def _packs(ws):
 
 
 def _forward(x, ws, bs, P, train: bool):
-    """Hidden layers and the output layer.  Returns (gam, z list, h list, out8)."""
+    """Hidden layers and the output layer.  Returns (gam, slope-code list, h
+    list, out8)."""
     M = x.shape[0]
     dev = x.device
     st = _native.stream_of(x)
     gam = torch.empty((M, 64), dtype=BF, device=dev)
     _native.call("geraf_mlp_positional_encoding", _ptr(x), M, L_BANDS, 1.0, _ptr(gam), 64, 0, 64,
                  st)
-    zs, hs = [], []
+    qs, hs = [], []
     a = gam
     for l in range(8):
         h = torch.empty((M, 256), dtype=BF, device=dev)
-        z = torch.empty((M, 256), dtype=BF, device=dev) if train else None
+        z = torch.empty((M, 256), dtype=CODE, device=dev) if train else None
+        gemm(a, P[l], OUT[l], IN[l], h, bias=bs[l], y2=z,
+             epilogue=_EPI_SOFTPLUS_TRAIN if train else _EPI_SOFTPLUS)
         if l == 3:
-            # the skip input [h_3, gamma] shares h_3's buffer; z_3's last
-            # columns hold a constant whose slope is 1 (the backward passes the
-            # skip's gradients through them)
+            # the skip input [h_3, gamma] shares h_3's buffer; q_3's last
+            # columns hold the code of slope 1 (the backward passes the skip's
+            # gradients through them).  Both after the product, which writes
+            # its 217 columns in groups of 4.
             _native.call("geraf_mlp_positional_encoding", _ptr(x), M, L_BANDS, 1.0,
                          _ptr(h[:, 217:]), 256, 0, 39, st)
             if train:
-                z[:, 217:].fill_(_BIG)
-        gemm(a, P[l], OUT[l], IN[l], h, bias=bs[l], y2=z,
-             epilogue=_EPI_SOFTPLUS_TRAIN if train else _EPI_SOFTPLUS)
-        zs.append(z)
+                z[:, 217:].fill_(_SLOPE_ONE)
+        qs.append(z)
         hs.append(h)
         a = h
     out8 = torch.empty((M, 264), dtype=BF, device=dev)
     gemm(a, P[8], OUT[8], IN[8], out8, bias=bs[8])
-    return gam, zs, hs, out8
+    return gam, qs, hs, out8
 
 
-def _normal_pass(x, ws, zs, Q):
+def _normal_pass(x, ws, qs, Q):
     """u_l for l = 0..6 (u_3's buffer also carries the skip's d sdf / d gamma
     in columns 217..255), u_7 = W_8[0] (one broadcast row), and the normals."""
     M = x.shape[0]
@@ -172,11 +178,11 @@ def _normal_pass(x, ws, zs, Q):
     us[7] = u7
     for l in range(7, 0, -1):
         u = torch.empty((M, 256), dtype=BF, device=dev)
-        gemm(us[l], Q[l], IN[l], OUT[l], u, x2=zs[l], prologue=_PRO_SLOPE)
+        gemm(us[l], Q[l], IN[l], OUT[l], u, x2=qs[l], prologue=_PRO_SLOPE)
         us[l - 1] = u
     g0 = torch.empty((M, 40), dtype=BF, device=dev)
     # layer 0's product with W_0^T: the packed W_0 read as [in, out]
-    gemm(us[0], pack(ws[0], True), IN[0], OUT[0], g0, x2=zs[0], prologue=_PRO_SLOPE)
+    gemm(us[0], pack(ws[0], True), IN[0], OUT[0], g0, x2=qs[0], prologue=_PRO_SLOPE)
     n = torch.empty((M, 3), dtype=torch.float32, device=dev)
     skip = us[3][:, 217:]
     _native.call("geraf_mlp_pe_gradient", _ptr(x), M, L_BANDS, 1.0, _ptr(g0), 40, _ptr(skip), 256,
@@ -191,15 +197,15 @@ class _FusedSDF(Function):
         _check_network(ws)
         x = x.detach().float().contiguous()
         P, Q = _packs(ws)
-        gam, zs, hs, out8 = _forward(x, ws, bs, P, train=True)
+        gam, qs, hs, out8 = _forward(x, ws, bs, P, train=True)
         sdf = out8[:, 0].float()
         feat = out8[:, 1:257]
         if with_normals:
-            us, n = _normal_pass(x, ws, zs, Q)
+            us, n = _normal_pass(x, ws, qs, Q)
         else:
             us, n = None, torch.zeros((x.shape[0], 3), dtype=torch.float32, device=x.device)
         ctx.save_for_backward(*params)
-        ctx.state = (x, gam, zs, hs, us, P, Q, with_normals)
+        ctx.state = (x, gam, qs, hs, us, P, Q, with_normals)
         ctx.mark_non_differentiable(*([] if with_normals else [n]))
         return sdf, feat, n
 
@@ -208,7 +214,7 @@ class _FusedSDF(Function):
     def backward(ctx, d_sdf, d_feat, d_n):
         params = ctx.saved_tensors
         ws = list(params[0::2])
-        x, gam, zs, hs, us, P, Q, with_normals = ctx.state
+        x, gam, qs, hs, us, P, Q, with_normals = ctx.state
         ctx.state = None
         M = x.shape[0]
         dev = x.device
@@ -223,9 +229,9 @@ class _FusedSDF(Function):
             _native.call("geraf_mlp_pe_gradient_adjoint", _ptr(x), M, L_BANDS, 1.0, _ptr(dn),
                          _ptr(dgam), 40, 0, 40, _native.stream_of(x))
             # dW_0 += (u_0 s_0)^T d gamma (before u_0 is overwritten by t_0)
-            wgrad(us[0], dgam, OUT[0], IN[0], dws[0], g2=zs[0])
+            wgrad(us[0], dgam, OUT[0], IN[0], dws[0], g2=qs[0])
             vb = torch.empty((M, 256), dtype=BF, device=dev)
-            gemm(dgam, P[0], OUT[0], IN[0], vb, y2=us[0], e0=zs[0], e1=us[0],
+            gemm(dgam, P[0], OUT[0], IN[0], vb, y2=us[0], e0=qs[0], e1=us[0],
                  epilogue=_EPI_GRAD_ADJ)
             ts[0] = us[0]
             for l in range(7):
@@ -236,7 +242,7 @@ class _FusedSDF(Function):
                                  _ptr(dn), _ptr(vb[:, 217:]), 256, 0, 39,
                                  _native.stream_of(x))
                 u_n = us[nxt]
-                wgrad(u_n, vb, OUT[nxt], IN[nxt], dws[nxt], g2=zs[nxt], x2=zs[l],
+                wgrad(u_n, vb, OUT[nxt], IN[nxt], dws[nxt], g2=qs[nxt], x2=qs[l],
                       accumulate=True)
                 vb_n = torch.empty((M, 256), dtype=BF, device=dev)
                 if nxt == 7:
@@ -245,13 +251,13 @@ class _FusedSDF(Function):
                 else:
                     t_out = u_n
                     ts[nxt] = u_n
-                gemm(vb, P[nxt], OUT[nxt], IN[nxt], vb_n, x2=zs[l], y2=t_out, e0=zs[nxt],
+                gemm(vb, P[nxt], OUT[nxt], IN[nxt], vb_n, x2=qs[l], y2=t_out, e0=qs[nxt],
                      e1=u_n, prologue=_PRO_SLOPE, epilogue=_EPI_GRAD_ADJ)
                 vb = vb_n
             # dW_8[0] += sum over the points of vbar_7 s_7
             ones = torch.ones((1, 8), dtype=BF, device=dev).expand(M, 8)
             row = torch.zeros((1, 256), dtype=torch.float32, device=dev)
-            wgrad(ones, vb, 1, 256, row, x2=zs[7])
+            wgrad(ones, vb, 1, 256, row, x2=qs[7])
             dws[8][0] += row[0]
             del vb
         # primal adjoint
@@ -265,7 +271,7 @@ class _FusedSDF(Function):
             prev = l - 1
             zb_p = torch.empty((M, 256), dtype=BF, device=dev)
             t = ts[prev] if normal_terms else zero_row
-            gemm(zb, Q[l], IN[l], OUT[l], zb_p, e0=zs[prev], e1=t, epilogue=_EPI_LAYER_ADJ)
+            gemm(zb, Q[l], IN[l], OUT[l], zb_p, e0=qs[prev], e1=t, epilogue=_EPI_LAYER_ADJ)
             a_prev = hs[prev - 1] if prev >= 1 else gam  # layer 4: [h_3, gamma]
             wgrad(zb_p, a_prev, OUT[prev], IN[prev], dws[prev], db=dbs[prev], accumulate=True)
             zb = zb_p
