@@ -602,15 +602,15 @@ __global__ __launch_bounds__(256) void mlp_wgrad_reduce_kernel(const float* __re
 
 // ------------------------------------------------------------------ positional encoding
 //
-// A workgroup takes 256 points (128 for the gradient): each thread forms its
-// point's row in LDS, then the workgroup moves the rows between LDS and
-// HBM with consecutive threads on consecutive columns (coalesced 2-byte
-// accesses, any column offset and stride).
+// A workgroup takes 256 points: each thread forms (or reads) its point's row
+// in LDS (BF16, an odd row stride), and the workgroup moves the rows between
+// LDS and HBM a row per wave instruction, consecutive lanes on consecutive
+// columns (coalesced 2-byte accesses, any column offset and stride).
 
 constexpr int kPePts = 256;
-constexpr int kPeGradPts = 128;  // the gradient's FP32 rows
 constexpr int kPeMaxCols = 64;
-constexpr int kPeStride = kPeMaxCols + 1;
+
+__device__ __forceinline__ int pe_stride(int cols) { return cols | 1; }
 
 // gamma(x) = [x, sin(2^l x), cos(2^l x) for l < L] (x scaled), BF16, into
 // out[p * ldo + c0 + i] for i < 3 + 6 L (the band order of sdf_network.py's
@@ -619,17 +619,21 @@ constexpr int kPeStride = kPeMaxCols + 1;
 __global__ __launch_bounds__(kPePts) void pe_kernel(const float* __restrict__ x, int M, int L,
                                                     float scale, uint16_t* __restrict__ out,
                                                     int64_t ldo, int c0, int pad_to) {
-  __shared__ uint16_t rows[kPePts * kPeStride];
+  extern __shared__ uint16_t pe_rows[];
+  const int rs = pe_stride(pad_to);
   const int pbase = blockIdx.x * kPePts;
   const int p = pbase + threadIdx.x;
-  uint16_t* row = rows + threadIdx.x * kPeStride;
+  uint16_t* row = pe_rows + threadIdx.x * rs;
   if (p < M) {
     float v[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) v[d] = x[3 * (size_t)p + d] * scale;
-    for (int d = 0; d < 3; ++d) row[d] = to_bf(v[d]);
+    for (int d = 0; d < 3; ++d) {
+      v[d] = x[3 * (size_t)p + d] * scale;
+      row[d] = to_bf(v[d]);
+    }
     for (int l = 0; l < L; ++l) {
       const float f = (float)(1 << l);
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
         float sn, cs;
         sincosf(v[d] * f, &sn, &cs);
@@ -641,49 +645,59 @@ __global__ __launch_bounds__(kPePts) void pe_kernel(const float* __restrict__ x,
   }
   __syncthreads();
   const int np = min(kPePts, M - pbase);
-  for (int i = threadIdx.x; i < np * pad_to; i += kPePts) {
-    const int q = i / pad_to, c = i - q * pad_to;
-    out[(size_t)(pbase + q) * ldo + c0 + c] = rows[q * kPeStride + c];
+  for (int i = threadIdx.x; i < np * kPeMaxCols; i += kPePts) {
+    const int q = i >> 6, c = i & (kPeMaxCols - 1);
+    if (c < pad_to) out[(size_t)(pbase + q) * ldo + c0 + c] = pe_rows[q * rs + c];
   }
 }
 
 // n = d sdf / d x from d sdf / d gamma (the sum of two BF16 parts, [M, 3+6L]
 // each at their own stride): n_d = scale * (g[d] + sum_l 2^l (cos(2^l x_d)
 // g[3+6l+d] - sin(2^l x_d) g[3+6l+3+d])), FP32.
-__global__ __launch_bounds__(kPeGradPts) void pe_grad_kernel(const float* __restrict__ x, int M, int L,
+__global__ __launch_bounds__(kPePts) void pe_grad_kernel(const float* __restrict__ x, int M, int L,
                                                          float scale,
                                                          const uint16_t* __restrict__ ga,
                                                          int64_t lda,
                                                          const uint16_t* __restrict__ gb,
                                                          int64_t ldb, float* __restrict__ n) {
-  __shared__ float rows[kPeGradPts * kPeStride];
-  const int pbase = blockIdx.x * kPeGradPts;
-  const int np = min(kPeGradPts, M - pbase);
+  extern __shared__ uint16_t pe_rows[];
   const int nc = 3 + 6 * L;
-  for (int i = threadIdx.x; i < np * nc; i += kPeGradPts) {
-    const int q = i / nc, c = i - q * nc;
-    rows[q * kPeStride + c] = bf(ga[(size_t)(pbase + q) * lda + c]);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < np * nc; i += kPeGradPts) {
-    const int q = i / nc, c = i - q * nc;
-    rows[q * kPeStride + c] += bf(gb[(size_t)(pbase + q) * ldb + c]);
+  const int rs = pe_stride(nc);
+  uint16_t* ra = pe_rows;
+  uint16_t* rb = pe_rows + kPePts * rs;
+  const int pbase = blockIdx.x * kPePts;
+  const int np = min(kPePts, M - pbase);
+  for (int i = threadIdx.x; i < np * kPeMaxCols; i += kPePts) {
+    const int q = i >> 6, c = i & (kPeMaxCols - 1);
+    if (c < nc) {
+      ra[q * rs + c] = ga[(size_t)(pbase + q) * lda + c];
+      rb[q * rs + c] = gb[(size_t)(pbase + q) * ldb + c];
+    }
   }
   __syncthreads();
   const int p = pbase + threadIdx.x;
   if (p >= M) return;
-  const float* row = rows + threadIdx.x * kPeStride;
+  const uint16_t* a = ra + threadIdx.x * rs;
+  const uint16_t* b = rb + threadIdx.x * rs;
+  float v[3], acc[3];
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const float v = x[3 * (size_t)p + d] * scale;
-    float s = row[d];
-    for (int l = 0; l < L; ++l) {
-      const float f = (float)(1 << l);
-      float sn, cs;
-      sincosf(v * f, &sn, &cs);
-      s += f * (cs * row[3 + 6 * l + d] - sn * row[3 + 6 * l + 3 + d]);
-    }
-    n[3 * (size_t)p + d] = s * scale;
+    v[d] = x[3 * (size_t)p + d] * scale;
+    acc[d] = bf(a[d]) + bf(b[d]);
   }
+  for (int l = 0; l < L; ++l) {
+    const float f = (float)(1 << l);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float sn, cs;
+      sincosf(v[d] * f, &sn, &cs);
+      const float gs = bf(a[3 + 6 * l + d]) + bf(b[3 + 6 * l + d]);
+      const float gc = bf(a[3 + 6 * l + 3 + d]) + bf(b[3 + 6 * l + 3 + d]);
+      acc[d] += f * (cs * gs - sn * gc);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) n[3 * (size_t)p + d] = acc[d] * scale;
 }
 
 // The adjoint of pe_grad for d loss / d n: d loss / d gamma = J dn with J the
@@ -694,30 +708,36 @@ __global__ __launch_bounds__(kPePts) void pe_grad_adj_kernel(const float* __rest
                                                              const float* __restrict__ dn,
                                                              uint16_t* __restrict__ out,
                                                              int64_t ldo, int c0, int pad_to) {
-  __shared__ uint16_t rows[kPePts * kPeStride];
+  extern __shared__ uint16_t pe_rows[];
+  const int rs = pe_stride(pad_to);
   const int pbase = blockIdx.x * kPePts;
   const int p = pbase + threadIdx.x;
-  uint16_t* row = rows + threadIdx.x * kPeStride;
+  uint16_t* row = pe_rows + threadIdx.x * rs;
   if (p < M) {
+    float v[3], gd[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const float v = x[3 * (size_t)p + d] * scale;
-      const float gd = dn[3 * (size_t)p + d] * scale;
-      row[d] = to_bf(gd);
-      for (int l = 0; l < L; ++l) {
-        const float f = (float)(1 << l);
+      v[d] = x[3 * (size_t)p + d] * scale;
+      gd[d] = dn[3 * (size_t)p + d] * scale;
+      row[d] = to_bf(gd[d]);
+    }
+    for (int l = 0; l < L; ++l) {
+      const float f = (float)(1 << l);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
         float sn, cs;
-        sincosf(v * f, &sn, &cs);
-        row[3 + 6 * l + d] = to_bf(f * cs * gd);
-        row[3 + 6 * l + 3 + d] = to_bf(-f * sn * gd);
+        sincosf(v[d] * f, &sn, &cs);
+        row[3 + 6 * l + d] = to_bf(f * cs * gd[d]);
+        row[3 + 6 * l + 3 + d] = to_bf(-f * sn * gd[d]);
       }
     }
     for (int c = 3 + 6 * L; c < pad_to; ++c) row[c] = 0;
   }
   __syncthreads();
   const int np = min(kPePts, M - pbase);
-  for (int i = threadIdx.x; i < np * pad_to; i += kPePts) {
-    const int q = i / pad_to, c = i - q * pad_to;
-    out[(size_t)(pbase + q) * ldo + c0 + c] = rows[q * kPeStride + c];
+  for (int i = threadIdx.x; i < np * kPeMaxCols; i += kPePts) {
+    const int q = i >> 6, c = i & (kPeMaxCols - 1);
+    if (c < pad_to) out[(size_t)(pbase + q) * ldo + c0 + c] = pe_rows[q * rs + c];
   }
 }
 
@@ -1020,9 +1040,9 @@ extern "C" int geraf_mlp_positional_encoding(const float* x, int64_t M, int32_t 
     throw ArgError("columns out of range (pad_to in [3 + 6 L, 64])");
   if (M == 0) return GERAF_RF_OK;
   require_ptrs({x, out}, true);
-  hipLaunchKernelGGL(pe_kernel, dim3((unsigned)((M + kPePts - 1) / kPePts)), dim3(kPePts), 0,
-                     (hipStream_t)stream, x, (int)M, (int)L, scale, out, ldo, (int)col0,
-                     (int)pad_to);
+  hipLaunchKernelGGL(pe_kernel, dim3((unsigned)((M + kPePts - 1) / kPePts)), dim3(kPePts),
+                     (size_t)kPePts * (pad_to | 1) * sizeof(uint16_t), (hipStream_t)stream, x,
+                     (int)M, (int)L, scale, out, ldo, (int)col0, (int)pad_to);
   return check_launch("pe_kernel");
   GERAF_TRY_END
 }
@@ -1036,8 +1056,10 @@ extern "C" int geraf_mlp_pe_gradient(const float* x, int64_t M, int32_t L, float
   if (lda < 3 + 6 * L || ldb < 3 + 6 * L) throw ArgError("leading dimensions too small");
   if (M == 0) return GERAF_RF_OK;
   require_ptrs({x, ga, gb, n}, true);
-  hipLaunchKernelGGL(pe_grad_kernel, dim3((unsigned)((M + kPeGradPts - 1) / kPeGradPts)),
-                     dim3(kPeGradPts), 0, (hipStream_t)stream, x, (int)M, (int)L, scale, ga, lda, gb, ldb, n);
+  const size_t lds = (size_t)2 * kPePts * ((3 + 6 * L) | 1) * sizeof(uint16_t);
+  allow_lds(pe_grad_kernel, lds);
+  hipLaunchKernelGGL(pe_grad_kernel, dim3((unsigned)((M + kPePts - 1) / kPePts)), dim3(kPePts),
+                     lds, (hipStream_t)stream, x, (int)M, (int)L, scale, ga, lda, gb, ldb, n);
   return check_launch("pe_grad_kernel");
   GERAF_TRY_END
 }
@@ -1054,8 +1076,9 @@ extern "C" int geraf_mlp_pe_gradient_adjoint(const float* x, int64_t M, int32_t 
   if (M == 0) return GERAF_RF_OK;
   require_ptrs({x, dn, out}, true);
   hipLaunchKernelGGL(pe_grad_adj_kernel, dim3((unsigned)((M + kPePts - 1) / kPePts)),
-                     dim3(kPePts), 0, (hipStream_t)stream, x, (int)M, (int)L, scale, dn, out,
-                     ldo, (int)col0, (int)pad_to);
+                     dim3(kPePts), (size_t)kPePts * (pad_to | 1) * sizeof(uint16_t),
+                     (hipStream_t)stream, x, (int)M, (int)L, scale, dn, out, ldo, (int)col0,
+                     (int)pad_to);
   return check_launch("pe_grad_adj_kernel");
   GERAF_TRY_END
 }
