@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 VALU_FP32_PEAK_TFLOPS = 157.3
 MFMA_FP16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
 FLOPS_PER_TERM = 8
 MMA_FLOPS_PER_TERM = 24  # 4 real products per complex term, each as 3 FP16 products
 
@@ -233,7 +234,8 @@ def bench_stage_step(dev, target_volume, axes, steps: int = 5, warmup: int = 2):
     from rf_workloads import stage_step as st
 
     out = {"workload": "stage-1-style step: 8192 rays x 64 samples (valid 16..64), K=64, "
-                       "SDF MLP 8x256 + signal MLP 4x256, L1 vs |MF| of the 128^3 volume"}
+                       "SDF MLP 8x256 + signal MLP 4x256, transmittance calibration, "
+                       "Charbonnier loss of |MF| prediction vs the 128^3 volume / sub_adc"}
     for form in ("fused", "fp32_autograd"):
         step = st.StageStep(st.STAGE1, target_volume, axes, dev, form=form)
         for _ in range(warmup):
@@ -377,9 +379,10 @@ def bench_stage2_step(dev, target_volume, axes, steps: int = 2, warmup: int = 1)
     ms = 1e3 * (time.perf_counter() - t0) / steps
     samples = int(step.valid.sum())
     peak = torch.cuda.max_memory_allocated() / 2 ** 30
-    out = {"workload": "stage-2-style step: 32768 rays x 256 samples, K=512, SDF MLP 8x256 + "
-                       "signal MLP 4x256 (BF16), consistency term on 262144 points, L1 vs |MF| "
-                       "of the 256^3 target",
+    out = {"workload": "stage-2-style step: 32768 rays x 256 samples, K=512, hand-written BF16 "
+                       "SDF MLP 8x256 (normals and double backward) + signal MLP 4x256, "
+                       "transmittance calibration, consistency term on 262144 points, "
+                       "Charbonnier loss of |MF| prediction vs the 256^3 target / sub_adc",
            "ms_per_step": ms, "samples_per_step": samples, "samples_per_s": samples / ms * 1e3,
            "last_loss": loss, "peak_memory_gib": peak,
            "mf_tiled_fraction": step.plan.tiled_fraction,
@@ -387,6 +390,107 @@ def bench_stage2_step(dev, target_volume, axes, steps: int = 2, warmup: int = 1)
     log(f"stage-2 step: {ms:.0f} ms/step, {samples} samples, peak {peak:.1f} GiB, "
         f"MF tiled fraction {step.plan.tiled_fraction:.3f}")
     del step
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_sdf_network(dev, M: int = 8388608, reps: int = 3):
+    """The SDF network of the RF stages (8 x 256 softplus layers, skip at 4,
+    multires 6, BF16) at the stage-2 size: one pass of sdf, features and
+    normals with the full backward (the double-backward terms included),
+    hand-written (fused_sdf) vs torch's BF16 network under autograd; and each
+    layer-product / weight-gradient form of the hand-written network on its
+    own, timed per launch, with its algorithmic HBM bytes (the BF16 / FP16
+    tensors it must read and write, [M,256] each) against the 8 TB/s peak and
+    a device copy of the same size."""
+    import torch
+
+    from geraf.models.networks import fused_sdf as fs
+    from geraf.models.networks.sdf_network import SDFNetwork
+
+    BF, HF = torch.bfloat16, torch.float16
+    torch.manual_seed(0)
+    net = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6, compute_dtype=BF).to(dev)
+    x = torch.rand(M, 3, device=dev) * 1.6 - 0.8
+    g_sdf = torch.randn(M, device=dev)
+    g_feat = (torch.randn(M, 256, device=dev) * 0.05).to(BF)
+    g_n = torch.randn(M, 3, device=dev)
+
+    def fused_pass():
+        sdf, feat, n = fs.fused_sdf(net, x)
+        torch.autograd.backward([sdf, feat, n], [g_sdf, g_feat, g_n])
+
+    def torch_pass():
+        out, n = net.sdf_and_gradient(x.clone(), float_out=False)
+        torch.autograd.backward([out[:, 0], out[:, 1:], n],
+                                [g_sdf.to(out.dtype), g_feat.to(out.dtype), g_n.to(n.dtype)])
+
+    def timed(fn, n=reps):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(n):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return sorted(ts)[len(ts) // 2]
+
+    out = {"workload": f"SDF network pass (sdf, features, normals + backward) on {M} points",
+           "points": M}
+    out["fused_ms"] = timed(fused_pass)
+    net.zero_grad(set_to_none=True)
+    torch.cuda.empty_cache()
+    out["torch_bf16_ms"] = timed(torch_pass)
+    out["speedup_vs_torch_bf16"] = out["torch_bf16_ms"] / out["fused_ms"]
+    net.zero_grad(set_to_none=True)
+    torch.cuda.empty_cache()
+    log(f"SDF network pass on {M} points: hand-written {out['fused_ms']:.1f} ms, torch BF16 "
+        f"{out['torch_bf16_ms']:.1f} ms")
+
+    # the kernel forms, per launch, [M,256] operands
+    xa = (torch.randn(M, 256, device=dev) * 0.5).to(BF)
+    qa = (torch.randn(M, 256, device=dev) * 0.3).to(HF)
+    ea = torch.randn(M, 256, device=dev).to(BF)
+    ya = torch.empty(M, 256, dtype=BF, device=dev)
+    y2 = torch.empty(M, 256, dtype=BF, device=dev)
+    yq = torch.empty(M, 256, dtype=HF, device=dev)
+    W = fs.pack(torch.randn(256, 256, device=dev) / 16, False)
+    b = torch.randn(256, device=dev) * 0.01
+    dw = torch.zeros(256, 256, device=dev)
+    db = torch.zeros(256, device=dev)
+    tensor_gb = M * 256 * 2 / 1e9
+    forms = [
+        ("device copy (torch)", 2, lambda: ya.copy_(xa)),
+        ("layer product + bias", 2, lambda: fs.gemm(xa, W, 256, 256, ya, bias=b)),
+        ("layer product + softplus", 2, lambda: fs.gemm(xa, W, 256, 256, ya, bias=b, epilogue=1)),
+        ("layer product + softplus + slope code", 3,
+         lambda: fs.gemm(xa, W, 256, 256, ya, bias=b, y2=yq, epilogue=2)),
+        ("slope prologue + layer product (normals)", 3,
+         lambda: fs.gemm(xa, W, 256, 256, ya, x2=qa, prologue=1)),
+        ("slope prologue + product + second-order term", 6,
+         lambda: fs.gemm(xa, W, 256, 256, ya, x2=qa, y2=y2, e0=qa, e1=ea, prologue=1,
+                         epilogue=4)),
+        ("product + hidden-layer adjoint", 4,
+         lambda: fs.gemm(xa, W, 256, 256, ya, e0=qa, e1=ea, epilogue=5)),
+        ("weight gradient", 2, lambda: fs.wgrad(xa, ea, 256, 256, dw, db=db)),
+        ("weight gradient, slope prologues", 4, lambda: fs.wgrad(xa, ea, 256, 256, dw, g2=qa,
+                                                                  x2=qa)),
+    ]
+    kernels = []
+    for name, tensors, fn in forms:
+        ms = timed(fn, 5)
+        gbs = tensors * tensor_gb / ms * 1e3
+        kernels.append({"form": name, "ms": ms, "algorithmic_gb": tensors * tensor_gb,
+                        "achieved_gbs": gbs, "hbm_frac": gbs / (HBM_PEAK_GBS)})
+        log(f"  {name}: {ms:.3f} ms, {gbs / 1e3:.2f} TB/s")
+    copy_gbs = kernels[0]["achieved_gbs"]
+    for k in kernels[1:]:
+        k["vs_device_copy"] = k["achieved_gbs"] / copy_gbs
+    out["kernels"] = kernels
+    del xa, qa, ea, ya, y2, yq
     torch.cuda.empty_cache()
     return out
 
@@ -451,7 +555,7 @@ def bench_ap_tracer(dev, reps: int = 10):
         res[name] = {"ms": ms, "terms_per_s": terms / (ms / 1e3),
                      "roofline": {"bound": "valu", "achieved_tflops": tf,
                                   "fp32_frac": tf / VALU_FP32_PEAK_TFLOPS, "hbm_gbs": gbs,
-                                  "hbm_frac": gbs / 8000.0, "bytes": by}}
+                                  "hbm_frac": gbs / HBM_PEAK_GBS, "bytes": by}}
     res["kernels"] = {"forward": "st_forward_kernel", "backward": "st_backward_kernel"}
     # accuracy on one antenna row / a point subset against the FP64 oracle
     try:
@@ -538,7 +642,7 @@ def bench_bilinear(dev, reps: int = 20):
             torch.cuda.synchronize()
             ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in evs]))
             gbs = by / (ms / 1e3) / 1e9
-            res[name] = {"ms": ms, "bytes": by, "hbm_gbs": gbs, "hbm_frac": gbs / 8000.0}
+            res[name] = {"ms": ms, "bytes": by, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
         out[f"{R}x{S}to{T}"] = res
         log(f"bilinear {R} rays {S}->{T}: forward {res['forward']['ms'] * 1e3:.1f} us "
             f"({res['forward']['hbm_frac']:.3f} of HBM), backward "
@@ -715,7 +819,7 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         "timing": "median per-launch kernel durations (events around each launch); "
                   "back-to-back pair for comparison",
         "back_to_back_pair_ms": burst_pair_ms, "emitting_fraction": live,
-        "roofline": {"hbm_gbs": gbs, "hbm_frac": gbs / 8000.0, "fp32_tflops": tfs,
+        "roofline": {"hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfs,
                      "fp32_frac": tfs / VALU_FP32_PEAK_TFLOPS,
                      "fp32_units": "MFMA (signal for every K, h_s for K > 128), VALU otherwise; the "
                                    "FP32 peak is 157.3 TFLOP/s on either",
@@ -872,6 +976,7 @@ def main() -> int:
         large, vol2, axes2 = bench_large_target(dev)
         stage2 = bench_stage2_step(dev, vol2, axes2)
         del vol2
+        stage2["sdf_network"] = bench_sdf_network(dev)
 
     tracer = None
     if not args.no_tracer:

@@ -35,7 +35,10 @@
 // matrix cores (k = 16 a + b, see synth_mfma), for every K: at K = 32, 64 and
 // 128 it measured 22, 18 and 25 % faster than a rotation synthesis on the
 // vector ALUs (4,096 rays x 128 samples), although most of its 256-frequency
-// tile is idle there.
+// tile is idle there.  (A form on FP16 hi/lo operands, 32 samples per
+// v_mfma_f32_16x16x32_f16 product, measured 4 % slower at 32,768 x 256,
+// K = 512: its splits cost more vector instructions than the matrix time it
+// saves, and the kernel is bound by vector issue.)
 // Backward: pass A walks the tiles forward, keeps the transmittance carry at
 // each tile start, and queues (tau, sample) of every emitting sample; full
 // 64-sample batches evaluate h_s = Re sum_k g[r,k] exp(-j phi_sk), for
@@ -144,8 +147,8 @@ __device__ __forceinline__ RawSample load_raw(const LenslessArgs& a, const Ray& 
   return w;
 }
 
-__device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ray& y, int s) {
-  const RawSample w = load_raw(a, y, s);
+__device__ __forceinline__ Sample composite(const LenslessArgs& a, const Ray& y,
+                                            const RawSample& w) {
   Sample x;
   x.sdf = w.sdf;
   x.dist = w.dist;
@@ -177,44 +180,50 @@ __device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ra
 }
 
 // Specular geometry of the monostatic sample (ComputeSpecularGeometry,
-// SignalTracer_cuda_kernel.cu:172-197, with tx = rx = o; ret = -inc exactly).
+// SignalTracer_cuda_kernel.cu:172-197, with tx = rx = o, so ret = -inc).
+// The reference forms v = p - o, inc = v / (|v| + 1e-7), n = g / |g|,
+// rr = inc - 2 (inc.n) n, refl = rr / (|rr| + 1e-7) and spec = ret.refl.
+// With nu = |inc| = |v| / (|v| + 1e-7) and |n| = 1, |rr|^2 = nu^2 exactly, so
+//   spec = (2 (inc.n)^2 - nu^2) / (nu + 1e-7),
+// and inc.n = (v.g) / ((|v| + 1e-7) |g|) with v.g = z (d.g) (v = z d up to
+// the rounding of p, ~1e-7 relative).  |v| and the round-trip distance come
+// from the FP32 point p = o + z d (separately rounded product and sum, as
+// the tensor ops round it) in FP64.
 struct Geo {
-  V3 n;             // normalised normal
-  float gl;         // |g|
-  V3 inc;           // normalised incident direction
-  V3 refl;          // normalised reflected direction
-  float refl_norm;  // |inc - 2 (inc.n) n|
-  float idn;        // inc . n
-  float spec;       // ret . refl
-  double dpath;     // round-trip distance (no bias)
-  bool live;        // valid and not culled
+  V3 n;          // normalised normal
+  float gl;      // |g|
+  float incs;    // inc = incs * d
+  float idn;     // inc . n
+  float nu;      // |inc|
+  float spec;    // ret . refl
+  double dpath;  // round-trip distance (no bias)
+  bool live;     // valid and not culled
 };
 
 __device__ __forceinline__ Geo geometry(const Ray& y, const Sample& x, bool valid) {
   Geo e;
   e.gl = sqrt_fast(dot(x.g, x.g));
-  e.n = rcp_fast(fmaxf(e.gl, 1.0e-12f)) * x.g;
-  // point = o + z*d with separately rounded product and sum (as a tensor op would)
+  const float inv_gl = rcp_fast(fmaxf(e.gl, 1.0e-12f));
+  e.n = inv_gl * x.g;
   const V3 zd = v3(__fmul_rn(x.zz, y.dir.x), __fmul_rn(x.zz, y.dir.y), __fmul_rn(x.zz, y.dir.z));
   const V3 p = v3(__fadd_rn(y.o.x, zd.x), __fadd_rn(y.o.y, zd.y), __fadd_rn(y.o.z, zd.z));
-  const V3 v = p - y.o;
-  e.inc = rcp_fast(sqrt_fast(dot(v, v)) + 1.0e-7f) * v;
-  e.idn = dot(e.inc, e.n);
-  const V3 rr = e.inc - (2.0f * e.idn) * e.n;
-  e.refl_norm = sqrt_fast(dot(rr, rr));
-  e.refl = rcp_fast(e.refl_norm + 1.0e-7f) * rr;
-  e.spec = -dot(e.inc, e.refl);
   const double dx = (double)p.x - y.ox, dy = (double)p.y - y.oy, dz = (double)p.z - y.oz;
-  e.dpath = 2.0 * sqrt_f64_newton(fma(dx, dx, fma(dy, dy, dz * dz)));
+  const double vl = sqrt_f64_newton(fma(dx, dx, fma(dy, dy, dz * dz)));
+  e.dpath = 2.0 * vl;
+  const float vlf = (float)vl;
+  const float inv_v = rcp_fast(vlf + 1.0e-7f);
+  e.incs = x.zz * inv_v;
+  e.nu = vlf * inv_v;
+  e.idn = (e.incs * x.c) * inv_gl;
+  e.spec = fmaf(2.0f * e.idn, e.idn, -e.nu * e.nu) * rcp_fast(e.nu + 1.0e-7f);
   e.live = valid && !(e.spec < 1.0e-6f || e.idn > 0.0f);
   return e;
 }
 
-// d spec / d n (see dspec_dnormal) with ret = -inc.
-__device__ __forceinline__ V3 dspec_dn_mono(const Geo& e) {
-  const V3 q = v3(-e.inc.x, -e.inc.y, -e.inc.z) - e.spec * e.refl;
-  const float s = -2.0f * rcp_fast(e.refl_norm + 1.0e-7f);
-  return s * (dot(q, e.n) * e.inc + e.idn * q);
+// d spec / d n up to its component along n (which the normalisation's
+// Jacobian removes): 4 (inc.n) inc / (nu + 1e-7).
+__device__ __forceinline__ V3 dspec_dn_mono(const Ray& y, const Geo& e) {
+  return (4.0f * e.idn * e.incs * rcp_fast(e.nu + 1.0e-7f)) * y.dir;
 }
 
 // Number of set bits of `mask` below this lane.
@@ -302,10 +311,17 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
 
   float carry = 1.0f;
   int queued = 0;
+  // each tile's inputs are loaded while the previous tile is composited (a
+  // ray with no valid samples loads nothing)
+  RawSample cur{};
+  if (y.nv > 0) cur = load_raw(a, y, min(lane, y.nv - 1));
   for (int base = 0; base < y.nv; base += kTile) {
     const int s = base + lane;
     const bool valid = s < y.nv;
-    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
+    RawSample nxt = cur;
+    if (base + kTile < y.nv) nxt = load_raw(a, y, min(base + kTile + lane, y.nv - 1));
+    const Sample x = composite(a, y, cur);
+    cur = nxt;
     const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
     const float T = carry * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
     carry *= readlane(incl, kWave - 1);
@@ -636,11 +652,16 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   wave_sync();
   float carry = 1.0f;
   int queued = 0;
+  RawSample cur{};
+  if (ntiles > 0) cur = load_raw(a, y, min(lane, y.nv - 1));
   for (int tile = 0; tile < ntiles; ++tile) {
     if (lane == 0) L.carry[tile] = carry;
     const int s = tile * kTile + lane;
     const bool valid = s < y.nv;
-    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
+    RawSample nxt = cur;
+    if (tile + 1 < ntiles) nxt = load_raw(a, y, min(s + kTile, y.nv - 1));
+    const Sample x = composite(a, y, cur);
+    cur = nxt;
     const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
     carry *= readlane(incl, kWave - 1);
     const Geo e = geometry(y, x, valid);
@@ -680,10 +701,14 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   float* gs_row = grad_sdf + y.row;
   float* gg_row = grad_g + 3 * y.row;
   float* gr_out = grad_refl ? grad_refl + y.row : nullptr;
+  if (ntiles > 0) cur = load_raw(a, y, min((ntiles - 1) * kTile + lane, y.nv - 1));
   for (int tile = ntiles - 1; tile >= 0; --tile) {
     const int s = tile * kTile + lane;
     const bool valid = s < y.nv;
-    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
+    RawSample nxt = cur;
+    if (tile > 0) nxt = load_raw(a, y, s - kTile);
+    const Sample x = composite(a, y, cur);
+    cur = nxt;
     const float tt = valid ? x.t : 1.0f;
     const float incl = wave_incl_prod_dpp(tt);
     const float T = L.carry[tile] * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
@@ -702,7 +727,7 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
     }
     const float gsig = h * e.spec * att;
     const float gspec = h * sigma * att;
-    const V3 gnhat = e.live ? gspec * dspec_dn_mono(e) : v3(0.0f, 0.0f, 0.0f);
+    const V3 gnhat = e.live ? gspec * dspec_dn_mono(y, e) : v3(0.0f, 0.0f, 0.0f);
     float grefl, gTc, galpha;
     if (unclamped) {
       grefl = gsig * y.power * Tc * aTc;
