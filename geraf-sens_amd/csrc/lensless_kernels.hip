@@ -147,8 +147,8 @@ __device__ __forceinline__ RawSample load_raw(const LenslessArgs& a, const Ray& 
   return w;
 }
 
-__device__ __forceinline__ Sample composite(const LenslessArgs& a, const Ray& y,
-                                            const RawSample& w) {
+__device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ray& y, int s) {
+  const RawSample w = load_raw(a, y, s);
   Sample x;
   x.sdf = w.sdf;
   x.dist = w.dist;
@@ -311,17 +311,10 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
 
   float carry = 1.0f;
   int queued = 0;
-  // each tile's inputs are loaded while the previous tile is composited (a
-  // ray with no valid samples loads nothing)
-  RawSample cur{};
-  if (y.nv > 0) cur = load_raw(a, y, min(lane, y.nv - 1));
   for (int base = 0; base < y.nv; base += kTile) {
     const int s = base + lane;
     const bool valid = s < y.nv;
-    RawSample nxt = cur;
-    if (base + kTile < y.nv) nxt = load_raw(a, y, min(base + kTile + lane, y.nv - 1));
-    const Sample x = composite(a, y, cur);
-    cur = nxt;
+    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
     const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
     const float T = carry * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
     carry *= readlane(incl, kWave - 1);
@@ -652,16 +645,11 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   wave_sync();
   float carry = 1.0f;
   int queued = 0;
-  RawSample cur{};
-  if (ntiles > 0) cur = load_raw(a, y, min(lane, y.nv - 1));
   for (int tile = 0; tile < ntiles; ++tile) {
     if (lane == 0) L.carry[tile] = carry;
     const int s = tile * kTile + lane;
     const bool valid = s < y.nv;
-    RawSample nxt = cur;
-    if (tile + 1 < ntiles) nxt = load_raw(a, y, min(s + kTile, y.nv - 1));
-    const Sample x = composite(a, y, cur);
-    cur = nxt;
+    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
     const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
     carry *= readlane(incl, kWave - 1);
     const Geo e = geometry(y, x, valid);
@@ -701,14 +689,10 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   float* gs_row = grad_sdf + y.row;
   float* gg_row = grad_g + 3 * y.row;
   float* gr_out = grad_refl ? grad_refl + y.row : nullptr;
-  if (ntiles > 0) cur = load_raw(a, y, min((ntiles - 1) * kTile + lane, y.nv - 1));
   for (int tile = ntiles - 1; tile >= 0; --tile) {
     const int s = tile * kTile + lane;
     const bool valid = s < y.nv;
-    RawSample nxt = cur;
-    if (tile > 0) nxt = load_raw(a, y, s - kTile);
-    const Sample x = composite(a, y, cur);
-    cur = nxt;
+    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
     const float tt = valid ? x.t : 1.0f;
     const float incl = wave_incl_prod_dpp(tt);
     const float T = L.carry[tile] * dpp_mov<dpp::kWaveShr1>(1.0f, incl);

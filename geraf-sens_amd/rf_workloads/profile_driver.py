@@ -21,7 +21,9 @@ adjoint on the 64x64 MIMO, K=512, 128^3 workload (twice: the second call is
 the one to read).  --ap-ops runs the [A,P] signal tracer forward and backward
 at its training shape (2,048 antenna pairs, 51,456 points, K=64) and the
 depth resampler (804 rays x 32 -> 64 and 32,768 x 256 -> 512 samples), each
-three times.
+three times.  --sdf-network runs the hand-written SDF network (the RF
+stages' 8 x 256 configuration, BF16) on 8,388,608 points: one warm-up and
+two passes of sdf, features and normals with the full backward.
 """
 import argparse
 import os
@@ -71,8 +73,12 @@ def main() -> None:
     ap.add_argument("--form", default="fused", choices=("fused", "fp32_autograd"))
     ap.add_argument("--tiled", action="store_true")
     ap.add_argument("--ap-ops", action="store_true")
+    ap.add_argument("--sdf-network", action="store_true")
     args = ap.parse_args()
     dev = torch.device("cuda:0")
+    if args.sdf_network:
+        sdf_network(dev)
+        return
     if args.ap_ops:
         ap_ops(dev)
         return
@@ -122,6 +128,32 @@ def main() -> None:
     torch.cuda.synchronize()
 
     trace_sizes(dev, args.stage2)
+
+
+def sdf_network(dev, M: int = 8388608) -> None:
+    """The hand-written SDF network's pass (fused_sdf: forward, normals and
+    the backward with the double-backward terms) on M random points."""
+    from geraf.models.networks.fused_sdf import fused_sdf
+    from geraf.models.networks.sdf_network import SDFNetwork
+
+    torch.manual_seed(0)
+    net = SDFNetwork(3, 257, 256, 8, skip_in=(4,), multires=6,
+                     compute_dtype=torch.bfloat16).to(dev)
+    x = torch.rand(M, 3, device=dev) * 1.6 - 0.8
+    g_sdf = torch.randn(M, device=dev)
+    g_feat = (torch.randn(M, 256, device=dev) * 0.05).to(torch.bfloat16)
+    g_n = torch.randn(M, 3, device=dev)
+    for i in range(3):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        sdf, feat, n = fused_sdf(net, x)
+        torch.autograd.backward([sdf, feat, n], [g_sdf, g_feat, g_n])
+        ev[1].record()
+        torch.cuda.synchronize()
+        print(f"profile_driver: SDF network pass {i} on {M} points: "
+              f"{ev[0].elapsed_time(ev[1]):.1f} ms", flush=True)
+        del sdf, feat, n
+        net.zero_grad(set_to_none=True)
 
 
 def tiled(dev) -> None:

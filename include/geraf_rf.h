@@ -366,33 +366,42 @@ int geraf_softplus_grad2_bf16(const uint16_t* ggx, const uint16_t* g, const uint
  * geraf_mlp_pack_weights: W [N,K] FP32 (w[n*ldw + k], or w[k*ldw + n] when
  * transpose != 0) times scale, rounded to BF16, into the operand order the
  * products read (geraf_mlp_pack_bytes(N, K) bytes, 16-byte aligned).
- * N <= 272, K <= 288; geraf_mlp_pack_bytes returns -1 outside that.
+ * N <= 272, K <= 288, and K <= 256 when N > 256 (the padded weights stay in
+ * LDS); geraf_mlp_pack_bytes returns -1 outside that.
+ *
+ * Slope codes: where the network stores a softplus pre-activation z, the
+ * training epilogue writes instead its FP16 code q = copysign(m, beta z),
+ * m = sigmoid(-|beta z|) (from the FP32 accumulator); the slope is
+ * s = 1 - |q| for q >= +0 and |q| for q <= -0 (sign bit), the curvature
+ * s (1 - s) = |q| (1 - |q|).
  *
  * geraf_mlp_gemm_bf16: y[M,N] = epilogue(prologue(x)[M,K] . W^T + bias):
- *   prologue 0: x; 1: x * s(x2) (s the softplus slope of the stored
- *     pre-activation x2, element by element);
+ *   prologue 0: x; 1: x * s(x2) (x2 the slope codes, element by element);
  *   epilogue (z = the product + bias, bias may be NULL):
- *     0: y = z;  1: y = softplus(z);  2: y = softplus(z), y2 = z;
+ *     0: y = z;  1: y = softplus(z);  2: y = softplus(z), y2 = code(z) (FP16);
  *     3: y = relu(z);  4: y = z, y2 = z * e1 * beta * s(e0) (1 - s(e0));
- *     5: y = z * s(e0) + e1.
- *   Softplus(beta, threshold) as torch's; where a pre-activation is rounded
- *   to BF16 (epilogues 1 and 2, and e0 and x2 as stored) the activation and
- *   its slopes are taken of the rounded value.  Leading dimensions are in
- *   elements: 0 broadcasts one row, otherwise a multiple of 8 covering the
- *   columns rounded up to 8 (y, y2, e0, e1 also multiples of 4).  x, x2 and
- *   wpacked 16-byte aligned, y, y2, e0, e1 8-byte aligned.
+ *     5: y = z * s(e0) + e1  (e0 slope codes, e1 BF16).
+ *   Softplus(beta, threshold) as torch's, evaluated as (max(t,0) +
+ *   log1p(exp(-|t|))) / beta, t = beta z, which equals torch's linear branch
+ *   to an FP32 ulp for t > threshold; threshold must be >= 15.  Outputs are
+ *   written in groups of 8 columns: columns N .. 8 ceil(N/8) - 1 of y and y2
+ *   receive unspecified values.  Leading dimensions are in elements: 0
+ *   broadcasts one row (inputs only), otherwise a multiple of 8 covering the
+ *   columns rounded up to 8.  All pointers 16-byte aligned.
  *
  * geraf_mlp_wgrad_bf16: dw[N,K] (+)= prologue(g)^T prologue(x) over the M
- * points (g [M,N], x [M,K], prologues as above), FP32, row stride lddw;
+ * points (g [M,N], x [M,K], prologues as above; beta and threshold are
+ * unused, the codes carry the slopes), FP32, row stride lddw;
  * db[N] (+)= the column sums of prologue(g) when db is not NULL; with
- * accumulate != 0 the results are added to dw / db.  N, K <= 512.  The
+ * accumulate != 0 the results are added to dw / db.  N, K <= 512 (a
+ * prologue needs N <= 256 or N > 272).  The
  * workspace (geraf_mlp_wgrad_workspace_bytes, caller-owned) holds per-block
  * partial sums, added in a fixed order (deterministic).
  *
  * geraf_mlp_positional_encoding: out[p*ldo + col0 + i] = gamma(x_p * scale)_i
  * (i < 3 + 6L: x, then per band l the 3 sines and the 3 cosines of
  * 2^l x, the Embedder order of sdf_network.py:19-75), BF16; zeros up to
- * pad_to columns.
+ * pad_to columns (3 + 6L <= pad_to <= 64, L <= 10).
  * geraf_mlp_pe_gradient: n[p] = d sdf / d x from d sdf / d gamma = ga + gb
  * (two BF16 [M, 3+6L] parts), FP32 [M,3].
  * geraf_mlp_pe_gradient_adjoint: its transpose, d loss / d gamma = J dn
