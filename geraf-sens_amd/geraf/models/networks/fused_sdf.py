@@ -121,21 +121,38 @@ def _check_network(ws: List[torch.Tensor]) -> None:
                              f"{(OUT[l], IN[l])}")
 
 
+def _out_rows(t: torch.Tensor) -> torch.Tensor:
+    """The output layer's rows in the kernels' order: the 256 feature rows,
+    then the sdf row (the reference's row 0).  The layer's output and its
+    gradient are laid out [features | sdf] so that the feature block starts
+    16-byte aligned."""
+    return torch.cat([t[1:], t[:1]])
+
+
 def _packs(ws):
     """P_l packs W_l ([out, in]; the forward and the normal-pass adjoint),
     Q_l packs W_l^T ([in, out]; the normal pass and the primal adjoint);
-    layer 4 carries the skip's 1/sqrt 2."""
+    layer 4 carries the skip's 1/sqrt 2, layer 8 its rows in _out_rows
+    order."""
     P, Q = [], []
     for l, w in enumerate(ws):
         sc = 1.0 / SQRT2 if l == SKIP else 1.0
+        if l == NL - 1:
+            w = _out_rows(w.detach())
         P.append(pack(w, False, sc))
         Q.append(pack(w, True, sc) if l >= 1 else None)
     return P, Q
 
 
+def _biases(params) -> List[torch.Tensor]:
+    bs = [b.detach().float().contiguous() for b in params]
+    bs[NL - 1] = _out_rows(bs[NL - 1]).contiguous()
+    return bs
+
+
 def _forward(x, ws, bs, P, train: bool):
     """Hidden layers and the output layer.  Returns (gam, slope-code list, h
-    list, out8)."""
+    list, out8 [M, 264] = [features | sdf | padding])."""
     M = x.shape[0]
     dev = x.device
     st = _native.stream_of(x)
@@ -193,13 +210,13 @@ def _normal_pass(x, ws, qs, Q):
 class _FusedSDF(Function):
     @staticmethod
     def forward(ctx, x, with_normals: bool, *params):
-        ws, bs = list(params[0::2]), [b.detach().float().contiguous() for b in params[1::2]]
+        ws, bs = list(params[0::2]), _biases(params[1::2])
         _check_network(ws)
         x = x.detach().float().contiguous()
         P, Q = _packs(ws)
         gam, qs, hs, out8 = _forward(x, ws, bs, P, train=True)
-        sdf = out8[:, 0].float()
-        feat = out8[:, 1:257]
+        sdf = out8[:, 256].float()
+        feat = out8[:, :256]
         if with_normals:
             us, n = _normal_pass(x, ws, qs, Q)
         else:
@@ -260,13 +277,28 @@ class _FusedSDF(Function):
             wgrad(ones, vb, 1, 256, row, x2=qs[7])
             dws[8][0] += row[0]
             del vb
-        # primal adjoint
-        zb = torch.zeros((M, 264), dtype=BF, device=dev)
-        if d_sdf is not None:
-            zb[:, 0] = d_sdf.to(BF)
+        # primal adjoint; the output layer's gradient [d features | d sdf]
+        zb = torch.empty((M, 264), dtype=BF, device=dev)
         if d_feat is not None:
-            zb[:, 1:257] = d_feat.to(BF)
-        wgrad(zb, hs[7], OUT[8], IN[8], dws[8], db=dbs[8], accumulate=True)
+            zb[:, :256].copy_(d_feat)
+        else:
+            zb[:, :256].zero_()
+        if d_sdf is not None:
+            zb[:, 256].copy_(d_sdf)
+        else:
+            zb[:, 256].zero_()
+        zb[:, 257:].zero_()
+        # dW_8: the 256 feature rows, then the sdf row
+        dw_f = torch.zeros((256, 256), dtype=torch.float32, device=dev)
+        db_f = torch.zeros(256, dtype=torch.float32, device=dev)
+        wgrad(zb, hs[7], 256, 256, dw_f, db=db_f)
+        dw_s = torch.zeros((1, 256), dtype=torch.float32, device=dev)
+        db_s = torch.zeros(1, dtype=torch.float32, device=dev)
+        wgrad(zb[:, 256:], hs[7], 1, 256, dw_s, db=db_s)
+        dws[8][1:] += dw_f
+        dws[8][0] += dw_s[0]
+        dbs[8][1:] += db_f
+        dbs[8][0] += db_s[0]
         for l in range(8, 0, -1):
             prev = l - 1
             zb_p = torch.empty((M, 256), dtype=BF, device=dev)
@@ -297,9 +329,10 @@ def fused_sdf(net, x: torch.Tensor, with_normals: bool = True
 def fused_sdf_value(net, x: torch.Tensor) -> torch.Tensor:
     """The SDF alone [M] FP32, no gradients (the calibration's evaluations)."""
     ws = [getattr(net, f"lin{l}").weight for l in range(NL)]
-    bs = [getattr(net, f"lin{l}").bias.detach().float().contiguous() for l in range(NL)]
+    bs = _biases([getattr(net, f"lin{l}").bias for l in range(NL)])
     _check_network(ws)
     x = x.detach().float().reshape(-1, 3).contiguous()
-    P = [pack(w, False, 1.0 / SQRT2 if l == SKIP else 1.0) for l, w in enumerate(ws)]
+    P = [pack(_out_rows(w.detach()) if l == NL - 1 else w, False,
+              1.0 / SQRT2 if l == SKIP else 1.0) for l, w in enumerate(ws)]
     _, _, _, out8 = _forward(x, ws, bs, P, train=False)
-    return out8[:, 0].float()
+    return out8[:, 256].float()
