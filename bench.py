@@ -696,18 +696,20 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
     gpw = torch.empty(R, device=dev)
     inv_s = b.inv_s.reshape(1).contiguous()
     power = b.power.reshape(1).contiguous()
+    ntile = (S + 63) // 64
+    carry = torch.empty((R, ntile), device=dev)  # the forward's tile carries
     P = _native.ptr
     stream = _native.stream_of(o_r)
     ins = [P(b.sdf), P(b.grad), P(b.refl), P(b.dists), P(b.z), P(b.o), P(b.d), P(None), P(None),
            P(inv_s), P(power)]
 
     def fwd():
-        _native.call("geraf_lensless_forward_f32", *ins, P(o_r), P(o_i), P(None), R, S, K,
-                     b.anneal, prm, stream)
+        _native.call("geraf_lensless_forward_f32", *ins, P(o_r), P(o_i), P(None), P(carry), R,
+                     S, K, b.anneal, prm, stream)
 
-    def bwd():
+    def bwd(tc=carry):
         _native.call("geraf_lensless_backward_f32", P(g_r), P(g_i), *ins, P(gs), P(gg), P(grf),
-                     P(gis), P(gpw), R, S, K, b.anneal, prm, stream)
+                     P(gis), P(gpw), P(tc), R, S, K, b.anneal, prm, stream)
 
     for _ in range(warmup):
         fwd()
@@ -740,6 +742,7 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
     n = max(steps, 4) if samples_per_call(R, S) > (1 << 22) else max(steps, 50)
     f_ms = per_launch(fwd, n)
     b_ms = per_launch(bwd, n)
+    b2_ms = per_launch(lambda: bwd(None), n)  # without the carries: two walks
     pair_ms = f_ms + b_ms
     burst_pair_ms = burst(lambda: (fwd(), bwd()), n)
     samples = R * S
@@ -748,13 +751,15 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
     # 3-vector gradient, reflectance, dists, z), 24 B of origin and direction
     # per ray and writes the K complex outputs; the backward reads the same
     # inputs and the K complex gradients and writes 20 B per sample (sdf,
-    # gradient and reflectance gradients) plus two per-ray partials
-    bytes_fwd = samples * 28 + R * (24 + 8 * K) + 8
-    bytes_bwd = samples * (28 + 20) + R * (24 + 8 * K + 8) + 8
+    # gradient and reflectance gradients) plus two per-ray partials; the
+    # forward writes and the backward reads one carry per 64-sample tile
+    bytes_fwd = samples * 28 + R * (24 + 8 * K + 4 * ntile) + 8
+    bytes_bwd = samples * (28 + 20) + R * (24 + 8 * K + 8 + 4 * ntile) + 8
     flops = 16.0 * K * samples * live  # only emitting samples carry signal work
     gbs = (bytes_fwd + bytes_bwd) / (pair_ms / 1e3) / 1e9
     tfs = flops / (pair_ms / 1e3) / 1e12
-    log(f"tracer {R}x{S} K={K}: fwd {f_ms:.4f} ms, bwd {b_ms:.4f} ms (per launch), pair "
+    log(f"tracer {R}x{S} K={K}: fwd {f_ms:.4f} ms, bwd {b_ms:.4f} ms (per launch; "
+        f"{b2_ms:.4f} ms without the forward's carries), pair "
         f"{pair_ms:.4f} ms (back to back {burst_pair_ms:.4f} ms), {rate:.3e} samples/s, "
         f"{gbs:.0f} GB/s, {tfs:.1f} TFLOP/s (emitting fraction {live:.3f})")
 
@@ -784,7 +789,7 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         m_r[:n], m_i[:n] = g_r[:n], g_i[:n]
         fwd()
         _native.call("geraf_lensless_backward_f32", P(m_r), P(m_i), *ins, P(gs), P(gg),
-                     P(grf), P(gis), P(gpw), R, S, K, b.anneal, prm, stream)
+                     P(grf), P(gis), P(gpw), P(carry), R, S, K, b.anneal, prm, stream)
         torch.cuda.synchronize()
         out = torch.complex(o_r[:n].double(), o_i[:n].double()).cpu()
         errs["rays_checked"] = n
@@ -815,7 +820,7 @@ def bench_tracer(dev, steps: int, warmup: int, cpu_seconds: float, no_cpu: bool,
         "metric": "tracer fwd+bwd samples/s",
         "value": rate, "unit": "samples/s",
         "workload": f"fused lensless tracer, {R} rays x {S} samples, K={K}",
-        "fwd_ms": f_ms, "bwd_ms": b_ms, "fwd_bwd_ms": pair_ms,
+        "fwd_ms": f_ms, "bwd_ms": b_ms, "bwd_two_walk_ms": b2_ms, "fwd_bwd_ms": pair_ms,
         "timing": "median per-launch kernel durations (events around each launch); "
                   "back-to-back pair for comparison",
         "back_to_back_pair_ms": burst_pair_ms, "emitting_fraction": live,

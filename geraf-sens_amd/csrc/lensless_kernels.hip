@@ -200,21 +200,21 @@ struct Geo {
   bool live;     // valid and not culled
 };
 
-__device__ __forceinline__ Geo geometry(const Ray& y, const Sample& x, bool valid) {
+__device__ __forceinline__ Geo geometry(const Ray& y, float zz, const V3& g, bool valid) {
   Geo e;
-  e.gl = sqrt_fast(dot(x.g, x.g));
+  e.gl = sqrt_fast(dot(g, g));
   const float inv_gl = rcp_fast(fmaxf(e.gl, 1.0e-12f));
-  e.n = inv_gl * x.g;
-  const V3 zd = v3(__fmul_rn(x.zz, y.dir.x), __fmul_rn(x.zz, y.dir.y), __fmul_rn(x.zz, y.dir.z));
+  e.n = inv_gl * g;
+  const V3 zd = v3(__fmul_rn(zz, y.dir.x), __fmul_rn(zz, y.dir.y), __fmul_rn(zz, y.dir.z));
   const V3 p = v3(__fadd_rn(y.o.x, zd.x), __fadd_rn(y.o.y, zd.y), __fadd_rn(y.o.z, zd.z));
   const double dx = (double)p.x - y.ox, dy = (double)p.y - y.oy, dz = (double)p.z - y.oz;
   const double vl = sqrt_f64_newton(fma(dx, dx, fma(dy, dy, dz * dz)));
   e.dpath = 2.0 * vl;
   const float vlf = (float)vl;
   const float inv_v = rcp_fast(vlf + 1.0e-7f);
-  e.incs = x.zz * inv_v;
+  e.incs = zz * inv_v;
   e.nu = vlf * inv_v;
-  e.idn = (e.incs * x.c) * inv_gl;
+  e.idn = (e.incs * dot(y.dir, g)) * inv_gl;
   e.spec = fmaf(2.0f * e.idn, e.idn, -e.nu * e.nu) * rcp_fast(e.nu + 1.0e-7f);
   e.live = valid && !(e.spec < 1.0e-6f || e.idn > 0.0f);
   return e;
@@ -296,7 +296,7 @@ template <int MT>
 __global__ __launch_bounds__(64 * kRaysPerBlock)
 __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kernel(
     LenslessArgs a, float* __restrict__ out_real, float* __restrict__ out_imag,
-    float* __restrict__ sigma_out, PhaseConsts pc) {
+    float* __restrict__ sigma_out, float* __restrict__ carry_out, PhaseConsts pc) {
   __shared__ FwdQueue q;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -311,9 +311,12 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
 
   float carry = 1.0f;
   int queued = 0;
+  const int ntmax = (a.S + kTile - 1) / kTile;
   for (int base = 0; base < y.nv; base += kTile) {
     const int s = base + lane;
     const bool valid = s < y.nv;
+    // the transmittance at the tile start, for the backward's single pass
+    if (carry_out != nullptr && lane == 0) carry_out[(size_t)r * ntmax + base / kTile] = carry;
     const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
     const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
     const float T = carry * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
@@ -321,7 +324,7 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
     const float Tc = T + x.off;
     const float sigma = fmaxf(x.refl * y.power * Tc, 1.0e-6f) * (x.alpha * Tc);
     if (sigma_out && valid) st_row(sigma_out + y.row, (unsigned)s, sigma);
-    const Geo e = geometry(y, x, valid);
+    const Geo e = geometry(y, x.zz, x.g, valid);
     float amp = 0.0f;
     if (e.live) amp = sigma * e.spec * rcp_fast((float)(e.dpath * e.dpath));
     const bool emit = e.live && amp != 0.0f;
@@ -560,7 +563,8 @@ __global__ __launch_bounds__(64 * kRaysPerBlock)
 __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_kernel(
     LenslessArgs a, const float* __restrict__ g_real, const float* __restrict__ g_imag,
     float* __restrict__ grad_sdf, float* __restrict__ grad_g, float* __restrict__ grad_refl,
-    float* __restrict__ grad_inv_s, float* __restrict__ grad_power, PhaseConsts pc) {
+    float* __restrict__ grad_inv_s, float* __restrict__ grad_power,
+    const float* __restrict__ carry_in, PhaseConsts pc) {
   extern __shared__ double lds_raw[];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -640,7 +644,13 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   };
 
   // pass A (forward over tiles): transmittance carries, and h_s of every
-  // emitting sample, evaluated in full 64-lane batches of such samples
+  // emitting sample, evaluated in full 64-lane batches of such samples.
+  // With the forward's carries (carry_in) there is no pass A: pass B then
+  // evaluates h_s tile by tile (each sample's h_s depends on its own delay
+  // alone, so the results are the same bit for bit).
+  const bool one_pass = carry_in != nullptr;
+  const int ntmax = (a.S + kTile - 1) / kTile;
+  if (!one_pass) {
   for (int s = lane; s < y.nv; s += kWave) L.h[s] = 0.0f;
   wave_sync();
   float carry = 1.0f;
@@ -652,7 +662,7 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
     const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
     const float incl = wave_incl_prod_dpp(valid ? x.t : 1.0f);
     carry *= readlane(incl, kWave - 1);
-    const Geo e = geometry(y, x, valid);
+    const Geo e = geometry(y, x.zz, x.g, valid);
     const unsigned long long mask = __ballot(e.live);
     if (e.live) {
       const int slot = queued + lane_prefix(mask);
@@ -682,6 +692,7 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   }
   if (queued > 0) horner_batch(queued);
   wave_sync();
+  }
 
   // pass B: reverse over tiles
   float w_carry = 0.0f;  // W at the first sample of the next tile
@@ -692,17 +703,34 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   for (int tile = ntiles - 1; tile >= 0; --tile) {
     const int s = tile * kTile + lane;
     const bool valid = s < y.nv;
+    if (one_pass) {
+      // h_s of this tile's emitting samples (which are known from z and the
+      // SDF gradient alone), before the compositing terms are formed
+      const unsigned u = (unsigned)(valid ? s : y.nv - 1);
+      const Geo e = geometry(y, ld_row(a.z + y.row, u), ld_row3(a.grad + 3 * y.row, u), valid);
+      const unsigned long long mask = __ballot(e.live);
+      if (e.live) {
+        const int slot = lane_prefix(mask);
+        L.btau[slot] = (e.dpath + pc.bias) * pc.inv_c;
+        L.bsid[slot] = s;
+      }
+      wave_sync();
+      const int cnt = __popcll(mask);
+      if (cnt > 0) horner_batch(cnt);
+      wave_sync();
+    }
     const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
     const float tt = valid ? x.t : 1.0f;
     const float incl = wave_incl_prod_dpp(tt);
-    const float T = L.carry[tile] * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
+    const float tile_T = one_pass ? carry_in[(size_t)r * ntmax + tile] : L.carry[tile];
+    const float T = tile_T * dpp_mov<dpp::kWaveShr1>(1.0f, incl);
     const float Tc = T + x.off;
     const float rp = x.refl * y.power;
     const float m = rp * Tc;
     const bool unclamped = m >= 1.0e-6f;
     const float aTc = x.alpha * Tc;
     const float sigma = fmaxf(m, 1.0e-6f) * aTc;
-    const Geo e = geometry(y, x, valid);
+    const Geo e = geometry(y, x.zz, x.g, valid);
 
     float h = 0.0f, att = 0.0f;
     if (e.live) {
@@ -812,8 +840,8 @@ extern "C" int geraf_lensless_forward_f32(
     const float* sdf, const float* grad, const float* refl, const float* dists, const float* z,
     const float* origin, const float* dirs, const float* trans_offset, const int32_t* n_valid,
     const float* inv_s, const float* power, float* out_real, float* out_imag, float* sigma_out,
-    int64_t R, int64_t S, int64_t K, float cos_anneal_ratio, const geraf_fmcw_params* params,
-    geraf_stream_t stream) {
+    float* tile_carry, int64_t R, int64_t S, int64_t K, float cos_anneal_ratio,
+    const geraf_fmcw_params* params, geraf_stream_t stream) {
   GERAF_TRY_BEGIN
   check_params(params);
   const LenslessArgs a = make_args(sdf, grad, refl, dists, z, origin, dirs, trans_offset,
@@ -828,16 +856,16 @@ extern "C" int geraf_lensless_forward_f32(
   hipStream_t st = (hipStream_t)stream;
   if (K <= 256)
     hipLaunchKernelGGL(lensless_forward_kernel<1>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc);
+                       sigma_out, tile_carry, pc);
   else if (K <= 512)
     hipLaunchKernelGGL(lensless_forward_kernel<2>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc);
+                       sigma_out, tile_carry, pc);
   else if (K <= 1024)
     hipLaunchKernelGGL(lensless_forward_kernel<4>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc);
+                       sigma_out, tile_carry, pc);
   else
     hipLaunchKernelGGL(lensless_forward_kernel<8>, grid, block, 0, st, a, out_real, out_imag,
-                       sigma_out, pc);
+                       sigma_out, tile_carry, pc);
   return check_launch("lensless_forward_kernel");
   GERAF_TRY_END
 }
@@ -847,8 +875,9 @@ extern "C" int geraf_lensless_backward_f32(
     const float* refl, const float* dists, const float* z, const float* origin,
     const float* dirs, const float* trans_offset, const int32_t* n_valid, const float* inv_s,
     const float* power, float* grad_sdf, float* grad_grad, float* grad_refl,
-    float* grad_inv_s_per_ray, float* grad_power_per_ray, int64_t R, int64_t S, int64_t K,
-    float cos_anneal_ratio, const geraf_fmcw_params* params, geraf_stream_t stream) {
+    float* grad_inv_s_per_ray, float* grad_power_per_ray, const float* tile_carry, int64_t R,
+    int64_t S, int64_t K, float cos_anneal_ratio, const geraf_fmcw_params* params,
+    geraf_stream_t stream) {
   GERAF_TRY_BEGIN
   check_params(params);
   const LenslessArgs a = make_args(sdf, grad, refl, dists, z, origin, dirs, trans_offset,
@@ -867,7 +896,7 @@ extern "C" int geraf_lensless_backward_f32(
 #define GERAF_LL_BWD(MT_)                                                                      \
   hipLaunchKernelGGL(lensless_backward_kernel<MT_>, grid, block, lds, st, a, grad_out_real,   \
                      grad_out_imag, grad_sdf, grad_grad, grad_refl, grad_inv_s_per_ray,        \
-                     grad_power_per_ray, pc)
+                     grad_power_per_ray, tile_carry, pc)
   // h_s: two-level Horner up to K = 128, where it measured 5-26 % faster
   // than the FP16 matrix form
   if (K <= 128)

@@ -66,12 +66,16 @@ class _LenslessTracerFunction(Function):
                              ray_d, trans_offset, n_valid)
         out_r = torch.empty((R, K), dtype=torch.float32, device=sdf.device)
         out_i = torch.empty_like(out_r)
+        # transmittance at each 64-sample tile start: the backward then walks
+        # the samples once instead of twice
+        carry = torch.empty((R, (S + 63) // 64), dtype=torch.float32, device=sdf.device)
         prm = _C.tracer_params(fc_start, As_sci, adcSampleRate)
         with torch.cuda.device(sdf.device):
             _native.call("geraf_lensless_forward_f32",
                          *map(_native.ptr, (ins["sdf"], ins["grad"], ins["refl"], ins["dists"],
                                             ins["z"], ins["o"], ins["d"], ins["off"], ins["nv"],
-                                            ins["inv_s"], ins["power"], out_r, out_i, None)),
+                                            ins["inv_s"], ins["power"], out_r, out_i, None,
+                                            carry)),
                          R, S, K, float(cos_anneal_ratio), prm, _native.stream_of(out_r))
         # the inputs themselves are saved (autograd then refuses a backward
         # after any of them was modified in place); the backward converts
@@ -79,6 +83,7 @@ class _LenslessTracerFunction(Function):
         ctx.save_for_backward(sdf, gradients, reflectance, inv_s, power, dists, z_vals, ray_o,
                               ray_d, trans_offset, n_valid)
         ctx.cfg = (R, S, K, float(cos_anneal_ratio), prm)
+        ctx.carry = carry
         return out_r, out_i
 
     @staticmethod
@@ -102,8 +107,9 @@ class _LenslessTracerFunction(Function):
                          *map(_native.ptr, (g_r, g_i, ins["sdf"], ins["grad"], ins["refl"],
                                             ins["dists"], ins["z"], ins["o"], ins["d"],
                                             ins["off"], ins["nv"], ins["inv_s"], ins["power"],
-                                            g_sdf, g_grad, g_refl, g_inv, g_pow)),
+                                            g_sdf, g_grad, g_refl, g_inv, g_pow, ctx.carry)),
                          R, S, K, anneal, prm, _native.stream_of(g_sdf))
+        ctx.carry = None
         need = ctx.needs_input_grad
         g_inv_s = g_inv.sum().reshape(inv_s.shape) if need[3] else None
         g_power = g_pow.sum().reshape(power.shape) if need[4] else None

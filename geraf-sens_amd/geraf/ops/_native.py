@@ -67,8 +67,8 @@ _SIGNATURES = {
     "geraf_bilinear_forward_f64": [_P] * 7 + [_I64] * 4 + [_P],
     "geraf_bilinear_backward_f32": [_P] * 6 + [_I64] * 4 + [_P],
     "geraf_bilinear_backward_f64": [_P] * 6 + [_I64] * 4 + [_P],
-    "geraf_lensless_forward_f32": [_P] * 14 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
-    "geraf_lensless_backward_f32": [_P] * 18 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
+    "geraf_lensless_forward_f32": [_P] * 15 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
+    "geraf_lensless_backward_f32": [_P] * 19 + [_I64] * 3 + [ctypes.c_float, _PARAMS, _P],
     "geraf_softplus_bf16": [_P] * 2 + [_I64, ctypes.c_float, ctypes.c_float, _P],
     "geraf_softplus_grad_bf16": [_P] * 3 + [_I64, ctypes.c_float, ctypes.c_float, _P],
     "geraf_softplus_grad2_bf16": [_P] * 5 + [_I64, ctypes.c_float, ctypes.c_float, _P],

@@ -52,15 +52,16 @@ def run_tracer(dev, bb) -> None:
     lr = torch.empty((R, Kt), device=dev)
     li = torch.empty_like(lr)
     st = _native.stream_of(lr)
-    _native.call("geraf_lensless_forward_f32", *ins, P(lr), P(li), P(None), R, S, Kt, bb.anneal,
-                 tp, st)
+    carry = torch.empty((R, (S + 63) // 64), device=dev)
+    _native.call("geraf_lensless_forward_f32", *ins, P(lr), P(li), P(None), P(carry), R, S, Kt,
+                 bb.anneal, tp, st)
     g_s = torch.empty((R, S), device=dev)
     g_g = torch.empty((R, S, 3), device=dev)
     g_r = torch.empty((R, S), device=dev)
     g_i = torch.empty(R, device=dev)
     g_p = torch.empty(R, device=dev)
     _native.call("geraf_lensless_backward_f32", P(lr), P(li), *ins, P(g_s), P(g_g), P(g_r),
-                 P(g_i), P(g_p), R, S, Kt, bb.anneal, tp, st)
+                 P(g_i), P(g_p), P(carry), R, S, Kt, bb.anneal, tp, st)
     torch.cuda.synchronize()
 
 

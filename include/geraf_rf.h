@@ -251,33 +251,40 @@ int geraf_bilinear_backward_f64(const double* grad_out_normals, const double* gr
  *   out[r,k] = (1/n_valid[r]) sum_s amp_s exp(+j*phi_{s,k})
  * sdf, dists, z, refl, trans_offset: [R,S]; grad: [R,S,3]; origin, dirs: [R,3];
  * n_valid: [R] int32; inv_s, power: device scalars [1]; out_real/out_imag
- * [R,K]; sigma_out [R,S] (optional). refl, trans_offset, n_valid, sigma_out
- * may be NULL (1, 0, S, not written). Limits: S <= 2048, K <= 2048.
+ * [R,K]; sigma_out [R,S] (optional); tile_carry [R, ceil(S/64)] (optional):
+ * the transmittance at the start of each 64-sample tile, for the backward.
+ * refl, trans_offset, n_valid, sigma_out, tile_carry may be NULL (1, 0, S,
+ * not written, not written). Limits: S <= 2048, K <= 2048.
  * No counterpart in the reference: it fuses the torch glue of
  * rf_rendering.py:740-780 with the tracer op for the per-ray case. */
 int geraf_lensless_forward_f32(const float* sdf, const float* grad, const float* refl,
                                const float* dists, const float* z, const float* origin,
                                const float* dirs, const float* trans_offset,
                                const int32_t* n_valid, const float* inv_s, const float* power,
-                               float* out_real, float* out_imag, float* sigma_out, int64_t R,
-                               int64_t S, int64_t K, float cos_anneal_ratio,
-                               const geraf_fmcw_params* params, geraf_stream_t stream);
+                               float* out_real, float* out_imag, float* sigma_out,
+                               float* tile_carry, int64_t R, int64_t S, int64_t K,
+                               float cos_anneal_ratio, const geraf_fmcw_params* params,
+                               geraf_stream_t stream);
 
 /* Adjoint of geraf_lensless_forward_f32: gradients into sdf, grad (the raw
  * SDF gradients, through iter_cos and the normalised normal), refl, and per-ray
  * partial sums of the inv_s and power gradients ([R] each; the caller sums
  * them). Recomputes the compositing instead of reading stored intermediates.
- * grad_refl may be NULL. Samples past n_valid get zeros. Limits as the
- * forward (S <= 2048, K <= 2048). */
+ * grad_refl may be NULL. Samples past n_valid get zeros. tile_carry: the
+ * forward's tile_carry output for the same inputs, or NULL; with it the
+ * kernel walks the samples once (without it, twice: the first walk rebuilds
+ * the carries); the results are the same bit for bit. Limits as the forward
+ * (S <= 2048, K <= 2048). */
 int geraf_lensless_backward_f32(const float* grad_out_real, const float* grad_out_imag,
                                 const float* sdf, const float* grad, const float* refl,
                                 const float* dists, const float* z, const float* origin,
                                 const float* dirs, const float* trans_offset,
                                 const int32_t* n_valid, const float* inv_s, const float* power,
                                 float* grad_sdf, float* grad_grad, float* grad_refl,
-                                float* grad_inv_s_per_ray, float* grad_power_per_ray, int64_t R,
-                                int64_t S, int64_t K, float cos_anneal_ratio,
-                                const geraf_fmcw_params* params, geraf_stream_t stream);
+                                float* grad_inv_s_per_ray, float* grad_power_per_ray,
+                                const float* tile_carry, int64_t R, int64_t S, int64_t K,
+                                float cos_anneal_ratio, const geraf_fmcw_params* params,
+                                geraf_stream_t stream);
 
 /* Sphere hits of the transmittance calibration (GeRaFStage1.intersect_sphere,
  * rf_rendering.py:455-495): for each of M points p [M,3] (SDF frame) and the

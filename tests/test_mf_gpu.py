@@ -159,6 +159,44 @@ def test_mma_forward_per_antenna_scaling(cuda):
     assert rel_l2(run_mf((echo, tx, rx, vox, prm), cuda, "mma"), ref) <= TOL_TUNED_VS_F64
 
 
+def test_mma_forward_weak_scatterers_per_voxel(cuda):
+    """A strong return and weak scatterers 60 and 80 dB below it (16x16
+    aperture, K=512, FP64-synthesised echo): the weak scatterers' voxels,
+    each on its own, against the FP64 oracle.  Their signal reaches every
+    frequency bin at 1e-3 / 1e-4 of the strong one, so the FP16 hi/lo
+    operands (2^-22 relative per entry of the row, scaled to the row
+    maximum) leave them an absolute error of that order of the strong
+    voxel's value: the bound is 2^-18 of the strong voxel's magnitude, and
+    relative 1e-3 (-60 dB) / 1e-2 (-80 dB) on the weak voxels themselves."""
+    K = 512
+    fs = 1.25e6 * K / 64
+    slope = float(np.float32(701500.0))
+    ap = mf_scenes.planar_grid(16, 3.9e-3).astype(np.float32)
+    c = np.array([0.0, 0.0, 0.30])
+    rng = np.random.default_rng(21)
+    offs = rng.uniform(-0.06, 0.06, (8, 3)) * np.array([1, 1, 0.5])
+    scat = np.concatenate([c[None], c + offs])
+    amp = np.array([1.0] + [1e-3] * 4 + [1e-4] * 4, dtype=np.complex128)
+    amp[1:] *= np.exp(2j * np.pi * rng.uniform(0, 1, 8))
+    echo = mf_scenes.synth_echo(ap, ap, scat, amp, K, fs, slope).numpy().astype(np.complex64)
+    vox = scat.astype(np.float32)
+    prm = dict(f0=mf_scenes.MF_F0, slope=slope, fs=fs, c=mf_scenes.SPEED_OF_LIGHT,
+               bias=mf_scenes.PATH_BIAS)
+    ref = orc.mf_forward(echo.real, echo.imag, ap, ap, vox, **prm)
+    out = run_mf((echo, ap, ap, vox, prm), cuda, "mma")
+    horner = run_mf((echo, ap, ap, vox, prm), cuda, "f32")
+    strong = abs(ref[0])
+    for name, sel, rel_tol in (("-60 dB", slice(1, 5), 1e-3), ("-80 dB", slice(5, 9), 1e-2)):
+        err = np.abs(out[sel] - ref[sel])
+        rel = err / np.abs(ref[sel])
+        herr = np.abs(horner[sel] - ref[sel]) / np.abs(ref[sel])
+        print(f"{name}: |ref| / strong {np.abs(ref[sel]).max() / strong:.1e}; matrix-core "
+              f"max rel {rel.max():.2e} (abs / strong {err.max() / strong:.2e}); Horner max rel "
+              f"{herr.max():.2e}", flush=True)
+        assert np.all(err <= 2.0 ** -18 * strong) and np.all(rel <= rel_tol)
+    assert abs(out[0] - ref[0]) <= 1e-5 * strong
+
+
 def test_mma_forward_cfg1_and_drop_in_dispatch(cuda, cfg1):
     """The 16x16 / K=128 / 32^3 volume through the matrix-core entry point;
     and the drop-in op at K = 512 (it dispatches to the matrix-core form)

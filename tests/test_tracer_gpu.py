@@ -304,6 +304,62 @@ def test_lensless_gradients_vs_finite_differences(cuda, K):
     assert checked >= 25
 
 
+@pytest.mark.parametrize("R,S,K,offset", [(200, 96, 64, True), (64, 256, 512, False),
+                                          (3, 2048, 129, True), (32, 80, 2048, False),
+                                          (96, 80, 17, True)])
+def test_lensless_backward_with_forward_carries(cuda, R, S, K, offset):
+    """The backward given the forward's per-tile transmittance carries (one
+    walk over the samples, h_s tile by tile) equals the backward that
+    rebuilds them (two walks, h_s in 64-sample batches) bit for bit, on
+    ragged rays (some empty), with and without offsets, over the Horner,
+    matrix-core and largest-S forms."""
+    from geraf.ops import _native
+    from geraf.ops.RFSignalTracer import _C
+
+    b = ts.rays(R, S, K, seed=K + S, with_offset=offset, uneven=(0 if R > 1 else 1, S)).to(cuda)
+    P = _native.ptr
+    r = b.radar
+    prm = _C.tracer_params(r["fc_start"], r["As_sci"], r["adcSampleRate"])
+    inv_s, power = b.inv_s.reshape(1).contiguous(), b.power.reshape(1).contiguous()
+    nv = b.n_valid.to(torch.int32).contiguous()
+    ins = [P(b.sdf), P(b.grad), P(b.refl), P(b.dists), P(b.z), P(b.o), P(b.d), P(b.offset),
+           P(nv), P(inv_s), P(power)]
+    o_r = torch.empty((R, K), device=cuda)
+    o_i = torch.empty_like(o_r)
+    carry = torch.full((R, (S + 63) // 64), float("nan"), device=cuda)
+    st = _native.stream_of(o_r)
+    _native.call("geraf_lensless_forward_f32", *ins, P(o_r), P(o_i), P(None), P(carry), R, S, K,
+                 b.anneal, prm, st)
+    o2_r, o2_i = torch.empty_like(o_r), torch.empty_like(o_r)
+    _native.call("geraf_lensless_forward_f32", *ins, P(o2_r), P(o2_i), P(None), P(None), R, S, K,
+                 b.anneal, prm, st)
+    torch.cuda.synchronize()
+    assert torch.equal(o_r, o2_r) and torch.equal(o_i, o2_i)
+    # the carries of the tiles with valid samples: products of the first
+    # 64 t samples, 1 at the first tile
+    ntile = (nv.long() + 63) // 64
+    written = torch.arange(carry.shape[1], device=cuda)[None, :] < ntile[:, None]
+    assert torch.all(torch.isfinite(carry[written]))
+    assert torch.all(carry[:, 0][ntile > 0] == 1.0)
+    assert torch.all(torch.isnan(carry[~written]))
+
+    gen = torch.Generator(device=cuda).manual_seed(K)
+    g_r = torch.randn((R, K), device=cuda, generator=gen)
+    g_i = torch.randn((R, K), device=cuda, generator=gen)
+    outs = []
+    for tc in (carry, None):
+        o = dict(gs=torch.full((R, S), 7.0, device=cuda), gg=torch.full((R, S, 3), 7.0, device=cuda),
+                 gr=torch.full((R, S), 7.0, device=cuda), gi=torch.empty(R, device=cuda),
+                 gp=torch.empty(R, device=cuda))
+        _native.call("geraf_lensless_backward_f32", P(g_r), P(g_i), *ins, P(o["gs"]), P(o["gg"]),
+                     P(o["gr"]), P(o["gi"]), P(o["gp"]), P(tc), R, S, K, b.anneal, prm, st)
+        outs.append(o)
+    torch.cuda.synchronize()
+    for k in outs[0]:
+        assert torch.equal(outs[0][k], outs[1][k]), k
+    assert bool(torch.any(outs[0]["gs"] != 0))
+
+
 def test_lensless_backward_refuses_inputs_modified_in_place(cuda):
     """The op saves its inputs for the backward: modifying one in place after
     the forward makes autograd raise instead of giving wrong gradients."""
