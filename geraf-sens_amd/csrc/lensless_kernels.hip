@@ -80,12 +80,6 @@ struct LenslessArgs {
   float anneal;
 };
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Element `i` of a row whose base is wave-uniform (SGPR base + 32-bit offset).
 __device__ __forceinline__ float ld_row(const float* row, unsigned i) {
   return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(row) + i * 4u);

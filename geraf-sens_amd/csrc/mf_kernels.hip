@@ -377,143 +377,58 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_kernel(
   }
 }
 
-// Matrix-core adjoint (128 < K <= 1024).  Block = antenna a, voxels staged in
-// tiles of 512 as above (cg = g_v exp(+j 2 pi f0 tau), the FP64 step cyc1, and
-// the tile rotation exp(+j 2 pi 256 cyc1)).  With k = 16 al + b (b < 16)
-//   grad[a, 16 al + b] = sum_v X[al, v] Y[v, b],
-//   X = cg_v exp(+j 2 pi 16 al cyc1_v),  Y = exp(+j 2 pi b cyc1_v),
-// so each wave takes 64 of the tile's voxels as 16 four-deep steps of
-// v_mfma_f32_16x16x4f32 over all MT 16-row tiles, three real products per
-// complex one (P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi)).  Operand
-// lanes: A[i][k] and B[k][j] from lane i + 16 k (j + 16 k); D[4 q + v][j] is
-// element v of lane j + 16 q.  Each lane forms one Y and one X phasor per
-// step (phase j cyc1 and 16 j cyc1 reduced exactly in FP32 from the split
-// step, hardware sine and cosine); further tiles follow by one rotation.
-// After every tile the FP32 matrix sums (64 voxels) are added into FP64
-// registers; the eight waves' FP64 sums are reduced through LDS in a fixed
-// order.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-struct AdjMfmaStage {
-  float2 cg[kAdjTile];        // output gradient times the carrier phasor
-  float2 rot[kAdjTile];       // exp(+j 2 pi 256 cyc1): one 256-frequency tile on
-  SplitCycles cyc1[kAdjTile];  // per-sample step [cycles], split for exact multiples
-};
-
-template <int MT>
-__global__ __launch_bounds__(kAdjTile) void mf_backward_mfma_kernel(
-    const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
-    const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
-    float* __restrict__ grad_real, float* __restrict__ grad_imag,
-    const float* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc) {
-  __shared__ AdjMfmaStage st;
-  __shared__ double red[kAdjWaves][kAdjTile];  // one tile's (Re, Im) x 4 rows per lane
-  const int a = blockIdx.x;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int j = lane & 15;
-  const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
-  const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
-
-  double sr[MT][4], si[MT][4];
-#pragma unroll
-  for (int m = 0; m < MT; ++m)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) sr[m][v] = si[m][v] = 0.0;
-
-  for (int base = 0; base < N; base += kAdjTile) {
-    {
-      const int v = base + threadIdx.x;
-      float2 cg = make_float2(0.0f, 0.0f), rot = make_float2(1.0f, 0.0f);
-      double cyc1 = 0.0;
-      if (v < N) {
-        const float gr = gout_real[v], gi = gout_imag[v];
-        const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
-        const double d =
-            dist3_f64_fast(tx, ty, tz, px, py, pz) + dist3_f64_fast(rx, ry, rz, px, py, pz);
-        const double tau = (d + pc.bias) * pc.inv_c;
-        cyc1 = pc.dfs * tau;
-        cg = cmul(make_float2(gr, gi), phasor_cycles((float)centered_frac(pc.f0 * tau), 1.0f));
-        if (MT > 1) rot = phasor_cycles_hw((float)centered_frac(256.0 * cyc1));
-      }
-      st.cg[threadIdx.x] = cg;
-      st.rot[threadIdx.x] = rot;
-      st.cyc1[threadIdx.x] = split_cycles(cyc1);
-    }
-    __syncthreads();
-    f32x4 p1[MT], p2[MT], p3[MT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m) p1[m] = p2[m] = p3[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 4
-    for (int t = 0; t < 16; ++t) {
-      const int i = wave * kWave + 4 * t + (lane >> 4);  // voxel of this lane's step
-      const SplitCycles c1 = st.cyc1[i];
-      const float2 y = phasor_cycles_hw(frac_multiple((float)j, c1));
-      float2 x = cmul(st.cg[i], phasor_cycles_hw(frac_multiple((float)(16 * j), c1)));
-      float2 rot = make_float2(1.0f, 0.0f);
-      if constexpr (MT > 1) rot = st.rot[i];
-      const float ys = y.x + y.y;
-#pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        if (m > 0) x = cmul(x, rot);  // X[al + 16, v] = X[al, v] exp(j 2 pi 256 cyc1)
-        p1[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, p1[m], 0, 0, 0);
-        p2[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, p2[m], 0, 0, 0);
-        p3[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x + x.y, ys, p3[m], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        sr[m][v] += (double)(p1[m][v] - p2[m][v]);
-        si[m][v] += (double)(p3[m][v] - p1[m][v] - p2[m][v]);
-      }
-    __syncthreads();
-  }
-
-  // reduce the waves, one 256-frequency tile at a time: thread t sums entry
-  // t = 8 lane + 2 v + (0: Re, 1: Im) over the waves
-  const int tl = threadIdx.x >> 3, te = threadIdx.x & 7;
-  const int k_of_t = 64 * (tl >> 4) + 16 * (te >> 1) + (tl & 15);
-#pragma unroll
-  for (int m = 0; m < MT; ++m) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      red[wave][8 * lane + 2 * v] = sr[m][v];
-      red[wave][8 * lane + 2 * v + 1] = si[m][v];
-    }
-    __syncthreads();
-    double acc = 0.0;
-#pragma unroll
-    for (int w = 0; w < kAdjWaves; ++w) acc += red[w][threadIdx.x];
-    const int k = 256 * m + k_of_t;
-    if (k < K) {
-      float* out = (te & 1) ? grad_imag : grad_real;
-      out[(size_t)a * K + k] = (float)(acc * inv_norm);
-    }
-    __syncthreads();
-  }
-}
-
-// The same adjoint with FP16 hi/lo operands on v_mfma_f32_16x16x32_f16
-// (depth 32 voxels per instruction instead of 4): A[i][d] = X[16 m + i, v_d]
+// Matrix-core adjoint (128 < K <= 1024).  Block = antenna a.  With
+// k = 256 m + 16 al + b (b < 16)
+//   grad[a, k] = sum_v X[16 m + al, v] Y[v, b],
+//   X = cg_v exp(+j 2 pi 16 (16 m + al) cyc1_v),  Y = exp(+j 2 pi b cyc1_v),
+// cg_v = g_v exp(+j 2 pi f0 tau_v), one 16 x 16 output tile per m,
+// on v_mfma_f32_16x16x32_f16 with FP16 hi/lo operands (depth 32 voxels per
+// instruction): A[i][d] = X[16 m + i, v_d]
 // and B[d][j] = Y[v_d, j] with d = 8 (lane / 16) + element, i.e. lane i + 16 g
 // forms X and Y for the eight voxels 8 g .. 8 g + 7 of its 32-voxel chunk.
 // Each real operand is split into hi = fp16(x) and lo = fp16(x - hi), a real
 // product is hi*hi + hi*lo + lo*hi into one FP32 accumulator, a complex
 // product four real ones (12 instructions per 16 x 16 tile and 32 voxels).
-// X and Y are unit phasors times cg; cg is scaled per voxel tile by a power
-// of two (max |cg| in [0.5, 1)) so FP16 neither overflows nor drops small
-// gradients, and the tile's sums are scaled back before the FP64 flush.
+//
+// Each wave streams its own 64-voxel bases (no block-wide synchronisation
+// until the final reduction).  Per base, lane l first forms voxel l's terms
+// and writes a table of phasor factors to the wave's LDS:
+//   Y[v, b]  = exp(+j 2 pi b c_v)               = ya[b & 3] * yb[b >> 2],
+//   X[al, v] = cg_v exp(+j 2 pi 16 al c_v)      = xa[al & 3] * xb[al >> 2],
+//   ya = w^{0,1,2,3}, yb = w^{0,4,8,12}, xa = cg w^{0,16,32,48}, xb = w^{0,64,128,192},
+// with c_v the per-frequency step in cycles, w = exp(+j 2 pi c_v) (the powers
+// 1, 4, 16, 64 and 256 from the hardware sine and cosine, the others by one
+// squaring and one product), cg_v the output gradient times the carrier
+// phasor and rot = w^256 the step to the next 256-frequency tile.  The
+// consumers then form every operand with one complex product of two table
+// entries instead of a sine and a cosine each.  cg is scaled per wave base by
+// a power of two (max |cg| in [0.5, 1)) so FP16 neither overflows nor drops
+// small gradients, and the base's sums are scaled back before the FP64 flush.
+#ifndef GERAF_ADJ16_WPE
+#define GERAF_ADJ16_WPE 3  // waves per SIMD the register allocation aims for
+#endif
+constexpr int kAdjFactors = 9;              // table entries per voxel
+constexpr int kAdjFactorStride = kWave + 1;  // float4 per entry (odd: bank spread)
+constexpr int kAdjTableFloat4 = kAdjFactors * kAdjFactorStride;  // per wave
+constexpr int kAdj16Waves = 4;                     // waves per block (one antenna)
+constexpr int kAdj16Threads = kWave * kAdj16Waves;
+constexpr int kAdjRedPerWave = 8 * kWave;  // (Re, Im) x 4 rows per lane, one tile
+
+__device__ __forceinline__ float2 csq(float2 a) {
+  return make_float2(fmaf(a.x, a.x, -a.y * a.y), 2.0f * a.x * a.y);
+}
+
 template <int MT>
-__global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
+__global__ __launch_bounds__(kAdj16Threads)
+__attribute__((amdgpu_waves_per_eu(MT >= 4 ? 2 : GERAF_ADJ16_WPE))) void mf_backward_mma16_kernel(
     const float* __restrict__ gout_real, const float* __restrict__ gout_imag,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
     float* __restrict__ grad_real, float* __restrict__ grad_imag,
     const float* __restrict__ voxel, int A, int K, int N, double inv_norm, PhaseConsts pc) {
-  __shared__ AdjMfmaStage st;
-  __shared__ double red[kAdjWaves][kAdjTile];
-  __shared__ float wmax[kAdjWaves];
+  // the waves' factor tables; after the voxel loop, the cross-wave reduction
+  __shared__ float4 lds4[kAdj16Waves * kAdjTableFloat4];
+  static_assert(sizeof(lds4) >= sizeof(double) * kAdj16Waves * kAdjRedPerWave,
+                "reduction space");
   const int a = blockIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -521,6 +436,11 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
   const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
   const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
   const bool mono = monostatic(tx_pos, rx_pos, a);
+  float4* tbl = lds4 + wave * kAdjTableFloat4;
+  // this lane's entries: ya/xa of index i & 3, yb/xb of index i >> 2
+  const float4* tp = tbl + (i & 3) * kAdjFactorStride;
+  const float4* tq = tbl + (4 + (i >> 2)) * kAdjFactorStride;
+  const float4* tr = tbl + 8 * kAdjFactorStride;
 
   double sr[MT][4], si[MT][4];
 #pragma unroll
@@ -528,69 +448,86 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
 #pragma unroll
     for (int v = 0; v < 4; ++v) sr[m][v] = si[m][v] = 0.0;
 
-  for (int base = 0; base < N; base += kAdjTile) {
-    float cmax;
+  for (int base = wave * kWave; base < N; base += kAdj16Threads) {
+    double up;
     {
-      const int v = base + threadIdx.x;
-      float2 cg = make_float2(0.0f, 0.0f), rot = make_float2(1.0f, 0.0f);
-      double cyc1 = 0.0;
+      const int v = base + lane;
+      float2 cg = make_float2(0.0f, 0.0f);
+      SplitCycles c1 = {0.0f, 0.0f};
       if (v < N) {
         const float gr = gout_real[v], gi = gout_imag[v];
         const double px = voxel[v], py = voxel[(size_t)N + v], pz = voxel[2 * (size_t)N + v];
         const double d = mono ? round_trip_f64<true>(tx, ty, tz, rx, ry, rz, px, py, pz)
                               : round_trip_f64<false>(tx, ty, tz, rx, ry, rz, px, py, pz);
         const double tau = (d + pc.bias) * pc.inv_c;
-        cyc1 = pc.dfs * tau;
-        cg = cmul(make_float2(gr, gi), phasor_cycles((float)centered_frac(pc.f0 * tau), 1.0f));
-        if (MT > 1) rot = phasor_cycles_hw((float)centered_frac(256.0 * cyc1));
+        cg = cmul(make_float2(gr, gi), phasor_cycles_hw((float)centered_frac(pc.f0 * tau)));
+        c1 = split_cycles(pc.dfs * tau);
       }
-      float mx = fmaxf(fabsf(cg.x), fabsf(cg.y));
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
-      if (lane == 0) wmax[wave] = mx;
-      st.cg[threadIdx.x] = cg;
-      st.rot[threadIdx.x] = rot;
-      st.cyc1[threadIdx.x] = split_cycles(cyc1);
+      const float mx = wave_max_dpp(fmaxf(fabsf(cg.x), fabsf(cg.y)));
+      // cg * 2^-e with 2^e > max |cg| (exact); the base's sums are scaled back
+      int e = 0;
+      if (mx > 0.0f && isfinite(mx)) (void)frexpf(mx, &e);
+      const float down = ldexpf(1.0f, -e);
+      up = ldexp(1.0, e);
+      cg = make_float2(cg.x * down, cg.y * down);
+      const float2 w1 = phasor_cycles_hw(frac_multiple(1.0f, c1));
+      const float2 w2 = csq(w1), w3 = cmul(w2, w1);
+      const float2 w4 = phasor_cycles_hw(frac_multiple(4.0f, c1));
+      const float2 w8 = csq(w4), w12 = cmul(w8, w4);
+      const float2 w16 = phasor_cycles_hw(frac_multiple(16.0f, c1));
+      const float2 w32 = csq(w16), w48 = cmul(w32, w16);
+      const float2 w64 = phasor_cycles_hw(frac_multiple(64.0f, c1));
+      const float2 w128 = csq(w64), w192 = cmul(w128, w64);
+      const float2 x1 = cmul(cg, w16), x2 = cmul(cg, w32), x3 = cmul(cg, w48);
+      tbl[0 * kAdjFactorStride + lane] = make_float4(1.0f, 0.0f, cg.x, cg.y);
+      tbl[1 * kAdjFactorStride + lane] = make_float4(w1.x, w1.y, x1.x, x1.y);
+      tbl[2 * kAdjFactorStride + lane] = make_float4(w2.x, w2.y, x2.x, x2.y);
+      tbl[3 * kAdjFactorStride + lane] = make_float4(w3.x, w3.y, x3.x, x3.y);
+      tbl[4 * kAdjFactorStride + lane] = make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+      tbl[5 * kAdjFactorStride + lane] = make_float4(w4.x, w4.y, w64.x, w64.y);
+      tbl[6 * kAdjFactorStride + lane] = make_float4(w8.x, w8.y, w128.x, w128.y);
+      tbl[7 * kAdjFactorStride + lane] = make_float4(w12.x, w12.y, w192.x, w192.y);
+      if constexpr (MT > 1) {
+        const float2 rot = phasor_cycles_hw(frac_multiple(256.0f, c1));
+        tbl[8 * kAdjFactorStride + lane] = make_float4(rot.x, rot.y, 0.0f, 0.0f);
+      }
     }
-    __syncthreads();
-    cmax = wmax[0];
+    wave_sync();
+    // Re = sum Xr Yr - sum Xi Yi from two accumulators (no negated operand),
+    // Im = sum (Xr Yi + Xi Yr)
+    f32x4v prr[MT], pii[MT], pim[MT];
 #pragma unroll
-    for (int w = 1; w < kAdjWaves; ++w) cmax = fmaxf(cmax, wmax[w]);
-    // cg * 2^-e with 2^e > max |cg| (exact); the tile's sums are scaled back
-    int e = 0;
-    if (cmax > 0.0f && isfinite(cmax)) (void)frexpf(cmax, &e);
-    const float down = ldexpf(1.0f, -e);
-    const double up = ldexp(1.0, e);
-    f32x4v pr[MT], pi[MT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m) pr[m] = pi[m] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int m = 0; m < MT; ++m) prr[m] = pii[m] = pim[m] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 1
     for (int c = 0; c < 2; ++c) {
-      const int v0 = wave * kWave + 32 * c + 8 * g;  // this lane's eight voxels
-      float2 x[8], y[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const SplitCycles c1 = st.cyc1[v0 + q];
-        y[q] = phasor_cycles_hw(frac_multiple((float)i, c1));
-        const float2 cg = st.cg[v0 + q];
-        x[q] = cmul(make_float2(cg.x * down, cg.y * down),
-                    phasor_cycles_hw(frac_multiple((float)(16 * i), c1)));
-      }
+      const int v0 = 32 * c + 8 * g;  // this lane's eight voxels (of the base)
+      float2 x[8];
       h8 yrh, yrl, yih, yil;
-      split8(y, yrh, yrl, yih, yil);
-      const h8 nyih = -yih, nyil = -yil;  // Re = Xr Yr + Xi (-Yi), for every tile
+      {
+        float2 y[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const float4 P = tp[v0 + q], Q = tq[v0 + q];
+          y[q] = cmul(make_float2(P.x, P.y), make_float2(Q.x, Q.y));
+          x[q] = cmul(make_float2(P.z, P.w), make_float2(Q.z, Q.w));
+        }
+        split8(y, yrh, yrl, yih, yil);
+      }
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         if (m > 0) {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = cmul(x[q], st.rot[v0 + q]);
+          for (int q = 0; q < 8; ++q) {
+            const float4 R = tr[v0 + q];
+            x[q] = cmul(x[q], make_float2(R.x, R.y));
+          }
         }
         h8 xrh, xrl, xih, xil;
         split8(x, xrh, xrl, xih, xil);
-        pr[m] = mma3(xrh, xrl, yrh, yrl, pr[m]);
-        pr[m] = mma3(xih, xil, nyih, nyil, pr[m]);
-        pi[m] = mma3(xrh, xrl, yih, yil, pi[m]);
-        pi[m] = mma3(xih, xil, yrh, yrl, pi[m]);
+        prr[m] = mma3(xrh, xrl, yrh, yrl, prr[m]);
+        pii[m] = mma3(xih, xil, yih, yil, pii[m]);
+        pim[m] = mma3(xrh, xrl, yih, yil, pim[m]);
+        pim[m] = mma3(xih, xil, yrh, yrl, pim[m]);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -598,15 +535,17 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
     for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        sr[m][v] += (double)pr[m][v] * up;
-        si[m][v] += (double)pi[m][v] * up;
+        sr[m][v] += (double)(prr[m][v] - pii[m][v]) * up;
+        si[m][v] += (double)pim[m][v] * up;
       }
-    __syncthreads();
+    wave_sync();  // the table is rewritten by the next base
   }
 
-  // reduce the waves, one 256-frequency tile at a time (as the FP32 form)
-  const int tl = threadIdx.x >> 3, te = threadIdx.x & 7;
-  const int k_of_t = 64 * (tl >> 4) + 16 * (te >> 1) + (tl & 15);
+  // reduce the waves in a fixed order, one 256-frequency tile at a time:
+  // entry t = 8 lane + 2 v + (0: Re, 1: Im) is row 4 (lane / 16) + v, column
+  // lane % 16 of the tile, i.e. frequency 64 (lane / 16) + 16 v + lane % 16
+  __syncthreads();
+  double (*red)[kAdjRedPerWave] = reinterpret_cast<double (*)[kAdjRedPerWave]>(lds4);
 #pragma unroll
   for (int m = 0; m < MT; ++m) {
 #pragma unroll
@@ -615,13 +554,17 @@ __global__ __launch_bounds__(kAdjTile) void mf_backward_mma16_kernel(
       red[wave][8 * lane + 2 * v + 1] = si[m][v];
     }
     __syncthreads();
-    double acc = 0.0;
 #pragma unroll
-    for (int w = 0; w < kAdjWaves; ++w) acc += red[w][threadIdx.x];
-    const int k = 256 * m + k_of_t;
-    if (k < K) {
-      float* out = (te & 1) ? grad_imag : grad_real;
-      out[(size_t)a * K + k] = (float)(acc * inv_norm);
+    for (int t = threadIdx.x; t < kAdjRedPerWave; t += kAdj16Threads) {
+      double acc = 0.0;
+#pragma unroll
+      for (int w = 0; w < kAdj16Waves; ++w) acc += red[w][t];
+      const int tl = t >> 3, te = t & 7;
+      const int k = 256 * m + 64 * (tl >> 4) + 16 * (te >> 1) + (tl & 15);
+      if (k < K) {
+        float* out = (te & 1) ? grad_imag : grad_real;
+        out[(size_t)a * K + k] = (float)(acc * inv_norm);
+      }
     }
     __syncthreads();
   }
@@ -655,9 +598,6 @@ __global__ __launch_bounds__(256) void mf_backward_f64_kernel(
   grad_imag[idx] = acc_i * inv_norm;
 }
 
-#ifndef GERAF_MF_ADJ_KERNEL
-#define GERAF_MF_ADJ_KERNEL mf_backward_mma16_kernel
-#endif
 
 // Waves wanted per launch: 8 per SIMD of the 256-CU part, so that two of them
 // are ready to dual-issue most of the time.
@@ -769,7 +709,8 @@ extern "C" int geraf_mf_backward_f32(const float* grad_out_real, const float* gr
     const double inv_norm = 1.0 / (double)norm_count;
     hipStream_t st = (hipStream_t)stream;
 #define GERAF_MF_ADJ(MT_)                                                                      \
-  hipLaunchKernelGGL(GERAF_MF_ADJ_KERNEL<MT_>, dim3((unsigned)A), dim3(kAdjTile), 0, st,     \
+  hipLaunchKernelGGL(mf_backward_mma16_kernel<MT_>, dim3((unsigned)A), dim3(kAdj16Threads), 0, \
+                     st,                                                                     \
                      grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag,     \
                      voxel, (int)A, (int)K, (int)N, inv_norm, pc)
     if (K <= 256)

@@ -54,13 +54,6 @@ constexpr int kMmaMaxTiles = 4;  // K <= 2048
 constexpr int kMmaWaves = 4;     // waves per block, 64 voxels each
 constexpr int kOpsPerTile = 6;   // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-
 // One antenna row, tile m: lane i + 16 g holds s[a, 512 m + 32 i + 8 g + e],
 // e = 0..7, scaled, as FP16 hi and lo of the real and imaginary parts (and of
 // the negated imaginary part, the operand of Re = Sr Yr - Si Yi).

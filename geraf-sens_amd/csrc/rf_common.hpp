@@ -199,6 +199,14 @@ __device__ __forceinline__ float2 cis_neg(float x) {
   return make_float2(__builtin_amdgcn_cosf(r), -__builtin_amdgcn_sinf(r));
 }
 
+// Orders one wave's LDS writes before its later LDS reads (a wave-level
+// barrier with release/acquire fences; no block synchronisation).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Complex helpers on float2 = (re, im).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
@@ -335,6 +343,15 @@ __device__ __forceinline__ float wave_total_dpp(float v) {
   v += dpp_mov<dpp::kRowHalfMirror>(0.0f, v);
   v += dpp_mov<dpp::kRowMirror>(0.0f, v);
   return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+}
+
+// Maximum over the 64 lanes, returned wave-uniform (as wave_total_dpp).
+__device__ __forceinline__ float wave_max_dpp(float v) {
+  v = fmaxf(v, dpp_mov<dpp::kQuadXor1>(v, v));
+  v = fmaxf(v, dpp_mov<dpp::kQuadXor2>(v, v));
+  v = fmaxf(v, dpp_mov<dpp::kRowHalfMirror>(v, v));
+  v = fmaxf(v, dpp_mov<dpp::kRowMirror>(v, v));
+  return fmaxf(fmaxf(readlane(v, 0), readlane(v, 16)), fmaxf(readlane(v, 32), readlane(v, 48)));
 }
 
 // Suffix composition of the affine maps W_l = b_l + m_l * W_{l+1} over the
