@@ -3,33 +3,37 @@
 //
 // Forward operator (MatchedFilter_cuda_kernel.cu:65-115):
 //   out[v] = (1/A) sum_a sum_k s[a,k] exp(-j 2 pi (f0 + k dfs) tau_av).
-// With k = 32 alpha + beta (beta < 32) the exponential factors:
+// With k = 16 alpha + beta (beta < 16) the exponential factors:
 //   exp(-j 2 pi (f0 + k dfs) tau) = X[alpha,v] Y[beta,v],
-//   X[alpha,v] = exp(-j 2 pi (f0 tau + 32 alpha dfs tau)),  Y[beta,v] = exp(-j 2 pi beta dfs tau),
+//   X[alpha,v] = exp(-j 2 pi (f0 tau + 16 alpha dfs tau)),  Y[beta,v] = exp(-j 2 pi beta dfs tau),
 // so per antenna the inner sum is a complex matrix product followed by a
 // contraction over alpha:
 //   T[alpha,v] = sum_beta S[alpha,beta] Y[beta,v],  out[v] += sum_alpha X[alpha,v] T[alpha,v].
-// T runs on v_mfma_f32_16x16x32_f16: 16 rows of alpha (one 512-frequency
-// tile), 16 voxels, depth 32 = beta.  Each real operand is split into FP16
-// hi + lo (hi = fp16(x), lo = fp16(x - hi)) and a real product is taken as
-// hi*hi + hi*lo + lo*hi (three instructions into one FP32 accumulator; the
-// dropped lo*lo is ~2^-22 relative), so the products carry the ~22 bits of
-// FP32 operands.  A complex product is four real ones (Re = Sr Yr - Si Yi,
-// Im = Sr Yi + Si Yr): 12 instructions per tile and 16 voxels.
+// T runs on v_mfma_f32_16x16x32_f16 with the complex product folded into the
+// depth: depth 32 = 16 beta x (Re, Im) of Y, and two real products
+//   Re T = sum_beta (Sr Yr - Si Yi),  Im T = sum_beta (Si Yr + Sr Yi)
+// with A operands [Sr, -Si] and [Si, Sr] against one B operand [Yr, Yi].
+// Each real operand is split into FP16 hi + lo (hi = fp16(x),
+// lo = fp16(x - hi)) and a real product is taken as hi*hi + hi*lo + lo*hi
+// (three instructions into one FP32 accumulator; the dropped lo*lo is ~2^-22
+// relative), so the products carry the ~22 bits of FP32 operands: 6
+// instructions per 16-row tile (256 frequencies) and 16 voxels.
 //
-// The echo operands (hi/lo of Re s, Im s and -Im s, scaled per antenna by a
-// power of two so that max |s| lies in [0.5, 1) and FP16 neither overflows
+// Per voxel there are 16 Y values (four per lane: one hardware phasor and
+// three complex products) and 16 NT rows of T; the MFMA rows of tile m are
+// assigned so that lane j + 16 g receives the consecutive rows
+// alpha = 4 NT g + 4 m + r (r < 4), contracted by a two-level Horner in the
+// 16- and 64-frequency steps against one X phasor per lane.
+//
+// The echo operands (hi/lo of [Sr, -Si] and [Si, Sr], scaled per antenna by
+// a power of two so that max |s| lies in [0.5, 1) and FP16 neither overflows
 // nor loses the small entries) are laid out once in operand order by
 // mf_mma_prep_kernel.  The main kernel gives each wave 64 voxels (four groups
 // of 16).  Per antenna, each lane forms its own voxel's delay in FP64 (FP64
 // distances, as the other forms) and publishes the reduced phase terms to
-// the wave through LDS, with the voxel's step phasors exp(-j 2 pi {1, 2, 4, 32}
-// dfs tau).  Per group, each lane then forms its eight Y phasors (beta =
-// 8 g .. 8 g + 7 for lane j + 16 g) from one hardware phasor and a rotation
-// tree, splits them, issues the 12 products and contracts its four rows of T
-// with X (one phasor, Horner in the 32-frequency step).  Antennas
-// accumulate per lane in FP64; the four lanes of a voxel are summed at the
-// end.
+// the wave through LDS, with the voxel's step phasors
+// exp(-j 2 pi {1, 2, 16, 64} dfs tau).  Antennas accumulate per lane in FP64;
+// the four lanes of a voxel are summed at the end.
 //
 // Operand layout of the instruction (checked on the GPU with an in-bounds
 // probe): A[i][d] from lane i + 16 (d / 8), element d % 8; B[d][j] from lane
@@ -49,18 +53,19 @@
 namespace geraf {
 namespace {
 
-constexpr int kMmaTile = 512;    // frequencies per tile: 16 rows of 32
-constexpr int kMmaMaxTiles = 4;  // K <= 2048
+constexpr int kMmaTile = 256;    // frequencies per tile: 16 rows of 16
+constexpr int kMmaMaxTiles = 8;  // K <= 2048
 constexpr int kMmaWaves = 4;     // waves per block, 64 voxels each
-constexpr int kOpsPerTile = 6;   // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
+constexpr int kOpsPerTile = 4;   // [Sr, -Si] hi, lo; [Si, Sr] hi, lo
 
-// One antenna row, tile m: lane i + 16 g holds s[a, 512 m + 32 i + 8 g + e],
-// e = 0..7, scaled, as FP16 hi and lo of the real and imaginary parts (and of
-// the negated imaginary part, the operand of Re = Sr Yr - Si Yi).
+// One antenna row, tile m of NT: lane i + 16 g holds row i of the tile,
+// alpha = 4 NT (i / 4) + 4 m + i % 4, depth elements e = 0..7 for
+// beta = 4 g + e / 2, (Re, Im) = e % 2: [Sr, -Si] and [Si, Sr] of
+// s[a, 16 alpha + beta], scaled, as FP16 hi and lo.
 // One wave per (antenna, tile); the scale (a power of two) is per antenna.
 __global__ __launch_bounds__(64) void mf_mma_prep_kernel(const float* __restrict__ real,
                                                           const float* __restrict__ imag,
-                                                          int K, int MT, h8* __restrict__ ops,
+                                                          int K, int NT, h8* __restrict__ ops,
                                                           float* __restrict__ unscale) {
   const int a = blockIdx.x;
   const int m = blockIdx.y;
@@ -78,34 +83,39 @@ __global__ __launch_bounds__(64) void mf_mma_prep_kernel(const float* __restrict
   const float down = ldexpf(1.0f, -e);
   if (m == 0 && lane == 0) unscale[a] = ldexpf(1.0f, e);
   const int i = lane & 15, g = lane >> 4;
-  h8 rh, rl, ih, il;
+  const int alpha = 4 * NT * (i >> 2) + 4 * m + (i & 3);
+  h8 reh, rel, imh, iml;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int k = kMmaTile * m + 32 * i + 8 * g + q;
+  for (int q = 0; q < 4; ++q) {
+    const int k = 16 * alpha + 4 * g + q;
     const float xr = k < K ? er[k] * down : 0.0f;
     const float xi = k < K ? ei[k] * down : 0.0f;
-    rh[q] = (_Float16)xr;
-    rl[q] = (_Float16)(xr - (float)rh[q]);
-    ih[q] = (_Float16)xi;
-    il[q] = (_Float16)(xi - (float)ih[q]);
+    // [Sr, -Si] against [Yr, Yi] is Re T; [Si, Sr] is Im T
+    const float re0 = xr, re1 = -xi, im0 = xi, im1 = xr;
+    reh[2 * q] = (_Float16)re0;
+    reh[2 * q + 1] = (_Float16)re1;
+    imh[2 * q] = (_Float16)im0;
+    imh[2 * q + 1] = (_Float16)im1;
+    rel[2 * q] = (_Float16)(re0 - (float)reh[2 * q]);
+    rel[2 * q + 1] = (_Float16)(re1 - (float)reh[2 * q + 1]);
+    iml[2 * q] = (_Float16)(im0 - (float)imh[2 * q]);
+    iml[2 * q + 1] = (_Float16)(im1 - (float)imh[2 * q + 1]);
   }
-  h8* out = ops + ((size_t)a * MT + m) * kOpsPerTile * kWave;
-  out[0 * kWave + lane] = rh;
-  out[1 * kWave + lane] = rl;
-  out[2 * kWave + lane] = ih;
-  out[3 * kWave + lane] = il;
-  out[4 * kWave + lane] = -ih;
-  out[5 * kWave + lane] = -il;
+  h8* out = ops + ((size_t)a * NT + m) * kOpsPerTile * kWave;
+  out[0 * kWave + lane] = reh;
+  out[1 * kWave + lane] = rel;
+  out[2 * kWave + lane] = imh;
+  out[3 * kWave + lane] = iml;
 }
 
-template <int MT>
+template <int NT>
 __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
     const h8* __restrict__ ops, const float* __restrict__ unscale,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
     float* __restrict__ out_real, float* __restrict__ out_imag, const float* __restrict__ voxel,
     int A, int N, PhaseConsts pc) {
   // per voxel of the wave: step cycles (hi, lo) and carrier cycles; the step
-  // phasors w1, w32 = exp(-j 2 pi {1, 32} dfs tau); w2 = w1^2, w4 = w2^2
+  // phasors w1, w2 = exp(-j 2 pi {1, 2} dfs tau), w16, w64
   __shared__ float4 par[kMmaWaves][3][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -114,6 +124,10 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
   const int j = lane & 15, g = lane >> 4;
   const int vo = min(v0 + lane, N - 1);
   const double px = voxel[vo], py = voxel[(size_t)N + vo], pz = voxel[2 * (size_t)N + vo];
+  // this lane's first row alpha = 4 NT g: frequency 64 NT g; its Y values
+  // beta = 4 g .. 4 g + 3
+  const float x_mult = (float)(64 * NT * g);
+  const float y_mult = (float)(4 * g);
 
   // per-lane sums over the antennas in FP64 (one conversion and one FMA per
   // component and antenna; more accurate than compensated FP32 and cheaper)
@@ -122,16 +136,14 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
   for (int G = 0; G < 4; ++G) acc_r[G] = acc_i[G] = 0.0;
 
   for (int a = 0; a < A; ++a) {
-    const h8* op = ops + (size_t)a * MT * kOpsPerTile * kWave + lane;
-    h8 rh[MT], rl[MT], ih[MT], il[MT], nh[MT], nl[MT];
+    const h8* op = ops + (size_t)a * NT * kOpsPerTile * kWave + lane;
+    h8 reh[NT], rel[NT], imh[NT], iml[NT];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      rh[m] = op[(m * kOpsPerTile + 0) * kWave];
-      rl[m] = op[(m * kOpsPerTile + 1) * kWave];
-      ih[m] = op[(m * kOpsPerTile + 2) * kWave];
-      il[m] = op[(m * kOpsPerTile + 3) * kWave];
-      nh[m] = op[(m * kOpsPerTile + 4) * kWave];
-      nl[m] = op[(m * kOpsPerTile + 5) * kWave];
+    for (int m = 0; m < NT; ++m) {
+      reh[m] = op[(m * kOpsPerTile + 0) * kWave];
+      rel[m] = op[(m * kOpsPerTile + 1) * kWave];
+      imh[m] = op[(m * kOpsPerTile + 2) * kWave];
+      iml[m] = op[(m * kOpsPerTile + 3) * kWave];
     }
     const double up = (double)unscale[a];
     const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
@@ -144,12 +156,13 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
     const double tau = (d + pc.bias) * pc.inv_c;
     const SplitCycles c1 = split_cycles13(pc.dfs * tau);
     {
-      const float2 w1 = cis_neg(frac_multiple(1.0f, c1));
-      const float2 w32 = cis_neg(frac_multiple(32.0f, c1));
-      const float2 w2 = cmul(w1, w1), w4 = cmul(w2, w2);
+      const float2 w1 = cis_neg_reduced(frac_multiple(1.0f, c1));
+      const float2 w16 = cis_neg_reduced(frac_multiple(16.0f, c1));
+      const float2 w64 = cis_neg_reduced(frac_multiple(64.0f, c1));
+      const float2 w2 = cmul(w1, w1);
       par[wave][0][lane] = make_float4(c1.hi, c1.lo, (float)centered_frac(pc.f0 * tau), 0.0f);
-      par[wave][1][lane] = make_float4(w1.x, w1.y, w32.x, w32.y);
-      par[wave][2][lane] = make_float4(w2.x, w2.y, w4.x, w4.y);
+      par[wave][1][lane] = make_float4(w1.x, w1.y, w2.x, w2.y);
+      par[wave][2][lane] = make_float4(w16.x, w16.y, w64.x, w64.y);
     }
     wave_sync();
     // groups unrolled (the accumulators stay in registers), one at a time: the
@@ -161,46 +174,35 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
       const float4 q1 = par[wave][1][16 * G + j];
       const float4 q2 = par[wave][2][16 * G + j];
       const SplitCycles c = {p.x, p.y};
-      const float2 w1 = make_float2(q1.x, q1.y), st = make_float2(q1.z, q1.w);
-      const float2 w2 = make_float2(q2.x, q2.y), w4 = make_float2(q2.z, q2.w);
-      // Y[8 g + e] for e = 0..7: one phasor and a rotation tree (depth <= 3)
-      const float2 y0 = cis_neg(frac_multiple((float)(8 * g), c));
-      float2 y[8];
-      y[0] = y0;
-      y[1] = cmul(y0, w1);
-      y[2] = cmul(y0, w2);
+      const float2 w1 = make_float2(q1.x, q1.y), w2 = make_float2(q1.z, q1.w);
+      const float2 w16 = make_float2(q2.x, q2.y), w64 = make_float2(q2.z, q2.w);
+      // Y[4 g + u], u = 0..3: one phasor and three products, as (Re, Im) pairs
+      float2 y[4];
+      y[0] = cis_neg_reduced(frac_multiple(y_mult, c));
+      y[1] = cmul(y[0], w1);
+      y[2] = cmul(y[0], w2);
       y[3] = cmul(y[1], w2);
-      y[4] = cmul(y0, w4);
-      y[5] = cmul(y[1], w4);
-      y[6] = cmul(y[2], w4);
-      y[7] = cmul(y[3], w4);
-      h8 yrh, yrl, yih, yil;
-      split8(y, yrh, yrl, yih, yil);
-      // rows alpha = 16 m + 4 g + r of this lane: X[alpha] = X[16 m + 4 g] st^r
-      // with st = w32; sum_r X T_r by Horner in st
-      float2 part;
+      h8 yh, yl;
+      split4c(y, yh, yl);
+      // every tile's products first (the Horner below then waits on none)
+      f32x4v tr[NT], ti[NT];
 #pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        f32x4v tr = {0.0f, 0.0f, 0.0f, 0.0f}, ti = {0.0f, 0.0f, 0.0f, 0.0f};
-        tr = mma3(rh[m], rl[m], yrh, yrl, tr);
-        tr = mma3(nh[m], nl[m], yih, yil, tr);
-        ti = mma3(rh[m], rl[m], yih, yil, ti);
-        ti = mma3(ih[m], il[m], yrh, yrl, ti);
-        float2 h = make_float2(tr[3], ti[3]);
-#pragma unroll
-        for (int r = 2; r >= 0; --r) {
-          const float2 t = cmul(h, st);
-          h = make_float2(t.x + tr[r], t.y + ti[r]);
-        }
-        const float2 x = cis_neg(p.z + frac_multiple((float)(32 * (16 * m + 4 * g)), c));
-        const float2 xh = cmul(x, h);
-        if (m == 0) {
-          part = xh;  // not 0 + xh: the compiler may not drop an add of +0
-        } else {
-          part.x += xh.x;
-          part.y += xh.y;
-        }
+      for (int m = 0; m < NT; ++m) {
+        tr[m] = ti[m] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+        tr[m] = mma3(reh[m], rel[m], yh, yl, tr[m]);
+        ti[m] = mma3(imh[m], iml[m], yh, yl, ti[m]);
       }
+      // h = sum_m w64^m sum_r w16^r T[4 NT g + 4 m + r], tiles from the last
+      float2 h;
+#pragma unroll
+      for (int m = NT - 1; m >= 0; --m) {
+        float2 t = make_float2(tr[m][3], ti[m][3]);
+#pragma unroll
+        for (int r = 2; r >= 0; --r) t = cmul_add(t, w16, make_float2(tr[m][r], ti[m][r]));
+        h = (m == NT - 1) ? t : cmul_add(h, w64, t);
+      }
+      const float2 x = cis_neg(p.z + frac_multiple(x_mult, c));
+      const float2 part = cmul(x, h);
       acc_r[G] = fma((double)part.x, up, acc_r[G]);
       acc_i[G] = fma((double)part.y, up, acc_i[G]);
       __builtin_amdgcn_sched_barrier(0);
@@ -267,22 +269,26 @@ int mma_forward(const float* real, const float* imag, const float* tx_pos, const
     throw ArgError("workspace too small (see geraf_mf_forward_mma_workspace_bytes)");
   if ((reinterpret_cast<uintptr_t>(workspace) & 255u) != 0)
     throw ArgError("workspace must be 256-byte aligned");
-  const int MT = mma_tiles(K);
+  const int NT = mma_tiles(K);
   h8* ops = reinterpret_cast<h8*>(workspace);
   float* unscale =
       reinterpret_cast<float*>(reinterpret_cast<char*>(workspace) + align256(mma_ops_bytes(A, K)));
-  hipLaunchKernelGGL(mf_mma_prep_kernel, dim3((unsigned)A, (unsigned)MT), dim3(kWave), 0, stream,
-                     real, imag, (int)K, MT, ops, unscale);
+  hipLaunchKernelGGL(mf_mma_prep_kernel, dim3((unsigned)A, (unsigned)NT), dim3(kWave), 0, stream,
+                     real, imag, (int)K, NT, ops, unscale);
   const int rc = check_launch("mf_mma_prep_kernel");
   if (rc != GERAF_RF_OK) return rc;
 #define GERAF_MMA_LAUNCH(T)                                                                   \
   hipLaunchKernelGGL((mf_forward_mma_kernel<T>), dim3(blocks), dim3(64 * kMmaWaves), 0, stream, \
                      ops, unscale, tx_pos, rx_pos, out_real, out_imag, voxel, (int)A, (int)N, pc)
-  switch (MT) {
+  switch (NT) {
     case 1: GERAF_MMA_LAUNCH(1); break;
     case 2: GERAF_MMA_LAUNCH(2); break;
     case 3: GERAF_MMA_LAUNCH(3); break;
-    default: GERAF_MMA_LAUNCH(4); break;
+    case 4: GERAF_MMA_LAUNCH(4); break;
+    case 5: GERAF_MMA_LAUNCH(5); break;
+    case 6: GERAF_MMA_LAUNCH(6); break;
+    case 7: GERAF_MMA_LAUNCH(7); break;
+    default: GERAF_MMA_LAUNCH(8); break;
   }
 #undef GERAF_MMA_LAUNCH
   return check_launch("mf_forward_mma_kernel");
@@ -356,7 +362,7 @@ extern "C" int geraf_mf_forward_f32_mma_noalloc(const float* real, const float* 
   check_mma_args(A, K, N, params);
   const int64_t need = geraf_mf_forward_mma_workspace_bytes(A, K);
   if (need > (int64_t)GERAF_MF_MMA_SCRATCH_BYTES)
-    throw ArgError("A * ceil(K / 512) too large for the static scratch: the echo needs " +
+    throw ArgError("A * ceil(K / 256) too large for the static scratch: the echo needs " +
                    std::to_string(need) + " bytes, the scratch holds " +
                    std::to_string((int64_t)GERAF_MF_MMA_SCRATCH_BYTES) +
                    " (use geraf_mf_forward_f32_mma with a workspace)");

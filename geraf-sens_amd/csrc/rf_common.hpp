@@ -184,6 +184,21 @@ __device__ __forceinline__ void split8(const float2* y, h8& rh, h8& rl, h8& ih, 
   }
 }
 
+// hi/lo of four complex values as interleaved (Re, Im) pairs: element 2u is
+// Re y[u], element 2u + 1 is Im y[u] (same conversions as split8).
+__device__ __forceinline__ void split4c(const float2* y, h8& h, h8& l) {
+  float negone = -1.0f;
+  asm("" : "+v"(negone));
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const h2 c = __builtin_convertvector((f2v){y[u].x, y[u].y}, h2);
+    h[2 * u] = c[0];
+    h[2 * u + 1] = c[1];
+    l[2 * u] = (_Float16)fmaf((float)c[0], negone, y[u].x);
+    l[2 * u + 1] = (_Float16)fmaf((float)c[1], negone, y[u].y);
+  }
+}
+
 // c += A B for split operands (three instructions).
 __device__ __forceinline__ f32x4v mma3(const h8& ah, const h8& al, const h8& bh, const h8& bl,
                                        f32x4v c) {
@@ -207,9 +222,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// exp(-j 2 pi x) for an x already reduced to about [-0.5, 0.5] (e.g. by
+// frac_multiple): the hardware sine and cosine without a second reduction.
+__device__ __forceinline__ float2 cis_neg_reduced(float x) {
+  return make_float2(__builtin_amdgcn_cosf(x), -__builtin_amdgcn_sinf(x));
+}
+
 // Complex helpers on float2 = (re, im).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
+// a * b + c as two FMA chains (four instructions).
+__device__ __forceinline__ float2 cmul_add(float2 a, float2 b, float2 c) {
+  return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, c.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, c.y)));
 }
 
 // Compensated (Kahan) accumulation of a complex value.

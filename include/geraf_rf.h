@@ -69,11 +69,11 @@ int geraf_mf_forward_f32(const float* real, const float* imag, const float* tx_p
 /* Same operator (_C.MatchedFilter_forward, vision.cpp:81-84;
  * MatchedFilter_cuda_kernel.cu:65-115) on the matrix cores, for
  * 1 <= K <= 2048 (faster than geraf_mf_forward_f32 from K ~ 300 on).  With
- * k = 32*alpha + beta, each antenna's sum is a complex matrix product over
- * beta on v_mfma_f32_16x16x32_f16, every real operand split into FP16 hi + lo
- * and each real product taken as three (FP32-accurate products, FP32
- * accumulation); phases as in geraf_mf_forward_f32 (FP64 distances and
- * reduction).  workspace: device memory of workspace_bytes >=
+ * k = 16*alpha + beta, each antenna's sum is a complex matrix product over
+ * beta on v_mfma_f32_16x16x32_f16 (the complex product folded into the
+ * depth), every real operand split into FP16 hi + lo and each real product
+ * taken as three (FP32-accurate products, FP32 accumulation); phases as in
+ * geraf_mf_forward_f32 (FP64 distances and reduction).  workspace: device memory of workspace_bytes >=
  * geraf_mf_forward_mma_workspace_bytes(A, K), 256-byte aligned, owned by the
  * caller (the echo in operand order); nothing is allocated.  K > 2048 is an
  * argument error.  geraf_mf_forward_mma_workspace_bytes returns -1 for
@@ -86,14 +86,15 @@ int geraf_mf_forward_f32_mma(const float* real, const float* imag, const float* 
                              int64_t workspace_bytes, geraf_stream_t stream);
 
 /* Size of the device-resident scratch of geraf_mf_forward_f32_mma_noalloc
- * (32 MiB per GPU that loads the library, held until the process ends). */
-#define GERAF_MF_MMA_SCRATCH_BYTES (32u << 20)
+ * (48 MiB per GPU that loads the library, held until the process ends). */
+#define GERAF_MF_MMA_SCRATCH_BYTES (48u << 20)
 
 /* geraf_mf_forward_f32_mma without a workspace argument and without any
  * allocation: the echo's operand buffer is a static device array of
  * GERAF_MF_MMA_SCRATCH_BYTES, so geraf_mf_forward_mma_workspace_bytes(A, K)
- * must not exceed that (A * ceil(K / 512) <= 5,457, e.g. 4,096 antennas at
- * K <= 512); larger sizes return GERAF_RF_EINVAL with a message.  Same
+ * must not exceed that (4,096 A ceil(K / 256) + 4 A bytes, each term rounded
+ * up to 256: A <= 6,141 at K <= 512, A <= 2,457 at K = 1,025..1,280); larger
+ * sizes return GERAF_RF_EINVAL with a message.  Same
  * kernels, same results.  The scratch is shared by every call on one GPU:
  * calls that overlap on one GPU (different streams or host threads) race.
  * No counterpart in the reference: for callers that cannot allocate device

@@ -55,7 +55,9 @@ def test_version_and_error_reporting_without_gpu():
     # the matrix-core forward: its K limit, its workspace contract and empty work
     assert lib.geraf_mf_forward_mma_workspace_bytes(-1, 8) == -1
     need = lib.geraf_mf_forward_mma_workspace_bytes(3, 513)
-    assert need >= 3 * 2 * 4 * 64 * 16 + 3 * 4
+    # 3 antennas x 3 tiles of 256 frequencies x 4 operands x 64 lanes x 16 B,
+    # and the per-antenna scales
+    assert need >= 3 * 3 * 4 * 64 * 16 + 3 * 4
     mma = lib.geraf_mf_forward_f32_mma
     st = mma(*([None] * 7), 2, 2049, 5, ctypes.byref(prm), None, 0, None)
     assert st == 1 and b"2048" in lib.geraf_rf_last_error()

@@ -54,8 +54,9 @@ def _both(dev, echo, tx, rx, vox, prm):
 
 
 def test_noalloc_matches_workspace_form_twice(cuda):
-    # the largest case the scratch holds (4,096 antennas, K = 512: 24 MiB of
-    # operands), then a smaller one that reads only part of what the first left
+    # a large case (4,096 antennas, K = 512: 32 MiB of operands in the
+    # 48 MiB scratch), then a smaller one that reads only part of what the
+    # first left
     for A, K, N, seed in ((4096, 512, 3000, 1), (33, 300, 700, 2)):
         print(f"no-alloc MMA forward A={A} K={K} N={N}", flush=True)
         echo, tx, rx, vox, prm = _case(A, K, N, seed)
@@ -76,12 +77,13 @@ def test_noalloc_rejects_sizes_beyond_the_scratch():
     lib = _native.load()
     prm = _native.make_params(mf_scenes.MF_F0, 701500.0, 1.25e6)
     fn = lib.geraf_mf_forward_f32_mma_noalloc
-    # 5,458 antennas at K = 512 need more than 32 MiB: refused before any GPU use
-    st = fn(*([None] * 7), 5458, 512, 10, ctypes.byref(prm), None)
+    # 6,142 antennas at K = 512 need more than 48 MiB: refused before any GPU use
+    st = fn(*([None] * 7), 6142, 512, 10, ctypes.byref(prm), None)
     assert st == 1 and b"static scratch" in lib.geraf_rf_last_error()
-    st = fn(*([None] * 7), 2000, 1025, 10, ctypes.byref(prm), None)
+    st = fn(*([None] * 7), 2458, 1025, 10, ctypes.byref(prm), None)
     assert st == 1 and b"static scratch" in lib.geraf_rf_last_error()
     st = fn(*([None] * 7), 2, 2049, 10, ctypes.byref(prm), None)
     assert st == 1 and b"2048" in lib.geraf_rf_last_error()
-    # the limit itself is accepted (empty volume: nothing is launched)
-    assert fn(*([None] * 7), 5457, 512, 0, ctypes.byref(prm), None) == 0
+    # the limits themselves are accepted (empty volume: nothing is launched)
+    assert fn(*([None] * 7), 6141, 512, 0, ctypes.byref(prm), None) == 0
+    assert fn(*([None] * 7), 2457, 1025, 0, ctypes.byref(prm), None) == 0
