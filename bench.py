@@ -1011,8 +1011,8 @@ def main() -> int:
                    "A": A, "K": K, "N": N, "terms": terms},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / MFMA_FP16_PEAK_TFLOPS,
-                     "traffic": profiled_traffic("mf_forward_mma_kernel<1>"),
-                     "kernel": "mf_forward_mma_kernel<1> (with mf_mma_prep_kernel)",
+                     "traffic": profiled_traffic("mf_forward_mma_kernel<2>"),
+                     "kernel": "mf_forward_mma_kernel<2> (with mf_mma_prep_kernel)",
                      "mean_launch_ms": 1e3 * mean_launch_s,
                      "flops_per_launch": FLOPS_PER_TERM * terms,
                      "fp32_peak_frac": achieved / VALU_FP32_PEAK_TFLOPS,

@@ -414,10 +414,6 @@ constexpr int kAdj16Waves = 4;                     // waves per block (one anten
 constexpr int kAdj16Threads = kWave * kAdj16Waves;
 constexpr int kAdjRedPerWave = 8 * kWave;  // (Re, Im) x 4 rows per lane, one tile
 
-__device__ __forceinline__ float2 csq(float2 a) {
-  return make_float2(fmaf(a.x, a.x, -a.y * a.y), 2.0f * a.x * a.y);
-}
-
 template <int MT>
 __global__ __launch_bounds__(kAdj16Threads)
 __attribute__((amdgpu_waves_per_eu(MT >= 4 ? 2 : GERAF_ADJ16_WPE))) void mf_backward_mma16_kernel(
@@ -498,7 +494,9 @@ __attribute__((amdgpu_waves_per_eu(MT >= 4 ? 2 : GERAF_ADJ16_WPE))) void mf_back
     f32x4v prr[MT], pii[MT], pim[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) prr[m] = pii[m] = pim[m] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 1
+    // both chunks unrolled: the first products of each base then start from
+    // a zero accumulator operand instead of registers zeroed per base
+#pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int v0 = 32 * c + 8 * g;  // this lane's eight voxels (of the base)
       float2 x[8];

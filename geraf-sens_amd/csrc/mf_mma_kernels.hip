@@ -19,9 +19,9 @@
 // relative), so the products carry the ~22 bits of FP32 operands: 6
 // instructions per 16-row tile (256 frequencies) and 16 voxels.
 //
-// Per voxel there are 16 Y values (four per lane: one hardware phasor and
-// three complex products) and 16 NT rows of T; the MFMA rows of tile m are
-// assigned so that lane j + 16 g receives the consecutive rows
+// Per voxel there are 16 Y values (four per lane: the group's first phasor
+// and three complex products) and 16 NT rows of T; the MFMA rows of tile m
+// are assigned so that lane j + 16 g receives the consecutive rows
 // alpha = 4 NT g + 4 m + r (r < 4), contracted by a two-level Horner in the
 // 16- and 64-frequency steps against one X phasor per lane.
 //
@@ -30,9 +30,11 @@
 // nor loses the small entries) are laid out once in operand order by
 // mf_mma_prep_kernel.  The main kernel gives each wave 64 voxels (four groups
 // of 16).  Per antenna, each lane forms its own voxel's delay in FP64 (FP64
-// distances, as the other forms) and publishes the reduced phase terms to
-// the wave through LDS, with the voxel's step phasors
-// exp(-j 2 pi {1, 2, 16, 64} dfs tau).  Antennas accumulate per lane in FP64;
+// distances, as the other forms) and publishes to the wave through LDS the
+// voxel's step phasors exp(-j 2 pi {1, 2, 16, 64} dfs tau) and, for each
+// lane group g, its first Y phasor Y[4 g] and its X phasor X[4 NT g] (the
+// hardware sine and cosine for the powers 1, 4, 16, 64 and 64 NT and for the
+// carrier, the others by products).  Antennas accumulate per lane in FP64;
 // the four lanes of a voxel are summed at the end.
 //
 // Operand layout of the instruction (checked on the GPU with an in-bounds
@@ -108,15 +110,24 @@ __global__ __launch_bounds__(64) void mf_mma_prep_kernel(const float* __restrict
   out[3 * kWave + lane] = iml;
 }
 
+#ifndef GERAF_MF_MMA_GROUP_BARRIER
+#define GERAF_MF_MMA_GROUP_BARRIER 1
+#endif
+#ifndef GERAF_MF_MMA_WPE
+#define GERAF_MF_MMA_WPE 1
+#endif
+
 template <int NT>
-__global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
+__global__ __launch_bounds__(64 * kMmaWaves)
+__attribute__((amdgpu_waves_per_eu(GERAF_MF_MMA_WPE))) void mf_forward_mma_kernel(
     const h8* __restrict__ ops, const float* __restrict__ unscale,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
     float* __restrict__ out_real, float* __restrict__ out_imag, const float* __restrict__ voxel,
     int A, int N, PhaseConsts pc) {
-  // per voxel of the wave: step cycles (hi, lo) and carrier cycles; the step
-  // phasors w1, w2 = exp(-j 2 pi {1, 2} dfs tau), w16, w64
-  __shared__ float4 par[kMmaWaves][3][kWave];
+  // per voxel of the wave: the step phasors w1, w2 = exp(-j 2 pi {1, 2} dfs
+  // tau) and w16, w64; per lane group g of the voxel: its first Y phasor
+  // Y[4 g] and its X phasor X[4 NT g]
+  __shared__ float4 par[kMmaWaves][6][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int v0 = (blockIdx.x * kMmaWaves + wave) * kWave;
@@ -124,10 +135,6 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
   const int j = lane & 15, g = lane >> 4;
   const int vo = min(v0 + lane, N - 1);
   const double px = voxel[vo], py = voxel[(size_t)N + vo], pz = voxel[2 * (size_t)N + vo];
-  // this lane's first row alpha = 4 NT g: frequency 64 NT g; its Y values
-  // beta = 4 g .. 4 g + 3
-  const float x_mult = (float)(64 * NT * g);
-  const float y_mult = (float)(4 * g);
 
   // per-lane sums over the antennas in FP64 (one conversion and one FMA per
   // component and antenna; more accurate than compensated FP32 and cheaper)
@@ -156,13 +163,22 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
     const double tau = (d + pc.bias) * pc.inv_c;
     const SplitCycles c1 = split_cycles13(pc.dfs * tau);
     {
+      // lane group g takes beta = 4 g .. 4 g + 3 and the rows from
+      // alpha = 4 NT g (frequency 64 NT g) on
       const float2 w1 = cis_neg_reduced(frac_multiple(1.0f, c1));
       const float2 w16 = cis_neg_reduced(frac_multiple(16.0f, c1));
       const float2 w64 = cis_neg_reduced(frac_multiple(64.0f, c1));
-      const float2 w2 = cmul(w1, w1);
-      par[wave][0][lane] = make_float4(c1.hi, c1.lo, (float)centered_frac(pc.f0 * tau), 0.0f);
-      par[wave][1][lane] = make_float4(w1.x, w1.y, w2.x, w2.y);
-      par[wave][2][lane] = make_float4(w16.x, w16.y, w64.x, w64.y);
+      const float2 w4 = cis_neg_reduced(frac_multiple(4.0f, c1));
+      const float2 wx = cis_neg_reduced(frac_multiple((float)(64 * NT), c1));
+      const float2 x0 = cis_neg_reduced((float)centered_frac(pc.f0 * tau));
+      const float2 w2 = csq(w1), w8 = csq(w4), w12 = cmul(w8, w4);
+      const float2 x1 = cmul(x0, wx), x2 = cmul(x1, wx), x3 = cmul(x2, wx);
+      par[wave][0][lane] = make_float4(w1.x, w1.y, w2.x, w2.y);
+      par[wave][1][lane] = make_float4(w16.x, w16.y, w64.x, w64.y);
+      par[wave][2][lane] = make_float4(1.0f, 0.0f, x0.x, x0.y);
+      par[wave][3][lane] = make_float4(w4.x, w4.y, x1.x, x1.y);
+      par[wave][4][lane] = make_float4(w8.x, w8.y, x2.x, x2.y);
+      par[wave][5][lane] = make_float4(w12.x, w12.y, x3.x, x3.y);
     }
     wave_sync();
     // groups unrolled (the accumulators stay in registers), one at a time: the
@@ -170,15 +186,15 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
     // would cost registers (occupancy) and gain nothing
 #pragma unroll
     for (int G = 0; G < 4; ++G) {
-      const float4 p = par[wave][0][16 * G + j];
-      const float4 q1 = par[wave][1][16 * G + j];
-      const float4 q2 = par[wave][2][16 * G + j];
-      const SplitCycles c = {p.x, p.y};
+      const float4 q1 = par[wave][0][16 * G + j];
+      const float4 q2 = par[wave][1][16 * G + j];
+      const float4 p = par[wave][2 + g][16 * G + j];
       const float2 w1 = make_float2(q1.x, q1.y), w2 = make_float2(q1.z, q1.w);
       const float2 w16 = make_float2(q2.x, q2.y), w64 = make_float2(q2.z, q2.w);
-      // Y[4 g + u], u = 0..3: one phasor and three products, as (Re, Im) pairs
+      // Y[4 g + u], u = 0..3: the group's first phasor and three products, as
+      // (Re, Im) pairs
       float2 y[4];
-      y[0] = cis_neg_reduced(frac_multiple(y_mult, c));
+      y[0] = make_float2(p.x, p.y);
       y[1] = cmul(y[0], w1);
       y[2] = cmul(y[0], w2);
       y[3] = cmul(y[1], w2);
@@ -201,11 +217,10 @@ __global__ __launch_bounds__(64 * kMmaWaves) void mf_forward_mma_kernel(
         for (int r = 2; r >= 0; --r) t = cmul_add(t, w16, make_float2(tr[m][r], ti[m][r]));
         h = (m == NT - 1) ? t : cmul_add(h, w64, t);
       }
-      const float2 x = cis_neg(p.z + frac_multiple(x_mult, c));
-      const float2 part = cmul(x, h);
+      const float2 part = cmul(make_float2(p.z, p.w), h);
       acc_r[G] = fma((double)part.x, up, acc_r[G]);
       acc_i[G] = fma((double)part.y, up, acc_i[G]);
-      __builtin_amdgcn_sched_barrier(0);
+      if (GERAF_MF_MMA_GROUP_BARRIER) __builtin_amdgcn_sched_barrier(0);
     }
     wave_sync();
   }

@@ -233,6 +233,11 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 
+// a^2 (three instructions).
+__device__ __forceinline__ float2 csq(float2 a) {
+  return make_float2(fmaf(a.x, a.x, -a.y * a.y), 2.0f * a.x * a.y);
+}
+
 // a * b + c as two FMA chains (four instructions).
 __device__ __forceinline__ float2 cmul_add(float2 a, float2 b, float2 c) {
   return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, c.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, c.y)));
