@@ -141,8 +141,8 @@ __device__ __forceinline__ RawSample load_raw(const LenslessArgs& a, const Ray& 
   return w;
 }
 
-__device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ray& y, int s) {
-  const RawSample w = load_raw(a, y, s);
+__device__ __forceinline__ Sample composite(const LenslessArgs& a, const Ray& y,
+                                            const RawSample& w) {
   Sample x;
   x.sdf = w.sdf;
   x.dist = w.dist;
@@ -171,6 +171,10 @@ __device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ra
   x.alpha = fminf(fmaxf(q, 0.0f), 1.0f);
   x.t = 1.0f - x.alpha + 1.0e-7f;
   return x;
+}
+
+__device__ __forceinline__ Sample load_composite(const LenslessArgs& a, const Ray& y, int s) {
+  return composite(a, y, load_raw(a, y, s));
 }
 
 // Specular geometry of the monostatic sample (ComputeSpecularGeometry,
@@ -697,11 +701,13 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   for (int tile = ntiles - 1; tile >= 0; --tile) {
     const int s = tile * kTile + lane;
     const bool valid = s < y.nv;
+    // the tile's inputs and geometry, once: they serve both the h_s batch
+    // and the gradients
+    const RawSample w = load_raw(a, y, valid ? s : y.nv - 1);
+    const Geo e = geometry(y, w.zz, w.g, valid);
     if (one_pass) {
       // h_s of this tile's emitting samples (which are known from z and the
       // SDF gradient alone), before the compositing terms are formed
-      const unsigned u = (unsigned)(valid ? s : y.nv - 1);
-      const Geo e = geometry(y, ld_row(a.z + y.row, u), ld_row3(a.grad + 3 * y.row, u), valid);
       const unsigned long long mask = __ballot(e.live);
       if (e.live) {
         const int slot = lane_prefix(mask);
@@ -713,7 +719,7 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
       if (cnt > 0) horner_batch(cnt);
       wave_sync();
     }
-    const Sample x = load_composite(a, y, valid ? s : y.nv - 1);
+    const Sample x = composite(a, y, w);
     const float tt = valid ? x.t : 1.0f;
     const float incl = wave_incl_prod_dpp(tt);
     const float tile_T = one_pass ? carry_in[(size_t)r * ntmax + tile] : L.carry[tile];
@@ -724,7 +730,6 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
     const bool unclamped = m >= 1.0e-6f;
     const float aTc = x.alpha * Tc;
     const float sigma = fmaxf(m, 1.0e-6f) * aTc;
-    const Geo e = geometry(y, x.zz, x.g, valid);
 
     float h = 0.0f, att = 0.0f;
     if (e.live) {
