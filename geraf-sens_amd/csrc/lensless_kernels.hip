@@ -605,15 +605,45 @@ __attribute__((amdgpu_waves_per_eu(MT <= 2 ? 5 : 1))) void lensless_backward_ker
   if constexpr (kH16) {
     float2 gv[kT16][8];
     float mx = 0.0f;
+    // a lane's eight elements are contiguous: two 16-byte loads per row when
+    // the rows are 16-byte aligned and K is a multiple of 8 (then the eight
+    // are all in range or all out), else one element at a time
+    const bool vec = (a.K % 8 == 0) &&
+                     ((reinterpret_cast<uintptr_t>(gr_row) | reinterpret_cast<uintptr_t>(gi_row)) &
+                      15u) == 0;
 #pragma unroll
-    for (int m = 0; m < kT16; ++m)
+    for (int m = 0; m < kT16; ++m) {
+      const int k0 = 512 * m + 32 * (lane & 15) + 8 * (lane >> 4);
+      if (vec) {
+        float4 r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), r1 = r0, i0 = r0, i1 = r0;
+        if (k0 < a.K) {
+          const float4* pr = reinterpret_cast<const float4*>(gr_row + k0);
+          const float4* pi = reinterpret_cast<const float4*>(gi_row + k0);
+          r0 = pr[0];
+          r1 = pr[1];
+          i0 = pi[0];
+          i1 = pi[1];
+        }
+        gv[m][0] = make_float2(r0.x, i0.x);
+        gv[m][1] = make_float2(r0.y, i0.y);
+        gv[m][2] = make_float2(r0.z, i0.z);
+        gv[m][3] = make_float2(r0.w, i0.w);
+        gv[m][4] = make_float2(r1.x, i1.x);
+        gv[m][5] = make_float2(r1.y, i1.y);
+        gv[m][6] = make_float2(r1.z, i1.z);
+        gv[m][7] = make_float2(r1.w, i1.w);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = 512 * m + 32 * (lane & 15) + 8 * (lane >> 4) + e;
-        gv[m][e] = k < a.K ? make_float2(ld_row(gr_row, (unsigned)k), ld_row(gi_row, (unsigned)k))
-                           : make_float2(0.0f, 0.0f);
-        mx = fmaxf(mx, fmaxf(fabsf(gv[m][e].x), fabsf(gv[m][e].y)));
+        for (int e = 0; e < 8; ++e) {
+          const int k = k0 + e;
+          gv[m][e] = k < a.K ? make_float2(ld_row(gr_row, (unsigned)k),
+                                           ld_row(gi_row, (unsigned)k))
+                             : make_float2(0.0f, 0.0f);
+        }
       }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fmaxf(fabsf(gv[m][e].x), fabsf(gv[m][e].y)));
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
     int ex = 0;
