@@ -168,6 +168,105 @@ def mf_scenes_nearest(sc):
     return mf_scenes.nearest_voxels(sc, sc.scatterers)
 
 
+# FP64 operations per (voxel, antenna) pair of the range-bin forms that the
+# roofline counts: two distances (3 subtractions, 1 multiply, 2 FMAs (4
+# flops), 1 square root each = 9) and the carrier-phase reduction (2
+# multiplies, 1 rounding, 1 subtraction = 4).
+RANGE_BIN_FP64_FLOPS_PER_PAIR = 22
+VALU_FP64_PEAK_TFLOPS = 78.6
+
+
+def bench_range_bin(dev, sc, tx, rx, vox, prm, reps: int = 3):
+    """Range-bin (FFT) matched filter and its adjoint on the 64x64 MIMO
+    aperture and the 128^3 voxels, with range-compressed echoes of 512 bins
+    (spacing 2.5 mm: the round trips of 0.55-1.07 m fall in bins ~110-215).
+    FP32 forms timed per launch (median of `reps`); errors against the FP64
+    forms on a voxel subset (forward) and on 64 antennas (adjoint)."""
+    import numpy as np
+    import torch
+
+    from geraf.ops import _native
+
+    A, N = sc.A, sc.N
+    R, spacing = 512, float(np.float32(0.0025))
+    rng = np.random.default_rng(5)
+    s = rng.standard_normal((A, R)) + 1j * rng.standard_normal((A, R))
+    g = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    f32 = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float32, device=dev)  # noqa
+    f64 = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)  # noqa
+    sr, si, gr, gi = f32(s.real), f32(s.imag), f32(g.real), f32(g.imag)
+    o_r = torch.empty(N, device=dev)
+    o_i = torch.empty_like(o_r)
+    b_r = torch.empty((A, R), device=dev)
+    b_i = torch.empty_like(b_r)
+    P = _native.ptr
+    st = _native.stream_of(o_r)
+    fwd = lambda: _native.call("geraf_mf_fft_forward_f32", P(sr), P(si), P(tx), P(rx),  # noqa
+                               P(o_r), P(o_i), P(vox), A, R, N, spacing, prm, st)
+    bwd = lambda: _native.call("geraf_mf_fft_backward_f32", P(gr), P(gi), P(tx), P(rx),  # noqa
+                               P(b_r), P(b_i), P(vox), A, R, N, A, spacing, prm, st)
+
+    def per_launch(fn):
+        fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for e0, e1 in evs:
+            e0.record()
+            fn()
+            e1.record()
+        torch.cuda.synchronize()
+        return float(np.median([e0.elapsed_time(e1) for e0, e1 in evs]))
+
+    f_ms, b_ms = per_launch(fwd), per_launch(bwd)
+    first = (b_r.clone(), b_i.clone())
+    bwd()
+    torch.cuda.synchronize()
+    deterministic = bool(torch.equal(first[0], b_r) and torch.equal(first[1], b_i))
+    pairs = float(A) * N
+    res = {"workload": f"range-bin matched filter, 64x64 MIMO ({A} antennas), {N} voxels "
+                       f"(128^3), {R} range bins of {spacing} m",
+           "pairs": pairs, "range_bins": R,
+           "kernels": {"forward": "mf_fft_forward_kernel<float>",
+                       "backward": "mf_fft_backward_kernel<float, 4> (4 antennas per block at 512 bins)"},
+           "adjoint_bitwise_repeatable": deterministic}
+    for name, ms in (("forward", f_ms), ("backward", b_ms)):
+        tf = RANGE_BIN_FP64_FLOPS_PER_PAIR * pairs / (ms / 1e3) / 1e12
+        res[name] = {"ms": ms, "pairs_per_s": pairs / (ms / 1e3),
+                     "roofline": {"bound": "valu (fp64)", "achieved_tflops": tf,
+                                  "peak": VALU_FP64_PEAK_TFLOPS, "frac": tf / VALU_FP64_PEAK_TFLOPS,
+                                  "flops_per_pair": RANGE_BIN_FP64_FLOPS_PER_PAIR}}
+    # FP64 forms: the forward on 4,096 voxels, the adjoint on the first 64 antennas
+    sel = np.sort(rng.choice(N, 4096, replace=False))
+    vsel = vox[:, torch.tensor(sel, device=dev)].double().contiguous()
+    r64 = torch.empty(len(sel), device=dev, dtype=torch.float64)
+    i64 = torch.empty_like(r64)
+    tx64, rx64 = tx.double().contiguous(), rx.double().contiguous()
+    s64r, s64i = f64(s.real), f64(s.imag)
+    _native.call("geraf_mf_fft_forward_f64", P(s64r), P(s64i), P(tx64), P(rx64), P(r64), P(i64),
+                 P(vsel), A, R, len(sel), spacing, prm, _native.stream_of(r64))
+    torch.cuda.synchronize()
+    out = torch.complex(o_r.double(), o_i.double())[torch.tensor(sel, device=dev)]
+    ref = torch.complex(r64, i64)
+    res["forward"]["rel_l2_vs_fp64"] = float((out - ref).norm() / ref.norm())
+    A2 = 64
+    gb_r = torch.empty((A2, R), device=dev, dtype=torch.float64)
+    gb_i = torch.empty_like(gb_r)
+    vox64 = vox.double().contiguous()
+    g64r, g64i = f64(g.real), f64(g.imag)  # kept alive across the call
+    _native.call("geraf_mf_fft_backward_f64", P(g64r), P(g64i), P(tx64), P(rx64),
+                 P(gb_r), P(gb_i), P(vox64), A2, R, N, A, spacing, prm, _native.stream_of(gb_r))
+    torch.cuda.synchronize()
+    outb = torch.complex(b_r[:A2].double(), b_i[:A2].double())
+    refb = torch.complex(gb_r, gb_i)
+    res["backward"]["rel_l2_vs_fp64"] = float((outb - refb).norm() / refb.norm())
+    log(f"range-bin MF ({R} bins): forward {f_ms:.2f} ms, adjoint {b_ms:.2f} ms "
+        f"({pairs / f_ms * 1e3:.3e} / {pairs / b_ms * 1e3:.3e} pairs/s); rel-L2 vs FP64 "
+        f"{res['forward']['rel_l2_vs_fp64']:.2e} / {res['backward']['rel_l2_vs_fp64']:.2e}; "
+        f"adjoint repeatable: {deterministic}")
+    return res
+
+
 def bench_adjoint(dev, sc, re, im, tx, rx, vox, g_r, g_i, prm, sel):
     """Matched-filter adjoint (grad w.r.t. the echo) on the full 64x64 MIMO,
     K=512, 128^3 workload, with the forward output as the incoming gradient;
@@ -961,6 +1060,7 @@ def main() -> int:
         log(f"oracle pin skipped: {e}")
 
     adjoint = bench_adjoint(dev, sc, re, im, tx, rx, vox, o_r, o_i, prm, sel)
+    range_bin = bench_range_bin(dev, sc, tx, rx, vox, prm)
     tiled = bench_tiled(dev, sc, re, im, tx, rx, sel, ref, 1e3 * mean_launch_s)
 
     cpu = None
@@ -1036,6 +1136,7 @@ def main() -> int:
         "plain_forms_terms_per_s": dict(plain, tuned_speedup_vs_fp64=value / f64_rate,
                                         tuned_speedup_vs_fp32=value / p32_rate),
         "adjoint": adjoint,
+        "range_bin": range_bin,
         "tiled_expansion": tiled,
         "tracer": tracer,
         "stage1_step": stage,

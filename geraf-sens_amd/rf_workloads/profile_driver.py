@@ -18,7 +18,9 @@ not the values.  --stage2-step N does the same for the stage-2-style step
 (32,768 rays x 256 samples, K=512, consistency term; one warm-up step).
 --tiled runs the per-tile expansion matched filter and its
 adjoint on the 64x64 MIMO, K=512, 128^3 workload (twice: the second call is
-the one to read).  --ap-ops runs the [A,P] signal tracer forward and backward
+the one to read).  --range-bin runs the range-bin matched filter and its adjoint (FP32 forms) on
+the 64x64 MIMO aperture and the 128^3 voxels with 512 bins, three times.
+--ap-ops runs the [A,P] signal tracer forward and backward
 at its training shape (2,048 antenna pairs, 51,456 points, K=64) and the
 depth resampler (804 rays x 32 -> 64 and 32,768 x 256 -> 512 samples), each
 three times.  --sdf-network runs the hand-written SDF network (the RF
@@ -75,6 +77,7 @@ def main() -> None:
     ap.add_argument("--tiled", action="store_true")
     ap.add_argument("--ap-ops", action="store_true")
     ap.add_argument("--sdf-network", action="store_true")
+    ap.add_argument("--range-bin", action="store_true")
     args = ap.parse_args()
     dev = torch.device("cuda:0")
     if args.sdf_network:
@@ -85,6 +88,9 @@ def main() -> None:
         return
     if args.tiled:
         tiled(dev)
+        return
+    if args.range_bin:
+        range_bin(dev)
         return
     if args.stage1_step:
         stage_steps(dev, args.stage1_step, args.form)
@@ -169,6 +175,32 @@ def tiled(dev) -> None:
         print(f"profile_driver: tiled matched filter, call {rep}", flush=True)
         o_r, o_i, info = tiled_forward(re, im, tx, rx, vox, sc.K, sc.fs, sc.As_sci, 1e-6)
         tiled_adjoint(o_r, o_i, tx, rx, info["plan"], sc.A, sc.K, sc.A, sc.fs, sc.As_sci, vox)
+        torch.cuda.synchronize()
+    print("profile_driver: done", flush=True)
+
+
+def range_bin(dev) -> None:
+    """The range-bin matched filter (FP32 forms) and its adjoint on the 64x64
+    MIMO aperture and 128^3 voxels, 512 bins of 2.5 mm (bench.py's range_bin
+    leg): one warm-up call each, then two."""
+    sc = mf_scenes.config2(0, device=str(dev))
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float32, device=dev)  # noqa
+    A, N, R, spacing = sc.A, sc.N, 512, float(np.float32(0.0025))
+    rng = np.random.default_rng(5)
+    sr, si = t(rng.standard_normal((A, R))), t(rng.standard_normal((A, R)))
+    gr, gi = t(rng.standard_normal(N)), t(rng.standard_normal(N))
+    tx, rx, vox = t(sc.tx), t(sc.rx), t(sc.voxels.T)
+    o_r, o_i = torch.empty(N, device=dev), torch.empty(N, device=dev)
+    b_r, b_i = torch.empty((A, R), device=dev), torch.empty((A, R), device=dev)
+    prm = _native.make_params(sc.f0, sc.As_sci, sc.fs, sc.c, sc.bias)
+    P = _native.ptr
+    st = _native.stream_of(o_r)
+    for rep in range(3):
+        print(f"profile_driver: range-bin matched filter, call {rep}", flush=True)
+        _native.call("geraf_mf_fft_forward_f32", P(sr), P(si), P(tx), P(rx), P(o_r), P(o_i),
+                     P(vox), A, R, N, spacing, prm, st)
+        _native.call("geraf_mf_fft_backward_f32", P(gr), P(gi), P(tx), P(rx), P(b_r), P(b_i),
+                     P(vox), A, R, N, A, spacing, prm, st)
         torch.cuda.synchronize()
     print("profile_driver: done", flush=True)
 

@@ -145,7 +145,9 @@ int geraf_mf_backward_f64(const double* grad_out_real, const double* grad_out_im
  * or d > fft_spacing*num_range_bins are skipped; bin = floor(d/fft_spacing/2)
  * (skipped if >= num_range_bins);
  *   out[v] = (1/A) sum_a s[a,bin] * exp(-j*2*pi*f0*d/c)
- * (params->slope and params->fs are unused). Distance, bin and phase in FP64.
+ * (params->slope and params->fs are unused). Distance, bin and phase
+ * reduction in FP64; the f32 form takes the phasor of the reduced phase from
+ * the hardware sine and cosine (~1.3e-7), the f64 form from FP64 sincospi.
  * Replaces _C.MatchedFilter_fft_forward (vision.cpp:85-88;
  * MatchedFilter_cuda_kernel.cu:117-169). */
 int geraf_mf_fft_forward_f32(const float* real, const float* imag, const float* tx_pos,
@@ -160,7 +162,10 @@ int geraf_mf_fft_forward_f64(const double* real, const double* imag, const doubl
                              geraf_stream_t stream);
 
 /* Adjoint of the range-bin matched filter into grad_real/grad_imag
- * [A, num_range_bins], divided by norm_count; num_range_bins <= 8192.
+ * [A, num_range_bins], divided by norm_count.  The f32 form sums each bin
+ * exactly in 64-bit fixed point (deterministic: the result does not depend
+ * on the order of the additions); the f64 form sums in FP64 in whatever order
+ * the additions land (1e-16-level differences between runs).
  * Replaces _C.MatchedFilter_fft_backward_realimag (vision.cpp:93-96;
  * MatchedFilter_cuda_kernel.cu:222-279). */
 int geraf_mf_fft_backward_f32(const float* grad_out_real, const float* grad_out_imag,

@@ -378,19 +378,27 @@ def test_cfg2_subvolume_vs_fp64(cuda):
     assert rel_l2(out64[:8], ref) <= 1e-9
 
 
+# (A, R, N, spacing): 128 bins in one adjoint chunk; 5,200 bins in three
+# chunks of 2,048 with the paths (~0.72-0.78 m) in bins ~2,400-2,600, i.e. in
+# the second chunk (the reference skips d > spacing * R, so bins above R/2
+# are never reached)
+RANGE_BIN_CASES = [(12, 128, 3000, 0.008), (6, 5200, 2000, 1.5e-4)]
+
+
+@pytest.mark.parametrize("case", RANGE_BIN_CASES)
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_range_bin_mf_vs_oracle(cuda, dtype):
+def test_range_bin_mf_vs_oracle(cuda, dtype, case):
     from geraf.ops.RFMatchedFilter import _C
 
+    A, R, N, spacing = case
     rng = np.random.default_rng(21)
-    A, R, N = 12, 128, 3000
     tx = rng.uniform(-0.03, 0.03, (A, 3)).astype(np.float32)
     tx[:, 2] = 0
     rx = (tx + np.float32([0.001, 0, 0])).astype(np.float32)
     vox = (rng.uniform(-0.08, 0.08, (N, 3)) + [0, 0, 0.3]).astype(np.float32)
     s = (rng.standard_normal((A, R)) + 1j * rng.standard_normal((A, R))).astype(np.complex64)
     g = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
-    spacing = float(np.float32(0.008))  # cutoff 1.02 m > the ~0.75 m paths
+    spacing = float(np.float32(spacing))
     T = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda, dtype=dtype)  # noqa: E731
     o_r = torch.empty(N, device=cuda, dtype=dtype)
     o_i = torch.empty_like(o_r)
@@ -406,7 +414,44 @@ def test_range_bin_mf_vs_oracle(cuda, dtype):
                                            T(rx), g_r, g_i, T(vox.T), A, R, spacing, 1.25e6,
                                            701500.0, N)
     refg = orc.mf_fft_backward(g, tx, rx, vox, R, spacing, A, **prm)
+    assert np.count_nonzero(refg) > 0
     assert rel_l2(g_r.double().cpu().numpy() + 1j * g_i.double().cpu().numpy(), refg) <= tol
+    if dtype == torch.float32:
+        # fixed-point bin sums: bit for bit the same on a second run
+        g_r2, g_i2 = torch.empty_like(g_r), torch.empty_like(g_i)
+        _C.MatchedFilter_fft_backward_realimag(T(g.real), T(g.imag), T(s.real), T(s.imag),
+                                               T(tx), T(rx), g_r2, g_i2, T(vox.T), A, R,
+                                               spacing, 1.25e6, 701500.0, N)
+        assert torch.equal(g_r, g_r2) and torch.equal(g_i, g_i2)
+
+
+def test_range_bin_adjoint_non_finite_gradient(cuda):
+    """A NaN in the output gradient reaches the bins its voxel falls in (the
+    FP32 form then sums in FP64, as the FP64 form does); the other bins keep
+    their values."""
+    from geraf.ops.RFMatchedFilter import _C
+
+    rng = np.random.default_rng(22)
+    A, R, N = 4, 128, 500
+    tx = rng.uniform(-0.03, 0.03, (A, 3)).astype(np.float32)
+    tx[:, 2] = 0
+    rx = (tx + np.float32([0.001, 0, 0])).astype(np.float32)
+    vox = (rng.uniform(-0.08, 0.08, (N, 3)) + [0, 0, 0.3]).astype(np.float32)
+    g = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    g[7] = np.nan
+    spacing = float(np.float32(0.008))
+    T = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda)  # noqa: E731
+    g_r = torch.empty((A, R), device=cuda)
+    g_i = torch.empty_like(g_r)
+    z = torch.zeros((A, R), device=cuda)
+    _C.MatchedFilter_fft_backward_realimag(T(g.real), T(g.imag), z, z, T(tx), T(rx), g_r, g_i,
+                                           T(vox.T), A, R, spacing, 1.25e6, 701500.0, N)
+    prm = dict(f0=mf_scenes.MF_F0, c=mf_scenes.SPEED_OF_LIGHT, bias=mf_scenes.PATH_BIAS)
+    refg = orc.mf_fft_backward(g, tx, rx, vox, R, spacing, A, **prm)
+    out = g_r.double().cpu().numpy() + 1j * g_i.double().cpu().numpy()
+    bad = np.isnan(refg)
+    assert bad.any() and np.array_equal(np.isnan(out), bad)
+    assert rel_l2(out[~bad], refg[~bad]) <= 1e-6
 
 
 def test_adc_bank_selected_rows_matched_filter(cuda):
