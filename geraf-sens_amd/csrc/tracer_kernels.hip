@@ -14,7 +14,7 @@
 // chunk); lanes stride over the points, each seeds its phasor from the FP64
 // reduced phase and rotates it through the chunk into 32 private complex
 // accumulators, which the block reduces in a fixed order.  Backward: a lane
-// owns a point, the wave walks the antennas in lock step, and h_ap is a
+// owns a point, the block's waves split the antennas, and h_ap is a
 // two-level Horner sum over k with the gradient row g[a,:] read through the
 // scalar cache (the same evaluation as the matched-filter forward).
 #include <hip/hip_runtime.h>
@@ -28,7 +28,8 @@ namespace geraf {
 namespace {
 
 constexpr int kChunk = 32;
-constexpr int kWaves = 4;
+constexpr int kWaves = 4;     // forward: waves per (antenna, chunk) block
+constexpr int kBwdWaves = 8;  // backward: waves per 64 points, splitting the antennas
 
 __global__ __launch_bounds__(64 * kWaves) void st_forward_kernel(
     const float* __restrict__ normals, const float* __restrict__ sigmas,
@@ -54,7 +55,10 @@ __global__ __launch_bounds__(64 * kWaves) void st_forward_kernel(
     const V3 pt = load3(points + 3 * (size_t)p);
     const SpecGeom g = spec_geom(txa, rxa, n, pt);
     if (g.culled()) continue;
-    const double d = dist3_f64(txa, pt) + dist3_f64(rxa, pt);
+    const double d = dist3_f64_fast((double)txa.x, (double)txa.y, (double)txa.z, (double)pt.x,
+                                    (double)pt.y, (double)pt.z) +
+                     dist3_f64_fast((double)rxa.x, (double)rxa.y, (double)rxa.z, (double)pt.x,
+                                    (double)pt.y, (double)pt.z);
     const float amp = sigmas[(size_t)a * P + p] * g.spec / (float)(d * d);
     const double tau = (d + pc.bias) * pc.inv_c;
     const double cyc1 = pc.dfs * tau;
@@ -91,28 +95,39 @@ __global__ __launch_bounds__(64 * kWaves) void st_forward_kernel(
   }
 }
 
-__global__ __launch_bounds__(256) void st_backward_kernel(
+// Backward: a block owns 64 points (one per lane) and its kBwdWaves waves
+// split the antennas (wave w takes a = w, w + kBwdWaves, ...), so that the
+// launch has kBwdWaves waves per 64 points instead of one; each wave writes
+// its antennas' sigma gradients and keeps a partial normal gradient per
+// point, summed over the waves in LDS in a fixed order (deterministic).
+__global__ __launch_bounds__(64 * kBwdWaves) void st_backward_kernel(
     const float* __restrict__ g_real, const float* __restrict__ g_imag,
     const float* __restrict__ normals, const float* __restrict__ sigmas,
     const float* __restrict__ points, const float* __restrict__ tx_pos,
     const float* __restrict__ rx_pos, const uint8_t* __restrict__ inbounds,
     float* __restrict__ grad_normals, float* __restrict__ grad_sigmas, int A, int P, int K,
     float inv_p, PhaseConsts pc) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float red[kBwdWaves][3][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p = blockIdx.x * kWave + lane;
   const bool valid = p < P;
   const int pcl = valid ? p : P - 1;
   const bool inb = valid && inbounds[pcl];
   const V3 n = load3(normals + 3 * (size_t)pcl);
   const V3 pt = load3(points + 3 * (size_t)pcl);
   V3 gn = v3(0.0f, 0.0f, 0.0f);
-  for (int a = 0; a < A; ++a) {
+  for (int a = wave; a < A; a += kBwdWaves) {
     const V3 txa = load3(tx_pos + 3 * a);
     const V3 rxa = load3(rx_pos + 3 * a);
     const SpecGeom g = spec_geom(txa, rxa, n, pt);
     float gs = 0.0f;
     if (inb && !g.culled()) {
-      const double d = dist3_f64(txa, pt) + dist3_f64(rxa, pt);
-      const float att = (float)(1.0 / (d * d));
+      const double d = dist3_f64_fast((double)txa.x, (double)txa.y, (double)txa.z, (double)pt.x,
+                                      (double)pt.y, (double)pt.z) +
+                       dist3_f64_fast((double)rxa.x, (double)rxa.y, (double)rxa.z, (double)pt.x,
+                                      (double)pt.y, (double)pt.z);
+      const float att = rcp_fast((float)(d * d));
       const float sv = sigmas[(size_t)a * P + p];
       const PathPhasors ph = path_phasors((d + pc.bias) * pc.inv_c, pc, -1.0f);
       const float2 S = horner2(g_real + (size_t)a * K, g_imag + (size_t)a * K, K, ph.w, ph.W);
@@ -122,10 +137,18 @@ __global__ __launch_bounds__(256) void st_backward_kernel(
     }
     if (valid) grad_sigmas[(size_t)a * P + p] = gs;
   }
-  if (valid) {
-    grad_normals[3 * (size_t)p + 0] = gn.x * inv_p;
-    grad_normals[3 * (size_t)p + 1] = gn.y * inv_p;
-    grad_normals[3 * (size_t)p + 2] = gn.z * inv_p;
+  red[wave][0][lane] = gn.x;
+  red[wave][1][lane] = gn.y;
+  red[wave][2][lane] = gn.z;
+  __syncthreads();
+  if (wave == 0 && valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float t = red[0][c][lane];
+#pragma unroll
+      for (int w = 1; w < kBwdWaves; ++w) t += red[w][c][lane];
+      grad_normals[3 * (size_t)p + c] = t * inv_p;
+    }
   }
 }
 
@@ -421,7 +444,8 @@ extern "C" int geraf_st_backward_f32(const float* grad_out_real, const float* gr
   require_ptrs({normals, sigmas, points, inbounds, grad_normals}, true);
   require_ptrs({tx_pos, rx_pos, grad_sigmas}, A > 0);
   require_ptrs({grad_out_real, grad_out_imag}, A > 0 && K > 0);
-  hipLaunchKernelGGL(st_backward_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(st_backward_kernel, dim3((unsigned)((P + kWave - 1) / kWave)),
+                     dim3(64 * kBwdWaves), 0,
                      (hipStream_t)stream, grad_out_real, grad_out_imag, normals, sigmas, points,
                      tx_pos, rx_pos, inbounds, grad_normals, grad_sigmas, (int)A, (int)P, (int)K,
                      (float)(1.0 / (double)P), make_phase_consts(*params));
