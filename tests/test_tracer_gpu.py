@@ -376,15 +376,18 @@ def test_lensless_backward_refuses_inputs_modified_in_place(cuda):
 
 
 @pytest.mark.parametrize("order", ["sorted", "shuffled"])
-def test_bilinear_backward_target_orders(cuda, order):
+@pytest.mark.parametrize("S,Tn", [(256, 512), (1536, 700), (5000, 300)])
+def test_bilinear_backward_target_orders(cuda, order, S, Tn):
     """The resampler's transpose with sorted target depths (contiguous
     bracket ranges found by binary search; targets below and above the source
     depths included) and with shuffled ones (the scan over all targets),
-    against the oracle; and each path is bit-reproducible."""
+    against the oracle; and each path is bit-reproducible.  Rays of >= 1,024
+    source samples share one bracket search per boundary through LDS; 5,000
+    samples exceed the depth staging in LDS (global-memory searches)."""
     from geraf.ops import _native
 
     rng = np.random.default_rng(31)
-    A, R, S, Tn = 2, 48, 256, 512
+    A, R = 2, 48
     z = np.sort(rng.uniform(0.2, 0.5, (R, S)), axis=1).astype(np.float32)
     tz = np.sort(rng.uniform(0.18, 0.52, (R, Tn)), axis=1).astype(np.float32)
     tz[:, 100:104] = z[:, 50:51]  # exact hits on a source depth
@@ -406,3 +409,18 @@ def test_bilinear_backward_target_orders(cuda, order):
     assert np.max(np.abs(outs[0][0] - rgn)) <= 2e-5
     assert np.max(np.abs(outs[0][1] - rgs)) <= 2e-5
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    # the forward at the same ray length (the ray's depths staged in LDS up to
+    # 4,096 samples, searched in global memory beyond)
+    nrm = rng.standard_normal((R, S, 3)).astype(np.float32)
+    sig = rng.uniform(0, 1, (A, R, S)).astype(np.float32)
+    f_n = torch.empty((R, Tn, 3), device=cuda)
+    f_s = torch.empty((A, R, Tn), device=cuda)
+    f_i = torch.empty((R, Tn), dtype=torch.bool, device=cuda)
+    fins = [T(nrm), T(sig), T(z), T(tz), f_n, f_s, f_i]
+    _native.call("geraf_bilinear_forward_f32", *map(_native.ptr, fins), A, R, S, Tn,
+                 _native.stream_of(f_n))
+    torch.cuda.synchronize()
+    rn, rs, rinb = orc.bilinear_forward(nrm, sig, z, tz)
+    assert np.array_equal(f_i.cpu().numpy(), rinb)
+    assert np.max(np.abs(f_n.cpu().numpy() - rn)) <= 1e-6
+    assert np.max(np.abs(f_s.cpu().numpy() - rs)) <= 1e-6
