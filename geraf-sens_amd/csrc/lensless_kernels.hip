@@ -35,7 +35,8 @@
 // matrix cores (k = 16 a + b, see synth_mfma), for every K: at K = 32, 64 and
 // 128 it measured 22, 18 and 25 % faster than a rotation synthesis on the
 // vector ALUs (4,096 rays x 128 samples), although most of its 256-frequency
-// tile is idle there.  (A form on FP16 hi/lo operands, 32 samples per
+// tile is idle there; for K <= 64 the 16-block 4 x 4 form (synth_mfma4) leaves
+// nothing idle and took the 32,768 x 256, K = 64 forward from 86.6 to 75.4 us.  (A form on FP16 hi/lo operands, 32 samples per
 // v_mfma_f32_16x16x32_f16 product, measured 4 % slower at 32,768 x 256,
 // K = 512: its splits cost more vector instructions than the matrix time it
 // saves, and the kernel is bound by vector issue.)
@@ -289,6 +290,38 @@ __device__ __forceinline__ void synth_mfma(const FwdQueue& q, int wave, int lane
   }
 }
 
+// The synthesis for K <= 64 on v_mfma_f32_4x4x1f32 (16 blocks of 4 x 4 outer
+// products; each lane 4 b + r gives row r of block b's A and column r of its
+// B, and element i of its result is D[b][i][r]).  With k = 16 a + b, a < 4:
+// block b = 4 q + c takes queued sample q of the 4-sample step and the
+// columns 4 c .. 4 c + 3, so lane 16 q + 4 c + r forms X[a = r, s_q] and
+// Y[s_q, b = 4 c + r] and accumulates partial sums of out[16 i + 4 c + r]
+// (i < 4) over the samples of slot q; the four slots are summed once per ray.
+// None of the output is idle (the 16 x 16 form uses 4 of its 16 rows here),
+// and each real product takes a quarter of the 16x16x4 instruction's cycles.
+// Three real products per complex one, as synth_mfma.
+__device__ __forceinline__ void synth_mfma4(const FwdQueue& q, int wave, int lane, int n,
+                                            f32x4& dr, f32x4& di, f32x4& ds) {
+  const int r = lane & 3, slot = lane >> 4, col = lane & 15;  // col = 4 c + r
+  for (int s0 = 0; s0 < n; s0 += 4) {
+    const int i = s0 + slot;
+    float am = 0.0f, f0 = 0.0f;
+    double c1 = 0.0;
+    if (i < n) {
+      am = q.amp[wave][i];
+      f0 = q.fr0[wave][i];
+      c1 = q.cyc1[wave][i];
+    }
+    const float2 yv = phasor_cycles_hw((float)centered_frac((double)col * c1));
+    const float2 xv =
+        phasor_cycles_hw((float)centered_frac(fma((double)(16 * r), c1, (double)f0)));
+    const float2 x = make_float2(am * xv.x, am * xv.y);
+    dr = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x, yv.x, dr, 0, 0, 0);
+    di = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, yv.y, di, 0, 0, 0);
+    ds = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x + x.y, yv.x + yv.y, ds, 0, 0, 0);
+  }
+}
+
 // Matrix-core synthesis with MT tiles of 256 frequencies (K <= 256 MT).
 template <int MT>
 __global__ __launch_bounds__(64 * kRaysPerBlock)
@@ -302,10 +335,17 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
   if (r >= a.R) return;  // only wave-level synchronisation below
   const Ray y = load_ray(a, r);
 
-  f32x4 dr[MT], di[MT], ds[MT];
+  // MT = 0: the K <= 64 form (synth_mfma4), one set of accumulators
+  constexpr int NA = MT > 0 ? MT : 1;
+  f32x4 dr[NA], di[NA], ds[NA];
 #pragma unroll
-  for (int m = 0; m < MT; ++m) dr[m] = di[m] = ds[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  auto drain = [&](int n) { synth_mfma<MT>(q, wave, lane, n, dr, di, ds); };
+  for (int m = 0; m < NA; ++m) dr[m] = di[m] = ds[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  auto drain = [&](int n) {
+    if constexpr (MT == 0)
+      synth_mfma4(q, wave, lane, n, dr[0], di[0], ds[0]);
+    else
+      synth_mfma<MT>(q, wave, lane, n, dr, di, ds);
+  };
 
   float carry = 1.0f;
   int queued = 0;
@@ -370,10 +410,34 @@ __attribute__((amdgpu_waves_per_eu(MT == 4 ? 6 : 1))) void lensless_forward_kern
   float* irow = out_imag + (size_t)r * a.K;
   // (Xr Yr - Xi Yi, (Xr + Xi)(Yr + Yi) - Xr Yr - Xi Yi)
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
+  for (int m = 0; m < NA; ++m) {
     const f32x4 p1 = dr[m], p2 = di[m];
     dr[m] = p1 - p2;
     di[m] = ds[m] - p1 - p2;
+  }
+  if constexpr (MT == 0) {
+    // the four sample slots (lanes l, l + 16, l + 32, l + 48) summed; lane
+    // l < 16 holds out[16 i + l] in element i
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      dr[0][v] += __shfl_xor(dr[0][v], 16, kWave);
+      di[0][v] += __shfl_xor(di[0][v], 16, kWave);
+      dr[0][v] += __shfl_xor(dr[0][v], 32, kWave);
+      di[0][v] += __shfl_xor(di[0][v], 32, kWave);
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = 16 * v + lane;
+        if (k < a.K) {
+          st_row(orow, (unsigned)k, dr[0][v] * inv_n);
+          st_row(irow, (unsigned)k, di[0][v] * inv_n);
+        }
+      }
+    }
+    if (sigma_out)
+      for (int s = y.nv + lane; s < a.S; s += kWave) st_row(sigma_out + y.row, (unsigned)s, 0.0f);
+    return;
   }
   const int kl = 64 * (lane >> 4) + (lane & 15);
 #pragma unroll
@@ -883,7 +947,10 @@ extern "C" int geraf_lensless_forward_f32(
   const PhaseConsts pc = make_phase_consts(*params);
   const dim3 grid(blocks), block(64 * kRaysPerBlock);
   hipStream_t st = (hipStream_t)stream;
-  if (K <= 256)
+  if (K <= 64)
+    hipLaunchKernelGGL(lensless_forward_kernel<0>, grid, block, 0, st, a, out_real, out_imag,
+                       sigma_out, tile_carry, pc);
+  else if (K <= 256)
     hipLaunchKernelGGL(lensless_forward_kernel<1>, grid, block, 0, st, a, out_real, out_imag,
                        sigma_out, tile_carry, pc);
   else if (K <= 512)
