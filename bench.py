@@ -1089,6 +1089,10 @@ def main() -> int:
                               args.no_cpu)
         tracer["stage2_size"] = bench_tracer(dev, max(args.steps, 3), 1, 0.0, True, rays=32768,
                                              samples=256, K=512)
+        # the same rays at the reference's training K (its stage configs' 64
+        # samples per chirp), where the pair's bytes outweigh its flops
+        tracer["training_k"] = bench_tracer(dev, max(args.steps, 3), 1, 0.0, True, rays=32768,
+                                            samples=256, K=64)
         tracer["ap_op_training_shape"] = bench_ap_tracer(dev)
         tracer["bilinear_resampler"] = bench_bilinear(dev)
 
