@@ -38,7 +38,10 @@
 // Epilogues (on z = acc + bias): z; softplus(z); softplus(z) with the code
 // of z (training); relu(z); z and z * e1 * beta * s(1-s) (the adjoint of
 // the gradient pass, its second-order term); z * s(e0) + e1 (the adjoint of
-// a hidden layer).  Softplus follows torch.nn.Softplus(beta, threshold): y
+// a hidden layer); z * s(e0) (the gradient pass, storing u s); z * s(e0)
+// and z * e1 * beta * (1 - s(e0)) with e1 = u s, or z * e1 * beta *
+// s(1 - s) with e1 = u (its adjoint, storing vbar s, so that the weight
+// gradients and the next product need no slope prologue).  Softplus follows torch.nn.Softplus(beta, threshold): y
 // = z if beta z > threshold else log1p(exp(beta z)) / beta, evaluated as
 // (max(t, 0) + log1p(exp(-|t|))) / beta with the hardware exp2 / log2 /
 // reciprocal (log1p by its series below 2^-6).
@@ -81,7 +84,10 @@ enum Epi {
   EPI_SOFTPLUS_TRAIN = 2,
   EPI_RELU = 3,
   EPI_GRAD_ADJ = 4,
-  EPI_LAYER_ADJ = 5
+  EPI_LAYER_ADJ = 5,
+  EPI_MUL_SLOPE = 6,
+  EPI_GRAD_ADJ_S = 7,
+  EPI_GRAD_ADJ_SU = 8
 };
 
 // ------------------------------------------------------------------ element-wise math
@@ -108,6 +114,12 @@ constexpr float kLn2 = 0.693147180559945309f;
 __device__ __forceinline__ float code_slope(uint16_t q) {
   const float f = hf(q);
   return 0.5f + copysignf(0.5f - fabsf(f), f);
+}
+
+// 1 - s from the code q: m for q >= +0, 1 - m for q <= -0
+__device__ __forceinline__ float code_one_minus_slope(uint16_t q) {
+  const float f = hf(q);
+  return 0.5f - copysignf(0.5f - fabsf(f), f);
 }
 
 // s (1 - s) = m (1 - m) from the code q
@@ -192,7 +204,15 @@ __device__ __forceinline__ bf8 make_frag(const GemmArgs& a, int ks, int g, u16x8
   return __builtin_bit_cast(bf8, v);
 }
 
-constexpr bool adj_epi(int epi) { return epi == EPI_GRAD_ADJ || epi == EPI_LAYER_ADJ; }
+// epilogues that read e0 (and e1) per output element
+constexpr bool adj_epi(int epi) { return epi >= EPI_GRAD_ADJ; }
+// epilogues that write y2
+constexpr bool two_out_epi(int epi) {
+  return epi == EPI_SOFTPLUS_TRAIN || epi == EPI_GRAD_ADJ || epi == EPI_GRAD_ADJ_S ||
+         epi == EPI_GRAD_ADJ_SU;
+}
+// epilogues that read e1
+constexpr bool e1_epi(int epi) { return adj_epi(epi) && epi != EPI_MUL_SLOPE; }
 
 // 8 consecutive 16-bit values at column f0 of row p (f0 a multiple of 8 and
 // below N; rows hold N rounded up to 8, so the access stays in the row)
@@ -280,7 +300,7 @@ __global__ __launch_bounds__(64 * kGemmWaves) void mlp_gemm_kernel(GemmArgs a) {
             const int f0 = 16 * ci * kNbChunk + 32 * pm + 8 * g;
             if (fast || f0 < a.N) {
               ea[pb][pm] = load8(a.e0, a.lde0, p, f0);
-              eb[pb][pm] = load8(a.e1, a.lde1, p, f0);
+              eb[pb][pm] = e1_epi(EPI) ? load8(a.e1, a.lde1, p, f0) : zero8();
             } else {
               ea[pb][pm] = zero8();
               eb[pb][pm] = zero8();
@@ -331,12 +351,21 @@ __global__ __launch_bounds__(64 * kGemmWaves) void mlp_gemm_kernel(GemmArgs a) {
             // y = vbar (the product), y2 = vbar * u * beta * s (1 - s)
             o[e] = to_bf(z);
             o2[e] = to_bf(z * bf(eb[pb][pm][e]) * (a.beta * code_curv(ea[pb][pm][e])));
-          } else {  // EPI_LAYER_ADJ: zbar = hbar * s + t
+          } else if constexpr (EPI == EPI_LAYER_ADJ) {  // zbar = hbar * s + t
             o[e] = to_bf(z * code_slope(ea[pb][pm][e]) + bf(eb[pb][pm][e]));
+          } else if constexpr (EPI == EPI_MUL_SLOPE) {
+            o[e] = to_bf(z * code_slope(ea[pb][pm][e]));
+          } else if constexpr (EPI == EPI_GRAD_ADJ_S) {
+            // y = vbar * s, y2 = vbar * (u s) * beta * (1 - s)
+            o[e] = to_bf(z * code_slope(ea[pb][pm][e]));
+            o2[e] = to_bf(z * bf(eb[pb][pm][e]) * (a.beta * code_one_minus_slope(ea[pb][pm][e])));
+          } else {  // EPI_GRAD_ADJ_SU: y = vbar * s, y2 = vbar * u * beta * s (1 - s)
+            o[e] = to_bf(z * code_slope(ea[pb][pm][e]));
+            o2[e] = to_bf(z * bf(eb[pb][pm][e]) * (a.beta * code_curv(ea[pb][pm][e])));
           }
         }
         store8(a.y, a.ldy, p, f0, o);
-        if constexpr (EPI == EPI_SOFTPLUS_TRAIN || EPI == EPI_GRAD_ADJ) store8(a.y2, a.ldy2, p, f0, o2);
+        if constexpr (two_out_epi(EPI)) store8(a.y2, a.ldy2, p, f0, o2);
       };
       if (fast) {
 #pragma unroll
@@ -811,6 +840,9 @@ int launch_gemm_epi(int epi, const GemmArgs& a, size_t lds, hipStream_t st) {
     GERAF_EPI(EPI_RELU)
     GERAF_EPI(EPI_GRAD_ADJ)
     GERAF_EPI(EPI_LAYER_ADJ)
+    GERAF_EPI(EPI_MUL_SLOPE)
+    GERAF_EPI(EPI_GRAD_ADJ_S)
+    GERAF_EPI(EPI_GRAD_ADJ_SU)
 #undef GERAF_EPI
     default:
       throw ArgError("unknown epilogue");
@@ -873,7 +905,7 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
     throw ArgError("N must be in [1, 272] and K in [1, 288], N > 256 with K <= 256");
   if (M == 0) return GERAF_RF_OK;
   if (prologue != PRO_NONE && prologue != PRO_MUL_SLOPE) throw ArgError("unknown prologue");
-  if (epilogue < EPI_BIAS || epilogue > EPI_LAYER_ADJ) throw ArgError("unknown epilogue");
+  if (epilogue < EPI_BIAS || epilogue > EPI_GRAD_ADJ_SU) throw ArgError("unknown epilogue");
   if (!(beta > 0.0f)) throw ArgError("beta must be positive");
   require_ptrs({x, wpacked, y}, true);
   check_ld(ldx, K, "x");
@@ -883,14 +915,17 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
     require_ptrs({x2}, true);
     check_ld(ldx2, K, "x2");
   }
-  if (epilogue == EPI_SOFTPLUS_TRAIN || epilogue == EPI_GRAD_ADJ) {
+  if (two_out_epi(epilogue)) {
     require_ptrs({y2}, true);
     check_ld(ldy2, N, "y2");
     if (ldy2 == 0) throw ArgError("y2: leading dimension must not be 0");
   }
-  if (epilogue == EPI_GRAD_ADJ || epilogue == EPI_LAYER_ADJ) {
-    require_ptrs({e0, e1}, true);
+  if (adj_epi(epilogue)) {
+    require_ptrs({e0}, true);
     check_ld(lde0, N, "e0");
+  }
+  if (e1_epi(epilogue)) {
+    require_ptrs({e1}, true);
     check_ld(lde1, N, "e1");
   }
   const uintptr_t align = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(x2) |

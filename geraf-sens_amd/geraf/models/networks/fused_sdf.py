@@ -11,14 +11,18 @@ otherwise, and the output layer [sdf, features] = W_8 h_7 + b_8.
 Normals, reverse mode by hand: u_7 = W_8[0], u_{l-1} = (u_l * s_l) W_l for
 l = 7..1 (s_l the softplus slope at z_l; layer 4's product splits into u_3
 and the skip's share of d sdf / d gamma), d sdf / d gamma = (u_0 * s_0) W_0 +
-the skip share, then through the encoding's Jacobian.  One product per layer.
+the skip share, then through the encoding's Jacobian.  One product per layer;
+each stores g_{l-1} = u_{l-1} * s_{l-1} (its epilogue applies the slope of the
+layer below), the form every later use needs.
 
 Backward, for incoming gradients (d sdf, d features, d normals):
   * the adjoint of the normal pass (layer 0 upward): d gamma = J^T d n;
     vbar_0 = d gamma W_0^T, and for l = 0..6 vbar_{l+1} = (vbar_l * s_l)
     W_{l+1}^T, with dW_{l+1} += (u_{l+1} s_{l+1})^T (vbar_l s_l), dW_0 +=
     (u_0 s_0)^T d gamma, dW_8[0] += sum (vbar_7 s_7); each layer leaves the
-    second-order term t_l = vbar_l * u_l * s'_l for the primal adjoint;
+    second-order term t_l = vbar_l * u_l * s'_l for the primal adjoint.  The
+    products store vbar_l s_l (and t_l = vbar_l (u_l s_l) beta (1 - s_l)), so
+    the weight gradients read two tensors and no slope codes;
   * the primal adjoint (layer 8 downward): zbar_8 = (d sdf, d features),
     zbar_{l-1} = (zbar_l W_l) * s_{l-1} + t_{l-1}, dW_l += zbar_l^T a_l,
     db_l = sum zbar_l.
@@ -46,7 +50,8 @@ BF = torch.bfloat16
 CODE = torch.float16  # softplus slope codes
 BETA, THRESHOLD = 100.0, 20.0
 _PRO_NONE, _PRO_SLOPE = 0, 1
-_EPI_BIAS, _EPI_SOFTPLUS, _EPI_SOFTPLUS_TRAIN, _EPI_RELU, _EPI_GRAD_ADJ, _EPI_LAYER_ADJ = range(6)
+(_EPI_BIAS, _EPI_SOFTPLUS, _EPI_SOFTPLUS_TRAIN, _EPI_RELU, _EPI_GRAD_ADJ, _EPI_LAYER_ADJ,
+ _EPI_MUL_SLOPE, _EPI_GRAD_ADJ_S, _EPI_GRAD_ADJ_SU) = range(9)
 _SLOPE_ONE = 0.0  # the slope code of slope 1, curvature 0
 
 
@@ -184,8 +189,9 @@ def _forward(x, ws, bs, P, train: bool):
 
 
 def _normal_pass(x, ws, qs, Q):
-    """u_l for l = 0..6 (u_3's buffer also carries the skip's d sdf / d gamma
-    in columns 217..255), u_7 = W_8[0] (one broadcast row), and the normals."""
+    """g_l = u_l s_l for l = 0..6 (g_3's buffer also carries the skip's
+    d sdf / d gamma in columns 217..255, where q_3 holds slope 1), u_7 =
+    W_8[0] (one broadcast row, not multiplied), and the normals."""
     M = x.shape[0]
     dev = x.device
     u7 = torch.zeros((1, 256), dtype=BF, device=dev)
@@ -195,11 +201,15 @@ def _normal_pass(x, ws, qs, Q):
     us[7] = u7
     for l in range(7, 0, -1):
         u = torch.empty((M, 256), dtype=BF, device=dev)
-        gemm(us[l], Q[l], IN[l], OUT[l], u, x2=qs[l], prologue=_PRO_SLOPE)
+        if l == 7:
+            gemm(u7, Q[l], IN[l], OUT[l], u, x2=qs[l], e0=qs[l - 1], prologue=_PRO_SLOPE,
+                 epilogue=_EPI_MUL_SLOPE)
+        else:
+            gemm(us[l], Q[l], IN[l], OUT[l], u, e0=qs[l - 1], epilogue=_EPI_MUL_SLOPE)
         us[l - 1] = u
     g0 = torch.empty((M, 40), dtype=BF, device=dev)
     # layer 0's product with W_0^T: the packed W_0 read as [in, out]
-    gemm(us[0], pack(ws[0], True), IN[0], OUT[0], g0, x2=qs[0], prologue=_PRO_SLOPE)
+    gemm(us[0], pack(ws[0], True), IN[0], OUT[0], g0)
     n = torch.empty((M, 3), dtype=torch.float32, device=dev)
     skip = us[3][:, 217:]
     _native.call("geraf_mlp_pe_gradient", _ptr(x), M, L_BANDS, 1.0, _ptr(g0), 40, _ptr(skip), 256,
@@ -245,11 +255,12 @@ class _FusedSDF(Function):
             dgam = torch.empty((M, 40), dtype=BF, device=dev)
             _native.call("geraf_mlp_pe_gradient_adjoint", _ptr(x), M, L_BANDS, 1.0, _ptr(dn),
                          _ptr(dgam), 40, 0, 40, _native.stream_of(x))
-            # dW_0 += (u_0 s_0)^T d gamma (before u_0 is overwritten by t_0)
-            wgrad(us[0], dgam, OUT[0], IN[0], dws[0], g2=qs[0])
+            # dW_0 += (u_0 s_0)^T d gamma (before g_0 = u_0 s_0 is overwritten
+            # by t_0); vb holds vbar_l s_l from here on
+            wgrad(us[0], dgam, OUT[0], IN[0], dws[0])
             vb = torch.empty((M, 256), dtype=BF, device=dev)
             gemm(dgam, P[0], OUT[0], IN[0], vb, y2=us[0], e0=qs[0], e1=us[0],
-                 epilogue=_EPI_GRAD_ADJ)
+                 epilogue=_EPI_GRAD_ADJ_S)
             ts[0] = us[0]
             for l in range(7):
                 nxt = l + 1
@@ -258,23 +269,23 @@ class _FusedSDF(Function):
                     _native.call("geraf_mlp_pe_gradient_adjoint", _ptr(x), M, L_BANDS, 1.0,
                                  _ptr(dn), _ptr(vb[:, 217:]), 256, 0, 39,
                                  _native.stream_of(x))
-                u_n = us[nxt]
-                wgrad(u_n, vb, OUT[nxt], IN[nxt], dws[nxt], g2=qs[nxt], x2=qs[l],
-                      accumulate=True)
+                u_n = us[nxt]  # g_nxt = u_nxt s_nxt, or the raw broadcast u_7
                 vb_n = torch.empty((M, 256), dtype=BF, device=dev)
                 if nxt == 7:
+                    wgrad(u_n, vb, OUT[nxt], IN[nxt], dws[nxt], g2=qs[nxt], accumulate=True)
                     ts[7] = torch.empty((M, 256), dtype=BF, device=dev)
-                    t_out = ts[7]
+                    gemm(vb, P[nxt], OUT[nxt], IN[nxt], vb_n, y2=ts[7], e0=qs[nxt], e1=u_n,
+                         epilogue=_EPI_GRAD_ADJ_SU)
                 else:
-                    t_out = u_n
+                    wgrad(u_n, vb, OUT[nxt], IN[nxt], dws[nxt], accumulate=True)
+                    gemm(vb, P[nxt], OUT[nxt], IN[nxt], vb_n, y2=u_n, e0=qs[nxt], e1=u_n,
+                         epilogue=_EPI_GRAD_ADJ_S)
                     ts[nxt] = u_n
-                gemm(vb, P[nxt], OUT[nxt], IN[nxt], vb_n, x2=qs[l], y2=t_out, e0=qs[nxt],
-                     e1=u_n, prologue=_PRO_SLOPE, epilogue=_EPI_GRAD_ADJ)
                 vb = vb_n
             # dW_8[0] += sum over the points of vbar_7 s_7
             ones = torch.ones((1, 8), dtype=BF, device=dev).expand(M, 8)
             row = torch.zeros((1, 256), dtype=torch.float32, device=dev)
-            wgrad(ones, vb, 1, 256, row, x2=qs[7])
+            wgrad(ones, vb, 1, 256, row)
             dws[8][0] += row[0]
             del vb
         # primal adjoint; the output layer's gradient [d features | d sdf]

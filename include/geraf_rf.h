@@ -393,7 +393,10 @@ int geraf_softplus_grad2_bf16(const uint16_t* ggx, const uint16_t* g, const uint
  *   epilogue (z = the product + bias, bias may be NULL):
  *     0: y = z;  1: y = softplus(z);  2: y = softplus(z), y2 = code(z) (FP16);
  *     3: y = relu(z);  4: y = z, y2 = z * e1 * beta * s(e0) (1 - s(e0));
- *     5: y = z * s(e0) + e1  (e0 slope codes, e1 BF16).
+ *     5: y = z * s(e0) + e1  (e0 slope codes, e1 BF16);
+ *     6: y = z * s(e0);
+ *     7: y = z * s(e0), y2 = z * e1 * beta * (1 - s(e0));
+ *     8: y = z * s(e0), y2 = z * e1 * beta * s(e0) (1 - s(e0)).
  *   Softplus(beta, threshold) as torch's, evaluated as (max(t,0) +
  *   log1p(exp(-|t|))) / beta, t = beta z, which equals torch's linear branch
  *   to an FP32 ulp for t > threshold; threshold must be >= 15.  Outputs are

@@ -109,6 +109,21 @@ def test_layer_product_epilogues(cuda, M, N, K, pro):
     fs.gemm(x, packed, N, K, y, x2=x2 if pro else None, e0=e0, e1=e1, prologue=pro, epilogue=5)
     want = z0 * _code_slope(e0[:, :N]) + e1[:, :N].float()
     assert _rel(y[:, :N].float(), want) <= 8e-3
+    # slope-multiplied outputs: y = z s(e0) (6, no e1); with y2 = z e1 beta
+    # (1 - s(e0)) (7) or z e1 beta s(e0) (1 - s(e0)) (8)
+    zs = z0 * _code_slope(e0[:, :N])
+    y.fill_(float("nan"))
+    fs.gemm(x, packed, N, K, y, x2=x2 if pro else None, e0=e0, prologue=pro, epilogue=6)
+    assert _rel(y[:, :N].float(), zs) <= 8e-3
+    one_minus = 1.0 - _code_slope(e0[:, :N])
+    for epi, fac in ((7, one_minus), (8, _code_curv(e0[:, :N]))):
+        y.fill_(float("nan"))
+        y2.fill_(float("nan"))
+        fs.gemm(x, packed, N, K, y, x2=x2 if pro else None, y2=y2, e0=e0, e1=e1, prologue=pro,
+                epilogue=epi)
+        assert _rel(y[:, :N].float(), zs) <= 8e-3, epi
+        want = z0 * e1[:, :N].float() * BETA * fac
+        assert _rel(y2[:, :N].float(), want) <= 1e-2, epi
 
 
 def test_layer_product_transposed_pack_and_broadcast(cuda):
