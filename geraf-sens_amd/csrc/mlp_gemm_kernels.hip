@@ -41,10 +41,11 @@
 // a hidden layer); z * s(e0) (the gradient pass, storing u s); z * s(e0)
 // and z * e1 * beta * (1 - s(e0)) with e1 = u s, or z * e1 * beta *
 // s(1 - s) with e1 = u (its adjoint, storing vbar s, so that the weight
-// gradients and the next product need no slope prologue).  Softplus follows torch.nn.Softplus(beta, threshold): y
-// = z if beta z > threshold else log1p(exp(beta z)) / beta, evaluated as
-// (max(t, 0) + log1p(exp(-|t|))) / beta with the hardware exp2 / log2 /
-// reciprocal (log1p by its series below 2^-6).
+// gradients and the next product need no slope prologue).  Softplus
+// follows torch.nn.Softplus(beta, threshold): y = z if beta z > threshold
+// else log1p(exp(beta z)) / beta, evaluated as (max(t, 0) +
+// log1p(exp(-|t|))) / beta with the hardware exp2 / log2 / reciprocal
+// (log1p by its series below 2^-6).
 //
 // Weight gradients dW[N,K] = G^T X (G: [M,N], X: [M,K], both tall) with the
 // same optional prologue on either side: split over the points, each
@@ -61,6 +62,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 #include "abi_util.hpp"
 #include "rf_common.hpp"
@@ -410,6 +412,239 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__
   out[idx] = v;
 }
 
+// ------------------------------------------------------------------ chained forward
+//
+// The SDF network's eight hidden layers in one kernel, for the forward
+// alone (the sdf of each point, no intermediate in HBM) or for training
+// (h_l and the slope codes q_l stored, as the layer products store them).
+// A wave carries 32 points through every layer with their activations in
+// registers: the packed weights permute each layer's output rows so that
+// the BF16 outputs of blocks 2m and 2m + 1 in lane (j, g) are exactly the
+// B fragment of step m of the next layer (8 consecutive features of point
+// j), so a layer's epilogue writes the next layer's operands directly.
+// The weights stream through LDS in chunks of 4 output blocks (32 KB,
+// double-buffered): while a chunk is multiplied, the next one is loaded
+// into registers and written to the other buffer, one workgroup barrier
+// per chunk.  A workgroup (8 waves, 256 points per tile) also keeps its
+// tile's encoded inputs gamma in LDS, for layer 0 and the skip at layer 4.
+// Every product, bias and softplus is the layer product's own (same
+// instruction, step order and rounding), so h_l and q_l are bit for bit
+// those of the layer-by-layer form; the sdf is W_8[0] . h_7 + b_8[0] summed
+// in FP32 (the layer form rounds it to BF16).
+
+constexpr int kChainWaves = 8;
+constexpr int kChainPB = 2;                               // 16-point blocks per wave
+constexpr int kChainPts = 16 * kChainPB * kChainWaves;    // points per workgroup tile
+constexpr int kChainBlk = 4;                              // output blocks per weight chunk
+constexpr int kChainFrags = kChainBlk * 8 * kWave;        // bf8 per chunk buffer (32 KB)
+constexpr int kChainLoads = kChainFrags / (kWave * kChainWaves);  // per thread
+constexpr int kGamStride = 72;                            // halfwords per gamma row in LDS
+
+struct ChainArgs {
+  const float* x;      // [M,3]
+  const bf8* w;        // packed W_0 .. W_7 back to back (layer 0: K = din; layer 3: N = n3)
+  const float* bias;   // [8][256]
+  const float* w_sdf;  // W_8[0] (rounded to BF16 here, as the pack does), then b_8[0]
+  float* sdf;          // [M] or null
+  uint16_t* h;         // layer l at h + l * sh, [M, ldh] BF16, or null (training)
+  uint16_t* q;         // the FP16 codes, likewise
+  int64_t ldh, ldq, sh, sq;
+  float scale, beta, inv_beta;
+  int M, L, din, n3, nb3;
+};
+
+// fragments (bf8) before layer l in the packed buffer: layer 0 has 2 steps
+// of 32 inputs, the others 8; layer 3 has nb3 output blocks, the others 16
+__device__ __forceinline__ int chain_w_offset(const ChainArgs& a, int l) {
+  if (l == 0) return 0;
+  const int full = l - 1 - (l > 3 ? 1 : 0);  // layers 1 .. l - 1 other than 3
+  return (16 * 2 + 16 * 8 * full + (l > 3 ? a.nb3 * 8 : 0)) * kWave;
+}
+
+// gamma feature f (< din) of the point v = x * scale, as pe_kernel forms it
+__device__ __forceinline__ float gamma_feature(const float (&v)[3], int f) {
+  if (f < 3) return v[f];
+  const int b = (f - 3) / 6, r = (f - 3) % 6, d = r % 3;
+  float sn, cs;
+  sincosf(v[d] * (float)(1 << b), &sn, &cs);
+  return r < 3 ? sn : cs;
+}
+
+template <bool TRAIN>
+__global__ __launch_bounds__(64 * kChainWaves) void mlp_sdf_chain_kernel(ChainArgs a) {
+  extern __shared__ bf8 cw[];  // [2][kChainFrags] weight chunks
+  float* bias_l = reinterpret_cast<float*>(cw + 2 * kChainFrags);  // [8][256]
+  float* wsdf_l = bias_l + 8 * 256;                                 // [257]: w, b
+  uint16_t* gam = reinterpret_cast<uint16_t*>(wsdf_l + 260);        // [kChainPts][kGamStride]
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  for (int i = tid; i < 8 * 256; i += blockDim.x) {
+    const int l = i >> 8, f = i & 255;
+    const int n = l == 3 ? a.n3 : 256;
+    bias_l[i] = f < n ? a.bias[i] : 0.0f;
+  }
+  if (a.sdf != nullptr)
+    for (int i = tid; i < 257; i += blockDim.x)
+      wsdf_l[i] = i < 256 ? bf(to_bf(a.w_sdf[i])) : a.w_sdf[i];
+  const int tiles = (a.M + kChainPts - 1) / kChainPts;
+
+  // chunk ci of layer l: blocks 4 ci .. 4 ci + 3 of the packed W_l, steps
+  // of 32 inputs each (2 for layer 0), zero past the layer's blocks
+  bf8 stage[kChainLoads];
+  auto load_chunk = [&](int l, int ci) __attribute__((always_inline)) {
+    const int ks = l == 0 ? 2 : 8;
+    const int nb = l == 3 ? a.nb3 : 16;
+    const int valid = max(0, min(kChainBlk, nb - kChainBlk * ci)) * ks * kWave;
+    const bf8* src = a.w + chain_w_offset(a, l) + kChainBlk * ci * ks * kWave;
+#pragma unroll
+    for (int i = 0; i < kChainLoads; ++i) {
+      const int e = tid + i * kWave * kChainWaves;
+      stage[i] = e < valid ? src[e] : __builtin_bit_cast(bf8, zero8());
+    }
+  };
+  auto store_chunk = [&](int buf, int l) __attribute__((always_inline)) {
+    const int n = (l == 0 ? 2 : 8) * kChainBlk * kWave;
+#pragma unroll
+    for (int i = 0; i < kChainLoads; ++i) {
+      const int e = tid + i * kWave * kChainWaves;
+      if (e < n) cw[buf * kChainFrags + e] = stage[i];
+    }
+  };
+  int t = blockIdx.x;
+  if (t < tiles) {
+    load_chunk(0, 0);
+    store_chunk(0, 0);
+  }
+
+  bf8 B[kChainPB][8], Bn[kChainPB][8];
+  for (; t < tiles; t += gridDim.x) {
+    const bool more = t + (int)gridDim.x < tiles;
+    // this tile's gamma rows: two threads per point (x and bands < 3, bands >= 3)
+    {
+      const int pt = tid >> 1, half = tid & 1;
+      const int p = min(t * kChainPts + pt, a.M - 1);
+      float v[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = a.x[3 * (size_t)p + d] * a.scale;
+      uint16_t* row = gam + pt * kGamStride;
+      const int fa = half == 0 ? 0 : 3 + 6 * ((a.L + 1) / 2);
+      const int fb = half == 0 ? 3 + 6 * ((a.L + 1) / 2) : 64;
+      for (int f = fa; f < fb; ++f) row[f] = f < a.din ? to_bf(gamma_feature(v, f)) : (uint16_t)0;
+    }
+    const int pw = t * kChainPts + 32 * wave;  // the wave's first point
+    // a lane's fragment of gamma for inputs 32 ks + 8 g .. + 7 of its point
+    auto gamma_frag = [&](int pb, int ks) __attribute__((always_inline)) {
+      const uint16_t* row = gam + (32 * wave + 16 * pb + j) * kGamStride + 32 * ks + 8 * g;
+      return *reinterpret_cast<const bf8*>(row);
+    };
+
+    // one chunk of layer l: products, epilogue into Bn (and the training
+    // stores); the next chunk's weights loaded meanwhile
+    auto chunk = [&](auto ks_tag, int l, int ci) __attribute__((always_inline)) {
+      constexpr int KS = decltype(ks_tag)::value;
+      __syncthreads();  // chunk ci is in its buffer; the other buffer is free
+      const int nl = ci < 3 ? l : l + 1;
+      const int nci = ci < 3 ? ci + 1 : 0;
+      const bool has_next = nl < 8 || more;
+      if (has_next) load_chunk(nl & 7, nci);
+      const bf8* wb = cw + (ci & 1) * kChainFrags + lane;
+      f32x4v acc[kChainPB][kChainBlk];
+#pragma unroll
+      for (int b = 0; b < kChainBlk; ++b)
+#pragma unroll
+        for (int pb = 0; pb < kChainPB; ++pb) acc[pb][b] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int b = 0; b < kChainBlk; ++b) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const bf8 w = wb[(b * KS + ks) * kWave];
+#pragma unroll
+          for (int pb = 0; pb < kChainPB; ++pb)
+            acc[pb][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, B[pb][ks], acc[pb][b], 0, 0, 0);
+        }
+      }
+      const float* bl = bias_l + 256 * l;
+#pragma unroll
+      for (int pb = 0; pb < kChainPB; ++pb) {
+        const int p = pw + 16 * pb + j;
+#pragma unroll
+        for (int pm = 0; pm < 2; ++pm) {
+          const int f0 = 64 * ci + 32 * pm + 8 * g;
+          const f32x4v b0 = *reinterpret_cast<const f32x4v*>(bl + f0);
+          const f32x4v b1 = *reinterpret_cast<const f32x4v*>(bl + f0 + 4);
+          u16x8 o, o2;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const float z = e < 4 ? acc[pb][2 * pm][e] + b0[e] : acc[pb][2 * pm + 1][e - 4] + b1[e - 4];
+            const Softplus sp = softplus_of<TRAIN>(z, a.beta, a.inv_beta);
+            o[e] = to_bf(sp.y);
+            if (TRAIN) o2[e] = to_hf(sp.q);
+          }
+          if (l == 3 && f0 + 8 > a.n3) {
+            // the skip input [h_3, gamma]: gamma past layer 3's outputs, slope 1
+            const uint16_t* row = gam + (32 * wave + 16 * pb + j) * kGamStride;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              if (f0 + e >= a.n3) {
+                o[e] = row[f0 + e - a.n3];
+                o2[e] = 0;
+              }
+          }
+          Bn[pb][2 * ci + pm] = __builtin_bit_cast(bf8, o);
+          if (TRAIN && p < a.M) {
+            store8(a.h + l * a.sh, a.ldh, p, f0, o);
+            store8(a.q + l * a.sq, a.ldq, p, f0, o2);
+          }
+        }
+      }
+      if (has_next) store_chunk(ci & 1 ? 0 : 1, nl & 7);
+    };
+    using Ks2 = std::integral_constant<int, 2>;
+    using Ks8 = std::integral_constant<int, 8>;
+
+    // layer 0 reads gamma (visible after the first chunk's barrier)
+    __syncthreads();
+#pragma unroll
+    for (int pb = 0; pb < kChainPB; ++pb) {
+      B[pb][0] = gamma_frag(pb, 0);
+      B[pb][1] = gamma_frag(pb, 1);
+    }
+    chunk(Ks2{}, 0, 0);
+    chunk(Ks2{}, 0, 1);
+    chunk(Ks2{}, 0, 2);
+    chunk(Ks2{}, 0, 3);
+#pragma unroll 1
+    for (int l = 1; l < 8; ++l) {
+#pragma unroll
+      for (int pb = 0; pb < kChainPB; ++pb)
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) B[pb][ks] = Bn[pb][ks];
+      chunk(Ks8{}, l, 0);
+      chunk(Ks8{}, l, 1);
+      chunk(Ks8{}, l, 2);
+      chunk(Ks8{}, l, 3);
+    }
+    if (a.sdf != nullptr) {
+#pragma unroll
+      for (int pb = 0; pb < kChainPB; ++pb) {
+        float s = 0.0f;
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+          const u16x8 hv = __builtin_bit_cast(u16x8, Bn[pb][ks]);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) s = fmaf(bf(hv[e]), wsdf_l[32 * ks + 8 * g + e], s);
+        }
+        s += __shfl_xor(s, 16, kWave);
+        s += __shfl_xor(s, 32, kWave);
+        const int p = pw + 16 * pb + j;
+        if (g == 0 && p < a.M) a.sdf[p] = s + wsdf_l[256];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ weight gradients
 
 constexpr int kWgWaves = 8;
@@ -605,22 +840,34 @@ __global__ __launch_bounds__(64 * kWgWaves) void mlp_wgrad_kernel(WgradArgs a) {
   }
 }
 
-// dW[n][k] (+)= sum over the partial blocks in order; db[n] likewise.
-__global__ __launch_bounds__(256) void mlp_wgrad_reduce_kernel(const float* __restrict__ partial,
-                                                               int blocks, int rows, int n0,
-                                                               int k0, int N, int K,
-                                                               float* __restrict__ dw,
-                                                               int64_t lddw,
-                                                               float* __restrict__ db,
-                                                               int accumulate) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rows * kWgCols) return;
+// dW[n][k] (+)= the sum over the partial blocks; db[n] likewise.  A
+// workgroup takes 64 consecutive cells: wave w sums the blocks b = w mod 8
+// (coalesced rows, eight waves in flight per cell instead of one serial
+// chain), then the eight sums are added in wave order (deterministic).
+constexpr int kRedWaves = 8;
+
+__global__ __launch_bounds__(64 * kRedWaves) void mlp_wgrad_reduce_kernel(
+    const float* __restrict__ partial, int blocks, int rows, int n0, int k0, int N, int K,
+    float* __restrict__ dw, int64_t lddw, float* __restrict__ db, int accumulate) {
+  __shared__ double part[kRedWaves][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x >> 6;
+  const int idx = blockIdx.x * kWave + lane;
+  const bool in = idx < rows * kWgCols;
+  double s = 0.0;
+  if (in)
+    for (int b = wave; b < blocks; b += kRedWaves)
+      s += (double)partial[(size_t)b * kWgRows * kWgCols + idx];
+  part[wave][lane] = s;
+  __syncthreads();
+  if (wave != 0 || !in) return;
+  s = part[0][lane];
+#pragma unroll
+  for (int w = 1; w < kRedWaves; ++w) s += part[w][lane];
   const int n = idx / kWgCols, c = idx % kWgCols;
   const bool is_bias = c == kWgTile;
   if (n0 + n >= N) return;
   if (is_bias ? (db == nullptr || k0 != 0) : (k0 + c >= K)) return;
-  double s = 0.0;
-  for (int b = 0; b < blocks; ++b) s += (double)partial[(size_t)b * kWgRows * kWgCols + idx];
   if (is_bias) {
     db[n0 + n] = (float)(accumulate ? (double)db[n0 + n] + s : s);
   } else {
@@ -867,10 +1114,85 @@ int launch_wgrad(const WgradArgs& b, int blocks, hipStream_t st) {
   return check_launch("mlp_wgrad_kernel");
 }
 
+size_t chain_lds_bytes() {
+  return (size_t)2 * kChainFrags * sizeof(bf8) + (size_t)(8 * 256 + 260) * sizeof(float) +
+         (size_t)kChainPts * kGamStride * sizeof(uint16_t);
+}
+
 }  // namespace
 }  // namespace geraf
 
 using namespace geraf;
+
+// layer 0 takes two steps of 32 inputs: 33 <= 3 + 6 L <= 64
+extern "C" int64_t geraf_mlp_sdf_chain_pack_bytes(int32_t L) {
+  if (L < 5 || L > 10) return -1;
+  const int n3 = 256 - (3 + 6 * L);
+  return (int64_t)(16 * 2 + 16 * 8 * 6 + nb_of(n3) * 8) * kWave * 16;
+}
+
+extern "C" int geraf_mlp_sdf_chain_bf16(const float* x, int64_t M, int32_t L, float scale,
+                                        const void* wpacked, const float* bias,
+                                        const float* w_sdf, float* sdf,
+                                        uint16_t* h, int64_t ldh, int64_t layer_stride_h,
+                                        uint16_t* q, int64_t ldq, int64_t layer_stride_q,
+                                        float beta, float threshold, geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  if (M < 0 || M > INT32_MAX) throw ArgError("M out of range");
+  if (L < 5 || L > 10) throw ArgError("L must be in [5, 10]");
+  if (!(beta > 0.0f)) throw ArgError("beta must be positive");
+  if (!(threshold >= 15.0f)) throw ArgError("threshold must be >= 15 (torch's default is 20)");
+  const bool train = h != nullptr || q != nullptr;
+  if (train && (h == nullptr || q == nullptr)) throw ArgError("h and q go together");
+  if (!train && sdf == nullptr) throw ArgError("nothing to compute (sdf, h and q are null)");
+  if (sdf != nullptr && w_sdf == nullptr) throw ArgError("sdf needs w_sdf");
+  if (M == 0) return GERAF_RF_OK;
+  require_ptrs({x, wpacked, bias}, true);
+  if ((reinterpret_cast<uintptr_t>(wpacked) & 15u) != 0) throw ArgError("wpacked must be 16-byte aligned");
+  if (train) {
+    require_ptrs({h, q}, true);
+    if (((reinterpret_cast<uintptr_t>(h) | reinterpret_cast<uintptr_t>(q)) & 15u) != 0)
+      throw ArgError("h and q must be 16-byte aligned");
+    if (ldh < 256 || ldh % 8 != 0 || ldq < 256 || ldq % 8 != 0)
+      throw ArgError("ldh and ldq must be multiples of 8 and >= 256");
+    if (layer_stride_h % 8 != 0 || layer_stride_q % 8 != 0 || layer_stride_h < M * ldh ||
+        layer_stride_q < M * ldq)
+      throw ArgError("layer strides must be multiples of 8 covering M rows");
+  }
+  ChainArgs a;
+  a.x = x;
+  a.w = reinterpret_cast<const bf8*>(wpacked);
+  a.bias = bias;
+  a.h = h;
+  a.q = q;
+  a.sh = layer_stride_h;
+  a.sq = layer_stride_q;
+  a.w_sdf = w_sdf;
+  a.sdf = sdf;
+  a.ldh = ldh;
+  a.ldq = ldq;
+  a.scale = scale;
+  a.beta = beta;
+  a.inv_beta = 1.0f / beta;
+  a.M = (int)M;
+  a.L = L;
+  a.din = 3 + 6 * L;
+  a.n3 = 256 - a.din;
+  a.nb3 = nb_of(a.n3);
+  const size_t lds = chain_lds_bytes();
+  const int64_t tiles = (M + kChainPts - 1) / kChainPts;
+  const dim3 grid((unsigned)std::min<int64_t>(tiles, cu_count()));
+  const hipStream_t st = (hipStream_t)stream;
+  if (train) {
+    allow_lds(mlp_sdf_chain_kernel<true>, lds);
+    hipLaunchKernelGGL(mlp_sdf_chain_kernel<true>, grid, dim3(64 * kChainWaves), lds, st, a);
+  } else {
+    allow_lds(mlp_sdf_chain_kernel<false>, lds);
+    hipLaunchKernelGGL(mlp_sdf_chain_kernel<false>, grid, dim3(64 * kChainWaves), lds, st, a);
+  }
+  return check_launch("mlp_sdf_chain_kernel");
+  GERAF_TRY_END
+}
 
 extern "C" int64_t geraf_mlp_pack_bytes(int64_t N, int64_t K) {
   if (N <= 0 || K <= 0 || N > kMaxN || K > kMaxK || gemm_lds_bytes(N, K) > kLdsBytes) return -1;
@@ -1054,7 +1376,8 @@ extern "C" int geraf_mlp_wgrad_bf16(const uint16_t* g, int64_t ldg, const uint16
         if (rc != GERAF_RF_OK) return rc;
       }
       const int cells = rows * kWgCols;
-      hipLaunchKernelGGL(mlp_wgrad_reduce_kernel, dim3((cells + 255) / 256), dim3(256), 0, st,
+      hipLaunchKernelGGL(mlp_wgrad_reduce_kernel, dim3((cells + kWave - 1) / kWave),
+                         dim3(64 * kRedWaves), 0, st,
                          partial, M == 0 ? 0 : blocks, rows, (int)n0, kz * kWgTile, (int)N,
                          (int)K, dw, lddw, (kz == 0) ? db : nullptr, (int)accumulate);
       const int rc = check_launch("mlp_wgrad_reduce_kernel");

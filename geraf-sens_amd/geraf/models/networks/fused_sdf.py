@@ -63,14 +63,19 @@ def _ld(t: Optional[torch.Tensor]) -> int:
     return 0 if t is None else (0 if t.stride(0) == 0 else t.stride(0))
 
 
-def pack(w: torch.Tensor, transpose: bool, scale: float = 1.0) -> torch.Tensor:
-    """Operand-order BF16 copy of W [N,K] (or of W^T with transpose)."""
+def pack(w: torch.Tensor, transpose: bool, scale: float = 1.0,
+         out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Operand-order BF16 copy of W [N,K] (or of W^T with transpose), into
+    ``out`` (a uint8 slice of the packed size) when given."""
     w = w.detach().float().contiguous()
     N, K = (w.shape[1], w.shape[0]) if transpose else (w.shape[0], w.shape[1])
     size = int(_native.load().geraf_mlp_pack_bytes(N, K))
     if size < 0:
         raise ValueError(f"layer of {N} x {K} is beyond the kernels' limits")
-    out = torch.empty(size, dtype=torch.uint8, device=w.device)
+    if out is None:
+        out = torch.empty(size, dtype=torch.uint8, device=w.device)
+    elif out.numel() != size:
+        raise ValueError("pack: out has the wrong size")
     _native.call("geraf_mlp_pack_weights", _ptr(w), w.stride(0), int(transpose), float(scale),
                  N, K, _ptr(out), _native.stream_of(w))
     return out
@@ -134,19 +139,32 @@ def _out_rows(t: torch.Tensor) -> torch.Tensor:
     return torch.cat([t[1:], t[:1]])
 
 
+def _hidden_pack(ws):
+    """The hidden layers' packed W_0 .. W_7 back to back (the chained
+    forward's operand) and the slice of each (the layer products')."""
+    lib = _native.load()
+    total = int(lib.geraf_mlp_sdf_chain_pack_bytes(L_BANDS))
+    buf = torch.empty(total, dtype=torch.uint8, device=ws[0].device)
+    views, o = [], 0
+    for l in range(NL - 1):
+        n = int(lib.geraf_mlp_pack_bytes(OUT[l], IN[l]))
+        views.append(pack(ws[l], False, 1.0 / SQRT2 if l == SKIP else 1.0, out=buf[o:o + n]))
+        o += n
+    if o != total:
+        raise RuntimeError("hidden-layer pack sizes disagree with the chained forward's")
+    return buf, views
+
+
 def _packs(ws):
     """P_l packs W_l ([out, in]; the forward and the normal-pass adjoint),
     Q_l packs W_l^T ([in, out]; the normal pass and the primal adjoint);
     layer 4 carries the skip's 1/sqrt 2, layer 8 its rows in _out_rows
-    order."""
-    P, Q = [], []
-    for l, w in enumerate(ws):
-        sc = 1.0 / SQRT2 if l == SKIP else 1.0
-        if l == NL - 1:
-            w = _out_rows(w.detach())
-        P.append(pack(w, False, sc))
-        Q.append(pack(w, True, sc) if l >= 1 else None)
-    return P, Q
+    order.  Also returns the buffer holding P_0 .. P_7."""
+    hidden, P = _hidden_pack(ws)
+    P.append(pack(_out_rows(ws[NL - 1].detach()), False))
+    Q = [None] + [pack(ws[l], True, 1.0 / SQRT2 if l == SKIP else 1.0) for l in range(1, NL - 1)]
+    Q.append(pack(_out_rows(ws[NL - 1].detach()), True))
+    return P, Q, hidden
 
 
 def _biases(params) -> List[torch.Tensor]:
@@ -155,9 +173,49 @@ def _biases(params) -> List[torch.Tensor]:
     return bs
 
 
-def _forward(x, ws, bs, P, train: bool):
-    """Hidden layers and the output layer.  Returns (gam, slope-code list, h
-    list, out8 [M, 264] = [features | sdf | padding])."""
+def _bias_table(bs) -> torch.Tensor:
+    """The hidden layers' biases as [8, 256] FP32 (zero past a layer's width)."""
+    t = torch.zeros((NL - 1, 256), dtype=torch.float32, device=bs[0].device)
+    for l in range(NL - 1):
+        t[l, :OUT[l]] = bs[l]
+    return t
+
+
+def _sdf_row(params) -> torch.Tensor:
+    """[W_8[0], b_8[0]] (257 FP32): the output layer's sdf row and bias."""
+    w8, b8 = params[2 * (NL - 1)], params[2 * (NL - 1) + 1]
+    return torch.cat([w8.detach()[0].float(), b8.detach()[:1].float()]).contiguous()
+
+
+def _chain(x, hidden, table, w_sdf=None, sdf=None, h=None, q=None) -> None:
+    M = x.shape[0]
+    _native.call("geraf_mlp_sdf_chain_bf16", _ptr(x), M, L_BANDS, 1.0, _ptr(hidden),
+                 _ptr(table), _ptr(w_sdf), _ptr(sdf), _ptr(h), 256, 0 if h is None else M * 256,
+                 _ptr(q), 256, 0 if q is None else M * 256, BETA, THRESHOLD,
+                 _native.stream_of(x))
+
+
+def _forward(x, P, hidden, table, b8):
+    """The training forward: the hidden layers chained in one kernel (h_l and
+    the slope codes q_l stored), then the output layer.  Returns (gam,
+    slope-code list, h list, out8 [M, 264] = [features | sdf | padding])."""
+    M = x.shape[0]
+    dev = x.device
+    gam = torch.empty((M, 64), dtype=BF, device=dev)
+    _native.call("geraf_mlp_positional_encoding", _ptr(x), M, L_BANDS, 1.0, _ptr(gam), 64, 0, 64,
+                 _native.stream_of(x))
+    h = torch.empty((NL - 1, M, 256), dtype=BF, device=dev)
+    q = torch.empty((NL - 1, M, 256), dtype=CODE, device=dev)
+    _chain(x, hidden, table, h=h, q=q)
+    hs, qs = list(h.unbind(0)), list(q.unbind(0))
+    out8 = torch.empty((M, 264), dtype=BF, device=dev)
+    gemm(hs[7], P[8], OUT[8], IN[8], out8, bias=b8)
+    return gam, qs, hs, out8
+
+
+def _forward_layers(x, ws, bs, P, train: bool):
+    """The same forward, one layer product per layer (the chained kernel's
+    reference in the tests).  Returns (gam, slope-code list, h list, out8)."""
     M = x.shape[0]
     dev = x.device
     st = _native.stream_of(x)
@@ -223,8 +281,8 @@ class _FusedSDF(Function):
         ws, bs = list(params[0::2]), _biases(params[1::2])
         _check_network(ws)
         x = x.detach().float().contiguous()
-        P, Q = _packs(ws)
-        gam, qs, hs, out8 = _forward(x, ws, bs, P, train=True)
+        P, Q, hidden = _packs(ws)
+        gam, qs, hs, out8 = _forward(x, P, hidden, _bias_table(bs), bs[NL - 1])
         sdf = out8[:, 256].float()
         feat = out8[:, :256]
         if with_normals:
@@ -338,12 +396,17 @@ def fused_sdf(net, x: torch.Tensor, with_normals: bool = True
 
 @torch.no_grad()
 def fused_sdf_value(net, x: torch.Tensor) -> torch.Tensor:
-    """The SDF alone [M] FP32, no gradients (the calibration's evaluations)."""
-    ws = [getattr(net, f"lin{l}").weight for l in range(NL)]
-    bs = _biases([getattr(net, f"lin{l}").bias for l in range(NL)])
+    """The SDF alone [M] FP32, no gradients (the calibration's evaluations):
+    the chained hidden layers and the sdf row, nothing stored per layer."""
+    params = []
+    for l in range(NL):
+        lin = getattr(net, f"lin{l}")
+        params += [lin.weight, lin.bias]
+    ws = list(params[0::2])
     _check_network(ws)
     x = x.detach().float().reshape(-1, 3).contiguous()
-    P = [pack(_out_rows(w.detach()) if l == NL - 1 else w, False,
-              1.0 / SQRT2 if l == SKIP else 1.0) for l, w in enumerate(ws)]
-    _, _, _, out8 = _forward(x, ws, bs, P, train=False)
-    return out8[:, 256].float()
+    hidden, _ = _hidden_pack(ws)
+    table = _bias_table([b.detach().float() for b in params[1::2]])
+    sdf = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+    _chain(x, hidden, table, w_sdf=_sdf_row(params), sdf=sdf)
+    return sdf

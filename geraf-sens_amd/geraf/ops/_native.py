@@ -87,6 +87,9 @@ _SIGNATURES = {
                               _P],
     "geraf_mlp_pe_gradient_adjoint": [_P, _I64, ctypes.c_int32, ctypes.c_float, _P, _P, _I64,
                                       ctypes.c_int32, ctypes.c_int32, _P],
+    "geraf_mlp_sdf_chain_bf16": [_P, _I64, ctypes.c_int32, ctypes.c_float, _P, _P, _P,
+                                 _P, _P, _I64, _I64, _P, _I64, _I64,
+                                 ctypes.c_float, ctypes.c_float, _P],
     "geraf_mf_forward_tiled_f32": [_P] * 8 + [_I64] * 3 + [ctypes.c_int32] * 2 +
                                   [_P, _I64, _PARAMS, _P],
     "geraf_mf_backward_tiled_f32": [_P] * 8 + [_I64] * 4 + [ctypes.c_int32] * 2 +
@@ -126,7 +129,8 @@ def load() -> ctypes.CDLL:
         lib.geraf_mf_forward_mma_workspace_bytes.restype = ctypes.c_int64
         lib.geraf_mf_forward_mma_workspace_bytes.argtypes = [_I64, _I64]
         for name, args in (("geraf_mlp_pack_bytes", [_I64, _I64]),
-                           ("geraf_mlp_wgrad_workspace_bytes", [_I64, _I64, _I64])):
+                           ("geraf_mlp_wgrad_workspace_bytes", [_I64, _I64, _I64]),
+                           ("geraf_mlp_sdf_chain_pack_bytes", [ctypes.c_int32])):
             getattr(lib, name).restype = ctypes.c_int64
             getattr(lib, name).argtypes = args
         _lib = lib
@@ -137,7 +141,8 @@ def exported_symbols():
     return list(_SIGNATURES) + ["geraf_rf_last_error", "geraf_rf_version",
                                 "geraf_mf_tiled_workspace_bytes",
                                 "geraf_mf_forward_mma_workspace_bytes", "geraf_mlp_pack_bytes",
-                                "geraf_mlp_wgrad_workspace_bytes"]
+                                "geraf_mlp_wgrad_workspace_bytes",
+                                "geraf_mlp_sdf_chain_pack_bytes"]
 
 
 def version() -> str:

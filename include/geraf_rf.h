@@ -421,7 +421,24 @@ int geraf_softplus_grad2_bf16(const uint16_t* ggx, const uint16_t* g, const uint
  * geraf_mlp_pe_gradient: n[p] = d sdf / d x from d sdf / d gamma = ga + gb
  * (two BF16 [M, 3+6L] parts), FP32 [M,3].
  * geraf_mlp_pe_gradient_adjoint: its transpose, d loss / d gamma = J dn
- * (BF16, into out at col0, zero-padded up to pad_to). */
+ * (BF16, into out at col0, zero-padded up to pad_to).
+ *
+ * geraf_mlp_sdf_chain_bf16: the SDF network's eight hidden layers in one
+ * kernel (gamma with 5 <= L <= 10 bands, d = 3 + 6L features, of the points x
+ * [M,3] times scale; 256-wide softplus layers, layer 3 of width 256 - d,
+ * the skip [h_3, gamma] into layer 4), activations carried in registers.
+ * wpacked holds the hidden layers' packed weights back to back, each as
+ * geraf_mlp_pack_weights writes it (W_0 [256, d], W_1 .. W_7, W_3 [256 - d,
+ * 256], W_4 times the skip's scale): geraf_mlp_sdf_chain_pack_bytes(L)
+ * bytes.  bias is [8][256] FP32 (layer 3's past its width unused).  With
+ * sdf not NULL: sdf[p] = bf16(w) . h_7 + b in FP32, w_sdf = [w, b] (257
+ * FP32: the output layer's sdf row and bias).  With h and q not NULL
+ * (training):
+ * h_l (BF16) and the slope codes q_l (FP16) of layer l at h + l *
+ * layer_stride_h and q + l * layer_stride_q, rows of ldh / ldq elements,
+ * 256 columns (h_3's last d columns gamma, q_3's there the code of slope
+ * 1), bit for bit what geraf_mlp_gemm_bf16 with epilogue 2 and the encoding
+ * write layer by layer. */
 int64_t geraf_mlp_pack_bytes(int64_t N, int64_t K);
 int geraf_mlp_pack_weights(const float* w, int64_t ldw, int32_t transpose, float scale,
                            int64_t N, int64_t K, void* packed, geraf_stream_t stream);
@@ -447,6 +464,13 @@ int geraf_mlp_pe_gradient(const float* x, int64_t M, int32_t L, float scale, con
 int geraf_mlp_pe_gradient_adjoint(const float* x, int64_t M, int32_t L, float scale,
                                   const float* dn, uint16_t* out, int64_t ldo, int32_t col0,
                                   int32_t pad_to, geraf_stream_t stream);
+int64_t geraf_mlp_sdf_chain_pack_bytes(int32_t L);
+int geraf_mlp_sdf_chain_bf16(const float* x, int64_t M, int32_t L, float scale,
+                             const void* wpacked, const float* bias, const float* w_sdf,
+                             float* sdf, uint16_t* h, int64_t ldh,
+                             int64_t layer_stride_h, uint16_t* q, int64_t ldq,
+                             int64_t layer_stride_q, float beta, float threshold,
+                             geraf_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -258,9 +258,40 @@ def test_fused_sdf_value_and_no_normals(cuda):
     with torch.no_grad():
         s64 = ref.double()(x.double())[:, 0]
     ref.float()
-    assert _rel(s1, s64) <= 1e-2 and torch.equal(s1, s2.detach())
+    # the value path sums the sdf row in FP32; the layer form rounds it to BF16
+    s2d = s2.detach()
+    assert _rel(s1, s64) <= 1e-2
+    assert float((s1 - s2d).abs().max()) <= 2.0 ** -8 * float(s2d.abs().max()) + 1e-6
     assert torch.all(n == 0)
     # no normals: the backward is the primal adjoint alone
     bfnet.zero_grad(set_to_none=True)
     s2.sum().backward()
     assert all(p.grad is not None and torch.isfinite(p.grad).all() for p in bfnet.parameters())
+
+
+@pytest.mark.parametrize("M", [1, 300, 70001])
+def test_chained_forward_matches_layers(cuda, M):
+    """The chained hidden layers (one kernel, activations in registers)
+    against the same forward run one layer product at a time: h_l and the
+    slope codes q_l bit for bit (the skip's gamma columns and slope-one
+    codes included), the output layer equal, and the value path's FP32 sdf
+    within the layer form's BF16 rounding.  M = 70,001 is 274 tiles of 256
+    points, more than one per workgroup."""
+    from geraf.models.networks import fused_sdf as fs
+
+    _, bfnet = _nets(cuda, seed=5)
+    g = torch.Generator(device="cpu").manual_seed(M)
+    x = (torch.rand(M, 3, generator=g) * 1.6 - 0.8).to(cuda)
+    params = [t for l in range(9)
+              for t in (getattr(bfnet, f"lin{l}").weight, getattr(bfnet, f"lin{l}").bias)]
+    ws, bs = list(params[0::2]), fs._biases(params[1::2])
+    P, _, hidden = fs._packs(ws)
+    _, q0, h0, out0 = fs._forward_layers(x, ws, bs, P, train=True)
+    _, q1, h1, out1 = fs._forward(x, P, hidden, fs._bias_table(bs), bs[8])
+    for l in range(8):
+        assert torch.equal(h0[l].view(torch.int16), h1[l].view(torch.int16)), l
+        assert torch.equal(q0[l].view(torch.int16), q1[l].view(torch.int16)), l
+    assert torch.equal(out0[:, :257].view(torch.int16), out1[:, :257].view(torch.int16))
+    v = fs.fused_sdf_value(bfnet, x)
+    s = out0[:, 256].float()
+    assert float((v - s).abs().max()) <= 2.0 ** -8 * float(s.abs().max()) + 1e-6
