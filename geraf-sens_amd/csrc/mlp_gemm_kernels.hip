@@ -354,7 +354,7 @@ __global__ __launch_bounds__(64 * kGemmWaves) void mlp_gemm_kernel(GemmArgs a) {
             o[e] = to_bf(z);
             o2[e] = to_bf(z * bf(eb[pb][pm][e]) * (a.beta * code_curv(ea[pb][pm][e])));
           } else if constexpr (EPI == EPI_LAYER_ADJ) {  // zbar = hbar * s + t
-            o[e] = to_bf(z * code_slope(ea[pb][pm][e]) + bf(eb[pb][pm][e]));
+            o[e] = to_bf(fmaf(z, code_slope(ea[pb][pm][e]), bf(eb[pb][pm][e])));
           } else if constexpr (EPI == EPI_MUL_SLOPE) {
             o[e] = to_bf(z * code_slope(ea[pb][pm][e]));
           } else if constexpr (EPI == EPI_GRAD_ADJ_S) {
@@ -641,6 +641,142 @@ __global__ __launch_bounds__(64 * kChainWaves) void mlp_sdf_chain_kernel(ChainAr
         const int p = pw + 16 * pb + j;
         if (g == 0 && p < a.M) a.sdf[p] = s + wsdf_l[256];
       }
+    }
+  }
+}
+
+// The reverse sweeps chained the same way: the normal pass (g_{l-1} =
+// (g_l W_l) * s_{l-1}, from g_7 = u_7 * s_7) and the primal adjoint (zbar_{l-1}
+// = (zbar_l W_l) * s_{l-1} + t_{l-1}) over `steps` layers of 256 x 256 (W_l^T
+// packed, 16 blocks x 8 steps each, back to back in the order of the
+// sweep).  Each step's output is stored (the weight gradients and the later
+// passes read it) and kept in registers as the next step's operand, so a
+// step reads the codes (and t) and writes its output: no operand re-read.
+// Step s reads e0 + s * se0 (and e1 + s * se1) and writes y + s * sy; y may
+// be e1 (in place: each lane reads its elements before it writes them).
+// Products, step order and epilogues are the layer product's (epilogues 6
+// and 5), so the outputs are bit for bit those of the layer-by-layer form.
+struct ChainAdjArgs {
+  const uint16_t* x;   // the first operand [M, ldx] (ldx 0: one broadcast row)
+  const uint16_t* x2;  // its slope codes (prologue x * s(x2)) or null
+  int64_t ldx, ldx2;
+  const bf8* w;
+  const uint16_t* e0;
+  const uint16_t* e1;  // null: no addend
+  uint16_t* y;
+  int64_t lde0, se0, lde1, se1, ldy, sy;
+  int M, steps;
+};
+
+template <bool PRO, bool ADD>
+__global__ __launch_bounds__(64 * kChainWaves) void mlp_chain_adj_kernel(ChainAdjArgs a) {
+  extern __shared__ bf8 cw[];  // [2][kChainFrags] weight chunks
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int tiles = (a.M + kChainPts - 1) / kChainPts;
+  constexpr int kStepFrags = 16 * 8 * kWave;
+  bf8 stage[kChainLoads];
+  auto load_chunk = [&](int st, int ci) __attribute__((always_inline)) {
+    const bf8* src = a.w + (size_t)st * kStepFrags + kChainBlk * ci * 8 * kWave;
+#pragma unroll
+    for (int i = 0; i < kChainLoads; ++i) stage[i] = src[tid + i * kWave * kChainWaves];
+  };
+  auto store_chunk = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < kChainLoads; ++i)
+      cw[buf * kChainFrags + tid + i * kWave * kChainWaves] = stage[i];
+  };
+  int t = blockIdx.x;
+  if (t < tiles) {
+    load_chunk(0, 0);
+    store_chunk(0);
+  }
+  bf8 B[kChainPB][8], Bn[kChainPB][8];
+  for (; t < tiles; t += gridDim.x) {
+    const bool more = t + (int)gridDim.x < tiles;
+    const int pw = t * kChainPts + 32 * wave;
+    // the first operand (with the slope prologue), rows past M read row M - 1
+#pragma unroll
+    for (int pb = 0; pb < kChainPB; ++pb) {
+      const int p = min(pw + 16 * pb + j, a.M - 1);
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        u16x8 v = load8(a.x, a.ldx, p, 32 * ks + 8 * g);
+        if (PRO) {
+          const u16x8 c = load8(a.x2, a.ldx2, p, 32 * ks + 8 * g);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) v[e] = to_bf(bf(v[e]) * code_slope(c[e]));
+        }
+        B[pb][ks] = __builtin_bit_cast(bf8, v);
+      }
+    }
+    auto chunk = [&](int st, int ci) __attribute__((always_inline)) {
+      __syncthreads();  // chunk ci is in its buffer; the other buffer is free
+      const int nst = ci < 3 ? st : st + 1;
+      const int nci = ci < 3 ? ci + 1 : 0;
+      const bool has_next = nst < a.steps || more;
+      if (has_next) load_chunk(nst < a.steps ? nst : 0, nci);
+      const uint16_t* e0 = a.e0 + st * a.se0;
+      const uint16_t* e1 = ADD ? a.e1 + st * a.se1 : nullptr;
+      u16x8 ea[kChainPB][2], eb[kChainPB][2];
+#pragma unroll
+      for (int pb = 0; pb < kChainPB; ++pb) {
+        const int p = min(pw + 16 * pb + j, a.M - 1);
+#pragma unroll
+        for (int pm = 0; pm < 2; ++pm) {
+          const int f0 = 64 * ci + 32 * pm + 8 * g;
+          ea[pb][pm] = load8(e0, a.lde0, p, f0);
+          if (ADD) eb[pb][pm] = load8(e1, a.lde1, p, f0);
+        }
+      }
+      const bf8* wb = cw + (ci & 1) * kChainFrags + lane;
+      f32x4v acc[kChainPB][kChainBlk];
+#pragma unroll
+      for (int b = 0; b < kChainBlk; ++b)
+#pragma unroll
+        for (int pb = 0; pb < kChainPB; ++pb) acc[pb][b] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int b = 0; b < kChainBlk; ++b) {
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+          const bf8 w = wb[(b * 8 + ks) * kWave];
+#pragma unroll
+          for (int pb = 0; pb < kChainPB; ++pb)
+            acc[pb][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, B[pb][ks], acc[pb][b], 0, 0, 0);
+        }
+      }
+      uint16_t* y = a.y + st * a.sy;
+#pragma unroll
+      for (int pb = 0; pb < kChainPB; ++pb) {
+        const int p = pw + 16 * pb + j;
+#pragma unroll
+        for (int pm = 0; pm < 2; ++pm) {
+          const int f0 = 64 * ci + 32 * pm + 8 * g;
+          u16x8 o;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const float z = e < 4 ? acc[pb][2 * pm][e] : acc[pb][2 * pm + 1][e - 4];
+            o[e] = ADD ? to_bf(fmaf(z, code_slope(ea[pb][pm][e]), bf(eb[pb][pm][e])))
+                       : to_bf(z * code_slope(ea[pb][pm][e]));
+          }
+          Bn[pb][2 * ci + pm] = __builtin_bit_cast(bf8, o);
+          if (p < a.M) store8(y, a.ldy, p, f0, o);
+        }
+      }
+      if (has_next) store_chunk(ci & 1 ? 0 : 1);
+    };
+#pragma unroll 1
+    for (int st = 0; st < a.steps; ++st) {
+      chunk(st, 0);
+      chunk(st, 1);
+      chunk(st, 2);
+      chunk(st, 3);
+#pragma unroll
+      for (int pb = 0; pb < kChainPB; ++pb)
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) B[pb][ks] = Bn[pb][ks];
     }
   }
 }
@@ -1285,6 +1421,65 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
   const hipStream_t st = (hipStream_t)stream;
   if (prologue == PRO_MUL_SLOPE) return launch_gemm<PRO_MUL_SLOPE>(epilogue, a, lds, st);
   return launch_gemm<PRO_NONE>(epilogue, a, lds, st);
+  GERAF_TRY_END
+}
+
+extern "C" int geraf_mlp_chain_adjoint_bf16(const uint16_t* x, int64_t ldx, const uint16_t* x2,
+                                           int64_t ldx2, const void* wpacked, int32_t steps,
+                                           const uint16_t* e0, int64_t lde0, int64_t step_e0,
+                                           const uint16_t* e1, int64_t lde1, int64_t step_e1,
+                                           uint16_t* y, int64_t ldy, int64_t step_y, int64_t M,
+                                           geraf_stream_t stream) {
+  GERAF_TRY_BEGIN
+  if (M < 0 || M > INT32_MAX) throw ArgError("M out of range");
+  if (steps < 1 || steps > 64) throw ArgError("steps must be in [1, 64]");
+  if (M == 0) return GERAF_RF_OK;
+  require_ptrs({x, wpacked, e0, y}, true);
+  check_ld(ldx, 256, "x");
+  check_ld(lde0, 256, "e0");
+  check_ld(ldy, 256, "y");
+  if (ldy == 0) throw ArgError("y: leading dimension must not be 0");
+  if (x2 != nullptr) check_ld(ldx2, 256, "x2");
+  if (e1 != nullptr) check_ld(lde1, 256, "e1");
+  if (((step_e0 | step_e1 | step_y) & 7) != 0) throw ArgError("step strides must be multiples of 8");
+  const uintptr_t align = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(x2) |
+                          reinterpret_cast<uintptr_t>(wpacked) | reinterpret_cast<uintptr_t>(e0) |
+                          reinterpret_cast<uintptr_t>(e1) | reinterpret_cast<uintptr_t>(y);
+  if ((align & 15u) != 0) throw ArgError("x, x2, wpacked, e0, e1, y must be 16-byte aligned");
+  ChainAdjArgs a;
+  a.x = x;
+  a.x2 = x2;
+  a.ldx = ldx;
+  a.ldx2 = ldx2;
+  a.w = reinterpret_cast<const bf8*>(wpacked);
+  a.e0 = e0;
+  a.e1 = e1;
+  a.y = y;
+  a.lde0 = lde0;
+  a.se0 = step_e0;
+  a.lde1 = lde1;
+  a.se1 = step_e1;
+  a.ldy = ldy;
+  a.sy = step_y;
+  a.M = (int)M;
+  a.steps = steps;
+  const size_t lds = (size_t)2 * kChainFrags * sizeof(bf8);
+  const int64_t tiles = (M + kChainPts - 1) / kChainPts;
+  const dim3 grid((unsigned)std::min<int64_t>(tiles, cu_count()));
+  const dim3 block(64 * kChainWaves);
+  const hipStream_t st = (hipStream_t)stream;
+  if (x2 != nullptr && e1 != nullptr) throw ArgError("a prologue with an addend is not supported");
+  if (x2 != nullptr) {
+    allow_lds(mlp_chain_adj_kernel<true, false>, lds);
+    hipLaunchKernelGGL((mlp_chain_adj_kernel<true, false>), grid, block, lds, st, a);
+  } else if (e1 != nullptr) {
+    allow_lds(mlp_chain_adj_kernel<false, true>, lds);
+    hipLaunchKernelGGL((mlp_chain_adj_kernel<false, true>), grid, block, lds, st, a);
+  } else {
+    allow_lds(mlp_chain_adj_kernel<false, false>, lds);
+    hipLaunchKernelGGL((mlp_chain_adj_kernel<false, false>), grid, block, lds, st, a);
+  }
+  return check_launch("mlp_chain_adj_kernel");
   GERAF_TRY_END
 }
 

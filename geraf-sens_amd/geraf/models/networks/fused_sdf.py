@@ -155,16 +155,42 @@ def _hidden_pack(ws):
     return buf, views
 
 
+def _reverse_pack(ws):
+    """W_7^T .. W_1^T packed back to back (the chained reverse sweeps'
+    operand; W_3^T's 217 inputs padded to 256 with zero weights, which adds
+    exact zeros) and the slice of each."""
+    n = int(_native.load().geraf_mlp_pack_bytes(256, 256))
+    buf = torch.empty(7 * n, dtype=torch.uint8, device=ws[0].device)
+    views = [None] * (NL - 1)
+    for s, l in enumerate(range(NL - 2, 0, -1)):
+        w = ws[l].detach().float()
+        if OUT[l] < 256:
+            w = torch.cat([w, w.new_zeros((256 - OUT[l], IN[l]))])
+        views[l] = pack(w, True, 1.0 / SQRT2 if l == SKIP else 1.0, out=buf[s * n:(s + 1) * n])
+    return buf, views
+
+
 def _packs(ws):
     """P_l packs W_l ([out, in]; the forward and the normal-pass adjoint),
     Q_l packs W_l^T ([in, out]; the normal pass and the primal adjoint);
     layer 4 carries the skip's 1/sqrt 2, layer 8 its rows in _out_rows
-    order.  Also returns the buffer holding P_0 .. P_7."""
+    order.  Also returns the buffers holding P_0 .. P_7 and Q_7 .. Q_1."""
     hidden, P = _hidden_pack(ws)
     P.append(pack(_out_rows(ws[NL - 1].detach()), False))
-    Q = [None] + [pack(ws[l], True, 1.0 / SQRT2 if l == SKIP else 1.0) for l in range(1, NL - 1)]
+    rev, Q = _reverse_pack(ws)
     Q.append(pack(_out_rows(ws[NL - 1].detach()), True))
-    return P, Q, hidden
+    return P, Q, hidden, rev
+
+
+def _sweep(x, rev, e0, e1, y, x2=None) -> None:
+    """The chained reverse sweep over layers 7 .. 1: e0, e1 (or None) and y
+    are [7, M, 256] tensors indexed by the output's layer (step s writes
+    layer 6 - s); x (and the prologue codes x2) the first operand."""
+    M = y.shape[1]
+    step = -M * 256
+    _native.call("geraf_mlp_chain_adjoint_bf16", _ptr(x), _ld(x), _ptr(x2), _ld(x2), _ptr(rev),
+                 NL - 2, _ptr(e0[6]), 256, step, _ptr(None if e1 is None else e1[6]), 256, step,
+                 _ptr(y[6]), 256, step, M, _native.stream_of(y))
 
 
 def _biases(params) -> List[torch.Tensor]:
@@ -197,8 +223,9 @@ def _chain(x, hidden, table, w_sdf=None, sdf=None, h=None, q=None) -> None:
 
 def _forward(x, P, hidden, table, b8):
     """The training forward: the hidden layers chained in one kernel (h_l and
-    the slope codes q_l stored), then the output layer.  Returns (gam,
-    slope-code list, h list, out8 [M, 264] = [features | sdf | padding])."""
+    the slope codes q_l stored), then the output layer.  Returns (gam, the
+    slope codes [8, M, 256], h list, out8 [M, 264] = [features | sdf |
+    padding])."""
     M = x.shape[0]
     dev = x.device
     gam = torch.empty((M, 64), dtype=BF, device=dev)
@@ -207,10 +234,9 @@ def _forward(x, P, hidden, table, b8):
     h = torch.empty((NL - 1, M, 256), dtype=BF, device=dev)
     q = torch.empty((NL - 1, M, 256), dtype=CODE, device=dev)
     _chain(x, hidden, table, h=h, q=q)
-    hs, qs = list(h.unbind(0)), list(q.unbind(0))
     out8 = torch.empty((M, 264), dtype=BF, device=dev)
-    gemm(hs[7], P[8], OUT[8], IN[8], out8, bias=b8)
-    return gam, qs, hs, out8
+    gemm(h[7], P[8], OUT[8], IN[8], out8, bias=b8)
+    return gam, q, list(h.unbind(0)), out8
 
 
 def _forward_layers(x, ws, bs, P, train: bool):
@@ -246,25 +272,19 @@ def _forward_layers(x, ws, bs, P, train: bool):
     return gam, qs, hs, out8
 
 
-def _normal_pass(x, ws, qs, Q):
-    """g_l = u_l s_l for l = 0..6 (g_3's buffer also carries the skip's
-    d sdf / d gamma in columns 217..255, where q_3 holds slope 1), u_7 =
-    W_8[0] (one broadcast row, not multiplied), and the normals."""
+def _normal_pass(x, ws, q, rev):
+    """g_l = u_l s_l for l = 0..6 as one [7, M, 256] tensor (g_3 also carries
+    the skip's d sdf / d gamma in columns 217..255, where q_3 holds slope 1),
+    from the chained sweep over layers 7 .. 1; u_7 = W_8[0] (one broadcast
+    row, not multiplied); the normals."""
     M = x.shape[0]
     dev = x.device
     u7 = torch.zeros((1, 256), dtype=BF, device=dev)
     u7[0] = ws[8].detach()[0].to(BF)
     u7 = u7.expand(M, 256)
-    us = [None] * 8
-    us[7] = u7
-    for l in range(7, 0, -1):
-        u = torch.empty((M, 256), dtype=BF, device=dev)
-        if l == 7:
-            gemm(u7, Q[l], IN[l], OUT[l], u, x2=qs[l], e0=qs[l - 1], prologue=_PRO_SLOPE,
-                 epilogue=_EPI_MUL_SLOPE)
-        else:
-            gemm(us[l], Q[l], IN[l], OUT[l], u, e0=qs[l - 1], epilogue=_EPI_MUL_SLOPE)
-        us[l - 1] = u
+    g = torch.empty((NL - 2, M, 256), dtype=BF, device=dev)
+    _sweep(u7, rev, q, None, g, x2=q[7])
+    us = list(g.unbind(0)) + [u7]
     g0 = torch.empty((M, 40), dtype=BF, device=dev)
     # layer 0's product with W_0^T: the packed W_0 read as [in, out]
     gemm(us[0], pack(ws[0], True), IN[0], OUT[0], g0)
@@ -272,7 +292,7 @@ def _normal_pass(x, ws, qs, Q):
     skip = us[3][:, 217:]
     _native.call("geraf_mlp_pe_gradient", _ptr(x), M, L_BANDS, 1.0, _ptr(g0), 40, _ptr(skip), 256,
                  _ptr(n), _native.stream_of(x))
-    return us, n
+    return g, us, n
 
 
 class _FusedSDF(Function):
@@ -281,16 +301,17 @@ class _FusedSDF(Function):
         ws, bs = list(params[0::2]), _biases(params[1::2])
         _check_network(ws)
         x = x.detach().float().contiguous()
-        P, Q, hidden = _packs(ws)
-        gam, qs, hs, out8 = _forward(x, P, hidden, _bias_table(bs), bs[NL - 1])
+        P, Q, hidden, rev = _packs(ws)
+        gam, q, hs, out8 = _forward(x, P, hidden, _bias_table(bs), bs[NL - 1])
         sdf = out8[:, 256].float()
         feat = out8[:, :256]
         if with_normals:
-            us, n = _normal_pass(x, ws, qs, Q)
+            g, us, n = _normal_pass(x, ws, q, rev)
         else:
-            us, n = None, torch.zeros((x.shape[0], 3), dtype=torch.float32, device=x.device)
+            g, us = None, None
+            n = torch.zeros((x.shape[0], 3), dtype=torch.float32, device=x.device)
         ctx.save_for_backward(*params)
-        ctx.state = (x, gam, qs, hs, us, P, Q, with_normals)
+        ctx.state = (x, gam, q, hs, g, us, P, Q, rev, with_normals)
         ctx.mark_non_differentiable(*([] if with_normals else [n]))
         return sdf, feat, n
 
@@ -299,8 +320,9 @@ class _FusedSDF(Function):
     def backward(ctx, d_sdf, d_feat, d_n):
         params = ctx.saved_tensors
         ws = list(params[0::2])
-        x, gam, qs, hs, us, P, Q, with_normals = ctx.state
+        x, gam, q, hs, g, us, P, Q, rev, with_normals = ctx.state
         ctx.state = None
+        qs = list(q.unbind(0))
         M = x.shape[0]
         dev = x.device
         dws = [torch.zeros((OUT[l], IN[l]), dtype=torch.float32, device=dev) for l in range(NL)]
@@ -368,14 +390,21 @@ class _FusedSDF(Function):
         dws[8][0] += dw_s[0]
         dbs[8][1:] += db_f
         dbs[8][0] += db_s[0]
-        for l in range(8, 0, -1):
-            prev = l - 1
-            zb_p = torch.empty((M, 256), dtype=BF, device=dev)
-            t = ts[prev] if normal_terms else zero_row
-            gemm(zb, Q[l], IN[l], OUT[l], zb_p, e0=qs[prev], e1=t, epilogue=_EPI_LAYER_ADJ)
+        # zbar_7 = (zbar_8 W_8) * s_7 + t_7 over t_7, then the chained sweep
+        # over layers 7 .. 1: zbar_l over t_l (which holds g_l's buffer), or
+        # into a new tensor without the normal terms
+        if normal_terms:
+            zb7, t7, tail, t_tail = ts[7], ts[7], g, g
+        else:
+            zb7, t7 = torch.empty((M, 256), dtype=BF, device=dev), zero_row
+            tail, t_tail = torch.empty((NL - 2, M, 256), dtype=BF, device=dev), None
+        gemm(zb, Q[8], IN[8], OUT[8], zb7, e0=qs[7], e1=t7, epilogue=_EPI_LAYER_ADJ)
+        _sweep(zb7, rev, q, t_tail, tail)
+        zbs = list(tail.unbind(0)) + [zb7]
+        for prev in range(NL - 2, -1, -1):
             a_prev = hs[prev - 1] if prev >= 1 else gam  # layer 4: [h_3, gamma]
-            wgrad(zb_p, a_prev, OUT[prev], IN[prev], dws[prev], db=dbs[prev], accumulate=True)
-            zb = zb_p
+            wgrad(zbs[prev], a_prev, OUT[prev], IN[prev], dws[prev], db=dbs[prev],
+                  accumulate=True)
         dws[SKIP] *= 1.0 / SQRT2
         grads = []
         for l in range(NL):

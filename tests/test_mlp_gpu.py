@@ -285,7 +285,7 @@ def test_chained_forward_matches_layers(cuda, M):
     params = [t for l in range(9)
               for t in (getattr(bfnet, f"lin{l}").weight, getattr(bfnet, f"lin{l}").bias)]
     ws, bs = list(params[0::2]), fs._biases(params[1::2])
-    P, _, hidden = fs._packs(ws)
+    P, _, hidden, _ = fs._packs(ws)
     _, q0, h0, out0 = fs._forward_layers(x, ws, bs, P, train=True)
     _, q1, h1, out1 = fs._forward(x, P, hidden, fs._bias_table(bs), bs[8])
     for l in range(8):
@@ -295,3 +295,40 @@ def test_chained_forward_matches_layers(cuda, M):
     v = fs.fused_sdf_value(bfnet, x)
     s = out0[:, 256].float()
     assert float((v - s).abs().max()) <= 2.0 ** -8 * float(s.abs().max()) + 1e-6
+
+
+@pytest.mark.parametrize("M", [1, 300, 70001])
+@pytest.mark.parametrize("mode", ["normal", "primal"])
+def test_chained_sweep_matches_layers(cuda, M, mode):
+    """The chained reverse sweep over layers 7 .. 1 (W_l^T, W_3^T padded with
+    zero weights) against seven layer products, bit for bit: the normal pass
+    (broadcast first row with the slope prologue, y = z s(q)) and the primal
+    adjoint (y = z s(q) + t, also written in place over t)."""
+    from geraf.models.networks import fused_sdf as fs
+
+    _, bfnet = _nets(cuda, seed=7)
+    ws = [getattr(bfnet, f"lin{l}").weight for l in range(9)]
+    rev, Q = fs._reverse_pack(ws)
+    g = torch.Generator(device="cpu").manual_seed(M + len(mode))
+    q = _code(torch.randn(8, M, 256, generator=g) * 0.02).to(cuda)
+    if mode == "normal":
+        x = _bf(torch.randn(1, 256, generator=g)).to(cuda).expand(M, 256)
+        x2, t = q[7], None
+    else:
+        x = _bf(torch.randn(M, 256, generator=g)).to(cuda)
+        x2, t = None, _bf(torch.randn(7, M, 256, generator=g) * 0.1).to(cuda)
+    y = torch.empty((7, M, 256), dtype=torch.bfloat16, device=cuda)
+    fs._sweep(x, rev, q, t, y, x2=x2)
+    cur = x
+    for l in range(7, 0, -1):
+        out = torch.empty((M, 256), dtype=torch.bfloat16, device=cuda)
+        if mode == "normal":
+            fs.gemm(cur, Q[l], 256, 256, out, x2=x2 if l == 7 else None, e0=q[l - 1],
+                    prologue=1 if l == 7 else 0, epilogue=6)
+        else:
+            fs.gemm(cur, Q[l], 256, 256, out, e0=q[l - 1], e1=t[l - 1], epilogue=5)
+        assert torch.equal(out.view(torch.int16), y[l - 1].view(torch.int16)), l
+        cur = out
+    if t is not None:
+        fs._sweep(x, rev, q, t, t)
+        assert torch.equal(t.view(torch.int16), y.view(torch.int16))
