@@ -549,34 +549,42 @@ def bench_sdf_network(dev, M: int = 8388608, reps: int = 3):
     log(f"SDF network pass on {M} points: hand-written {out['fused_ms']:.1f} ms, torch BF16 "
         f"{out['torch_bf16_ms']:.1f} ms")
 
-    # the kernel forms, per launch, [M,256] operands
+    # the kernel forms the pass runs, per launch, on [M,256] operands
+    params = [t for l in range(9)
+              for t in (getattr(net, f"lin{l}").weight, getattr(net, f"lin{l}").bias)]
+    ws = [w.detach() for w in params[0::2]]
+    hidden, _ = fs._hidden_pack(ws)
+    rev, _ = fs._reverse_pack(ws)
+    table = fs._bias_table([b.detach().float() for b in params[1::2]])
+    w_sdf = fs._sdf_row(params)
     xa = (torch.randn(M, 256, device=dev) * 0.5).to(BF)
     qa = (torch.randn(M, 256, device=dev) * 0.3).to(HF)
     ea = torch.randn(M, 256, device=dev).to(BF)
     ya = torch.empty(M, 256, dtype=BF, device=dev)
     y2 = torch.empty(M, 256, dtype=BF, device=dev)
-    yq = torch.empty(M, 256, dtype=HF, device=dev)
+    hh = torch.empty(8, M, 256, dtype=BF, device=dev)
+    hq = torch.empty(8, M, 256, dtype=HF, device=dev)
+    sdf = torch.empty(M, device=dev)
     W = fs.pack(torch.randn(256, 256, device=dev) / 16, False)
-    b = torch.randn(256, device=dev) * 0.01
     dw = torch.zeros(256, 256, device=dev)
     db = torch.zeros(256, device=dev)
     tensor_gb = M * 256 * 2 / 1e9
+
+    def primal():
+        fs._sweep(xa, rev, hq, hh[:7], hh[:7])
+
     forms = [
         ("device copy (torch)", 2, lambda: ya.copy_(xa)),
-        ("layer product + bias", 2, lambda: fs.gemm(xa, W, 256, 256, ya, bias=b)),
-        ("layer product + softplus", 2, lambda: fs.gemm(xa, W, 256, 256, ya, bias=b, epilogue=1)),
-        ("layer product + softplus + slope code", 3,
-         lambda: fs.gemm(xa, W, 256, 256, ya, bias=b, y2=yq, epilogue=2)),
-        ("slope prologue + layer product (normals)", 3,
-         lambda: fs.gemm(xa, W, 256, 256, ya, x2=qa, prologue=1)),
-        ("slope prologue + product + second-order term", 6,
-         lambda: fs.gemm(xa, W, 256, 256, ya, x2=qa, y2=y2, e0=qa, e1=ea, prologue=1,
-                         epilogue=4)),
+        ("chained forward, training (h_l and q_l of 8 layers written)", 16,
+         lambda: fs._chain(x, hidden, table, h=hh, q=hq)),
+        ("chained normal-pass sweep, 7 layers (q_l read, u_l s_l written)", 15,
+         lambda: fs._sweep(xa[:1].expand(M, 256), rev, hq, None, hh[:7], x2=hq[7])),
+        ("chained primal-adjoint sweep, 7 layers (q_l, t_l read, zbar_l written)", 22, primal),
+        ("product + second-order term (vbar s, u s, q in; vbar s, t out)", 5,
+         lambda: fs.gemm(xa, W, 256, 256, ya, y2=y2, e0=qa, e1=ea, epilogue=7)),
         ("product + hidden-layer adjoint", 4,
          lambda: fs.gemm(xa, W, 256, 256, ya, e0=qa, e1=ea, epilogue=5)),
         ("weight gradient", 2, lambda: fs.wgrad(xa, ea, 256, 256, dw, db=db)),
-        ("weight gradient, slope prologues", 4, lambda: fs.wgrad(xa, ea, 256, 256, dw, g2=qa,
-                                                                  x2=qa)),
     ]
     kernels = []
     for name, tensors, fn in forms:
@@ -588,8 +596,15 @@ def bench_sdf_network(dev, M: int = 8388608, reps: int = 3):
     copy_gbs = kernels[0]["achieved_gbs"]
     for k in kernels[1:]:
         k["vs_device_copy"] = k["achieved_gbs"] / copy_gbs
+    # the value-only chain moves 16 B per point: bound by its matrix and
+    # vector work (0.918 MFLOP of products per point)
+    ms = timed(lambda: fs._chain(x, hidden, table, w_sdf=w_sdf, sdf=sdf), 5)
+    flops = 2.0 * (256 * 39 + 6 * 256 * 256 + 217 * 256 + 256) * M
+    out["value_chain"] = {"form": "chained forward, sdf only", "ms": ms,
+                          "tflops": flops / ms / 1e9, "mfma_frac": flops / ms / 1e9 / 2500.0}
+    log(f"  chained forward, sdf only: {ms:.3f} ms, {flops / ms / 1e9:.0f} TFLOP/s of products")
     out["kernels"] = kernels
-    del xa, qa, ea, ya, y2, yq
+    del xa, qa, ea, ya, y2, hh, hq
     torch.cuda.empty_cache()
     return out
 
