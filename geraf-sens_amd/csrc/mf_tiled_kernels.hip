@@ -252,6 +252,37 @@ __global__ void mf_tile_finish_kernel(const double* __restrict__ acc_re,
 
 // ------------------------------------------------------------------ adjoint
 
+// Wave totals of moment pair q (moments 2q, 2q + 1, re and im) by recursive
+// halving: a permlane32 swap joins (Pr_m, Pi_m) in one register (lanes < 32
+// a partial of Pr_m over lanes i, i + 32; lanes >= 32 of Pi_m), a permlane16
+// swap joins two of those (row 0: Pr_2q, row 1: Pr_2q+1, row 2: Pi_2q, row
+// 3: Pi_2q+1, each over 4 lanes), and four DPP steps sum each row: about 10
+// instructions for the 4 totals, where a full-wave reduction per value
+// takes about 11.  Every lane of row rho ends with its row's total.
+__device__ __forceinline__ float wave_moment_pair_total(float pr0, float pi0, float pr1,
+                                                        float pi1) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(pr0), __float_as_uint(pi0),
+                                                  false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(pr1), __float_as_uint(pi1),
+                                                  false, false);
+  const float h0 = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const float h1 = __uint_as_float(b[0]) + __uint_as_float(b[1]);
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(h0), __float_as_uint(h1),
+                                                  false, false);
+  float v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  v += dpp_mov<dpp::kQuadXor1>(0.0f, v);
+  v += dpp_mov<dpp::kQuadXor2>(0.0f, v);
+  v += dpp_mov<dpp::kRowHalfMirror>(0.0f, v);
+  v += dpp_mov<dpp::kRowMirror>(0.0f, v);
+  return v;
+}
+
+constexpr float inv_factorial(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return (float)(1.0 / f);
+}
+
 // One wave per tile of kWave * AVPL voxels (AVPL voxels per lane).
 template <int NT, int AVPL>
 __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
@@ -314,22 +345,20 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
       pairs(std::true_type{});
     else
       pairs(std::false_type{});
-    // M'_n = j^n / n! * sum over the tile
-    float2* __restrict__ out = mom + ((size_t)ai * ntiles + t) * NT;
-    float inv_fact = 1.0f;
+    // M'_n = j^n / n! * sum over the tile.  The first lane of row rho holds
+    // the total of component rho >> 1 (re, im) of moment 2 q + (rho & 1); as
+    // j^k, k = n (+1 for im), it goes to component k & 1, negated for k & 2.
+    static_assert(NT % 2 == 0, "moment count must be even");
+    const int rho = lane >> 4;
+    float* __restrict__ out = reinterpret_cast<float*>(mom + ((size_t)ai * ntiles + t) * NT);
 #pragma unroll
-    for (int n = 0; n < NT; ++n) {
-      const float mr = wave_total_dpp(Pr[n]) * inv_fact;
-      const float mi = wave_total_dpp(Pi[n]) * inv_fact;
-      float2 c;
-      switch (n & 3) {
-        case 0: c = make_float2(mr, mi); break;
-        case 1: c = make_float2(-mi, mr); break;
-        case 2: c = make_float2(-mr, -mi); break;
-        default: c = make_float2(mi, -mr); break;
-      }
-      if (lane == 0) out[n] = c;
-      inv_fact /= (float)(n + 1);
+    for (int q = 0; q < NT / 2; ++q) {
+      const float tot = wave_moment_pair_total(Pr[2 * q], Pi[2 * q], Pr[2 * q + 1], Pi[2 * q + 1]);
+      const int n = 2 * q + (rho & 1);
+      const int k = n + (rho >> 1);
+      const float f = (rho & 1) ? inv_factorial(2 * q + 1) : inv_factorial(2 * q);
+      const float val = (k & 2) ? -tot * f : tot * f;
+      if ((lane & 15) == 0) out[2 * n + (k & 1)] = val;
     }
     if (lane == 0) tau_c[(size_t)ai * ntiles + t] = tac;
   }
