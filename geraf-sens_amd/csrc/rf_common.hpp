@@ -44,12 +44,13 @@ __device__ __forceinline__ double dist3_f64(double ax, double ay, double az, dou
 
 // sqrt(D) for D >= 0 to ~1e-14 relative: an FP32 reciprocal-square-root seed
 // (about 1 ulp, 6e-8) refined by one FP64 Newton step (error ~1.5 * 6e-8^2).
-// Far cheaper than a correctly rounded FP64 sqrt; returns 0 for D == 0.
+// Far cheaper than a correctly rounded FP64 sqrt.  The seed's argument is
+// held at or above FLT_MIN, so D == 0 gives a finite seed and the result 0
+// without a select; D below FLT_MIN (distances under 1e-19) is not resolved.
 __device__ __forceinline__ double sqrt_f64_newton(double D) {
-  const double y = (double)__builtin_amdgcn_rsqf((float)D);
+  const double y = (double)__builtin_amdgcn_rsqf(fmaxf((float)D, 1.17549435e-38f));
   const double h = 0.5 * D * (y * y);
-  const double r = D * fma(y, 0.5 - h, y);  // D * y * (1.5 - h)
-  return D > 0.0 ? r : 0.0;
+  return D * fma(y, 0.5 - h, y);  // D * y * (1.5 - h)
 }
 
 __device__ __forceinline__ double dist3_f64_fast(double ax, double ay, double az, double bx,
