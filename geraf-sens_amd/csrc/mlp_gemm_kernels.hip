@@ -656,19 +656,31 @@ __global__ __launch_bounds__(64 * kChainWaves) void mlp_sdf_chain_kernel(ChainAr
 // be e1 (in place: each lane reads its elements before it writes them).
 // Products, step order and epilogues are the layer product's (epilogues 6
 // and 5), so the outputs are bit for bit those of the layer-by-layer form.
+// The same sweep serves ReLU networks (RELU: the slopes are the masks
+// a > 0 of BF16 activations instead of softplus codes), with the first
+// operand optionally scaled per point (PRO 2: x * s(x2) * xs[p]) and stored
+// (x_out), and steps from mask_steps on unmasked (y = z [+ e1]).
 struct ChainAdjArgs {
   const uint16_t* x;   // the first operand [M, ldx] (ldx 0: one broadcast row)
-  const uint16_t* x2;  // its slope codes (prologue x * s(x2)) or null
-  int64_t ldx, ldx2;
+  const uint16_t* x2;  // its slopes (prologue x * s(x2)) or null
+  const float* xs;     // per-point scale of the prologue (PRO 2)
+  uint16_t* x_out;     // the first operand as multiplied, or null
+  int64_t ldx, ldx2, ldxo;
   const bf8* w;
   const uint16_t* e0;
   const uint16_t* e1;  // null: no addend
   uint16_t* y;
   int64_t lde0, se0, lde1, se1, ldy, sy;
-  int M, steps;
+  int M, steps, mask_steps;
 };
 
-template <bool PRO, bool ADD>
+template <bool RELU>
+__device__ __forceinline__ float sweep_slope(uint16_t m) {
+  if constexpr (RELU) return bf(m) > 0.0f ? 1.0f : 0.0f;
+  return code_slope(m);
+}
+
+template <int PRO, bool ADD, bool RELU>
 __global__ __launch_bounds__(64 * kChainWaves) void mlp_chain_adj_kernel(ChainAdjArgs a) {
   extern __shared__ bf8 cw[];  // [2][kChainFrags] weight chunks
   const int tid = threadIdx.x;
@@ -701,14 +713,19 @@ __global__ __launch_bounds__(64 * kChainWaves) void mlp_chain_adj_kernel(ChainAd
 #pragma unroll
     for (int pb = 0; pb < kChainPB; ++pb) {
       const int p = min(pw + 16 * pb + j, a.M - 1);
+      const float sc = PRO == 2 ? a.xs[p] : 1.0f;
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks) {
         u16x8 v = load8(a.x, a.ldx, p, 32 * ks + 8 * g);
-        if (PRO) {
+        if (PRO > 0) {
           const u16x8 c = load8(a.x2, a.ldx2, p, 32 * ks + 8 * g);
 #pragma unroll
-          for (int e = 0; e < 8; ++e) v[e] = to_bf(bf(v[e]) * code_slope(c[e]));
+          for (int e = 0; e < 8; ++e)
+            v[e] = PRO == 2 ? to_bf(bf(v[e]) * sc * sweep_slope<RELU>(c[e]))
+                            : to_bf(bf(v[e]) * sweep_slope<RELU>(c[e]));
         }
+        if (a.x_out != nullptr && pw + 16 * pb + j < a.M)
+          store8(a.x_out, a.ldxo, p, 32 * ks + 8 * g, v);
         B[pb][ks] = __builtin_bit_cast(bf8, v);
       }
     }
@@ -718,7 +735,8 @@ __global__ __launch_bounds__(64 * kChainWaves) void mlp_chain_adj_kernel(ChainAd
       const int nci = ci < 3 ? ci + 1 : 0;
       const bool has_next = nst < a.steps || more;
       if (has_next) load_chunk(nst < a.steps ? nst : 0, nci);
-      const uint16_t* e0 = a.e0 + st * a.se0;
+      const bool masked = st < a.mask_steps;  // uniform
+      const uint16_t* e0 = masked ? a.e0 + st * a.se0 : nullptr;
       const uint16_t* e1 = ADD ? a.e1 + st * a.se1 : nullptr;
       u16x8 ea[kChainPB][2], eb[kChainPB][2];
 #pragma unroll
@@ -727,7 +745,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void mlp_chain_adj_kernel(ChainAd
 #pragma unroll
         for (int pm = 0; pm < 2; ++pm) {
           const int f0 = 64 * ci + 32 * pm + 8 * g;
-          ea[pb][pm] = load8(e0, a.lde0, p, f0);
+          ea[pb][pm] = masked ? load8(e0, a.lde0, p, f0) : zero8();
           if (ADD) eb[pb][pm] = load8(e1, a.lde1, p, f0);
         }
       }
@@ -758,8 +776,8 @@ __global__ __launch_bounds__(64 * kChainWaves) void mlp_chain_adj_kernel(ChainAd
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
             const float z = e < 4 ? acc[pb][2 * pm][e] : acc[pb][2 * pm + 1][e - 4];
-            o[e] = ADD ? to_bf(fmaf(z, code_slope(ea[pb][pm][e]), bf(eb[pb][pm][e])))
-                       : to_bf(z * code_slope(ea[pb][pm][e]));
+            const float sl = masked ? sweep_slope<RELU>(ea[pb][pm][e]) : 1.0f;
+            o[e] = ADD ? to_bf(fmaf(z, sl, bf(eb[pb][pm][e]))) : to_bf(z * sl);
           }
           Bn[pb][2 * ci + pm] = __builtin_bit_cast(bf8, o);
           if (p < a.M) store8(y, a.ldy, p, f0, o);
@@ -1425,32 +1443,51 @@ extern "C" int geraf_mlp_gemm_bf16(const uint16_t* x, int64_t ldx, const uint16_
 }
 
 extern "C" int geraf_mlp_chain_adjoint_bf16(const uint16_t* x, int64_t ldx, const uint16_t* x2,
-                                           int64_t ldx2, const void* wpacked, int32_t steps,
-                                           const uint16_t* e0, int64_t lde0, int64_t step_e0,
-                                           const uint16_t* e1, int64_t lde1, int64_t step_e1,
-                                           uint16_t* y, int64_t ldy, int64_t step_y, int64_t M,
-                                           geraf_stream_t stream) {
+                                           int64_t ldx2, const float* x_scale, int32_t mask_kind,
+                                           const void* wpacked, int32_t steps,
+                                           int32_t mask_steps, const uint16_t* e0, int64_t lde0,
+                                           int64_t step_e0, const uint16_t* e1, int64_t lde1,
+                                           int64_t step_e1, uint16_t* y, int64_t ldy,
+                                           int64_t step_y, uint16_t* x_out, int64_t ldx_out,
+                                           int64_t M, geraf_stream_t stream) {
   GERAF_TRY_BEGIN
   if (M < 0 || M > INT32_MAX) throw ArgError("M out of range");
   if (steps < 1 || steps > 64) throw ArgError("steps must be in [1, 64]");
+  if (mask_steps < 0 || mask_steps > steps) throw ArgError("mask_steps must be in [0, steps]");
+  if (mask_kind != 0 && mask_kind != 1) throw ArgError("mask_kind must be 0 (codes) or 1 (relu)");
+  if (x_scale != nullptr && x2 == nullptr) throw ArgError("x_scale needs x2");
+  if (x2 != nullptr && e1 != nullptr) throw ArgError("a prologue with an addend is not supported");
+  if (mask_kind == 1 && e1 != nullptr) throw ArgError("relu masks with an addend are not supported");
+  if (mask_kind == 1 && x2 == nullptr) throw ArgError("relu masks need the prologue (x2)");
   if (M == 0) return GERAF_RF_OK;
-  require_ptrs({x, wpacked, e0, y}, true);
+  require_ptrs({x, wpacked, y}, true);
+  if (mask_steps > 0) {
+    require_ptrs({e0}, true);
+    check_ld(lde0, 256, "e0");
+  }
   check_ld(ldx, 256, "x");
-  check_ld(lde0, 256, "e0");
   check_ld(ldy, 256, "y");
   if (ldy == 0) throw ArgError("y: leading dimension must not be 0");
   if (x2 != nullptr) check_ld(ldx2, 256, "x2");
   if (e1 != nullptr) check_ld(lde1, 256, "e1");
+  if (x_out != nullptr) {
+    check_ld(ldx_out, 256, "x_out");
+    if (ldx_out == 0) throw ArgError("x_out: leading dimension must not be 0");
+  }
   if (((step_e0 | step_e1 | step_y) & 7) != 0) throw ArgError("step strides must be multiples of 8");
   const uintptr_t align = reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(x2) |
                           reinterpret_cast<uintptr_t>(wpacked) | reinterpret_cast<uintptr_t>(e0) |
-                          reinterpret_cast<uintptr_t>(e1) | reinterpret_cast<uintptr_t>(y);
-  if ((align & 15u) != 0) throw ArgError("x, x2, wpacked, e0, e1, y must be 16-byte aligned");
+                          reinterpret_cast<uintptr_t>(e1) | reinterpret_cast<uintptr_t>(y) |
+                          reinterpret_cast<uintptr_t>(x_out);
+  if ((align & 15u) != 0) throw ArgError("x, x2, wpacked, e0, e1, y, x_out must be 16-byte aligned");
   ChainAdjArgs a;
   a.x = x;
   a.x2 = x2;
+  a.xs = x_scale;
+  a.x_out = x_out;
   a.ldx = ldx;
   a.ldx2 = ldx2;
+  a.ldxo = ldx_out;
   a.w = reinterpret_cast<const bf8*>(wpacked);
   a.e0 = e0;
   a.e1 = e1;
@@ -1463,22 +1500,33 @@ extern "C" int geraf_mlp_chain_adjoint_bf16(const uint16_t* x, int64_t ldx, cons
   a.sy = step_y;
   a.M = (int)M;
   a.steps = steps;
+  a.mask_steps = mask_steps;
   const size_t lds = (size_t)2 * kChainFrags * sizeof(bf8);
   const int64_t tiles = (M + kChainPts - 1) / kChainPts;
   const dim3 grid((unsigned)std::min<int64_t>(tiles, cu_count()));
   const dim3 block(64 * kChainWaves);
   const hipStream_t st = (hipStream_t)stream;
-  if (x2 != nullptr && e1 != nullptr) throw ArgError("a prologue with an addend is not supported");
-  if (x2 != nullptr) {
-    allow_lds(mlp_chain_adj_kernel<true, false>, lds);
-    hipLaunchKernelGGL((mlp_chain_adj_kernel<true, false>), grid, block, lds, st, a);
-  } else if (e1 != nullptr) {
-    allow_lds(mlp_chain_adj_kernel<false, true>, lds);
-    hipLaunchKernelGGL((mlp_chain_adj_kernel<false, true>), grid, block, lds, st, a);
-  } else {
-    allow_lds(mlp_chain_adj_kernel<false, false>, lds);
-    hipLaunchKernelGGL((mlp_chain_adj_kernel<false, false>), grid, block, lds, st, a);
+#define GERAF_SWEEP(P, A, R)                                                   \
+  {                                                                            \
+    allow_lds(mlp_chain_adj_kernel<P, A, R>, lds);                             \
+    hipLaunchKernelGGL((mlp_chain_adj_kernel<P, A, R>), grid, block, lds, st, a); \
   }
+  if (mask_kind == 1) {
+    if (x_scale != nullptr)
+      GERAF_SWEEP(2, false, true)
+    else
+      GERAF_SWEEP(1, false, true)
+  } else if (x2 != nullptr) {
+    if (x_scale != nullptr)
+      GERAF_SWEEP(2, false, false)
+    else
+      GERAF_SWEEP(1, false, false)
+  } else if (e1 != nullptr) {
+    GERAF_SWEEP(0, true, false)
+  } else {
+    GERAF_SWEEP(0, false, false)
+  }
+#undef GERAF_SWEEP
   return check_launch("mlp_chain_adj_kernel");
   GERAF_TRY_END
 }

@@ -182,15 +182,28 @@ def _packs(ws):
     return P, Q, hidden, rev
 
 
+def sweep(x, wpacked, steps, y, y_step, e0=None, e0_step=0, e1=None, e1_step=0, x2=None,
+          x_scale=None, relu=False, mask_steps=None, x_out=None) -> None:
+    """Chained layer products (geraf_mlp_chain_adjoint_bf16): y, e0 and e1
+    are the first step's [M, 256] views, each step moving by *_step
+    elements; x (with the prologue slopes x2, optionally scaled per point by
+    x_scale) the first operand."""
+    M = y.shape[0]
+    _native.call("geraf_mlp_chain_adjoint_bf16", _ptr(x), _ld(x), _ptr(x2), _ld(x2),
+                 _ptr(x_scale), int(relu), _ptr(wpacked), steps,
+                 steps if mask_steps is None else mask_steps, _ptr(e0), _ld(e0), e0_step,
+                 _ptr(e1), _ld(e1), e1_step, _ptr(y), _ld(y), y_step, _ptr(x_out), _ld(x_out),
+                 M, _native.stream_of(y))
+
+
 def _sweep(x, rev, e0, e1, y, x2=None) -> None:
     """The chained reverse sweep over layers 7 .. 1: e0, e1 (or None) and y
     are [7, M, 256] tensors indexed by the output's layer (step s writes
     layer 6 - s); x (and the prologue codes x2) the first operand."""
     M = y.shape[1]
     step = -M * 256
-    _native.call("geraf_mlp_chain_adjoint_bf16", _ptr(x), _ld(x), _ptr(x2), _ld(x2), _ptr(rev),
-                 NL - 2, _ptr(e0[6]), 256, step, _ptr(None if e1 is None else e1[6]), 256, step,
-                 _ptr(y[6]), 256, step, M, _native.stream_of(y))
+    sweep(x, rev, NL - 2, y[6], step, e0=e0[6], e0_step=step,
+          e1=None if e1 is None else e1[6], e1_step=step, x2=x2)
 
 
 def _biases(params) -> List[torch.Tensor]:

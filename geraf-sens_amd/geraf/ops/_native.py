@@ -87,8 +87,9 @@ _SIGNATURES = {
                               _P],
     "geraf_mlp_pe_gradient_adjoint": [_P, _I64, ctypes.c_int32, ctypes.c_float, _P, _P, _I64,
                                       ctypes.c_int32, ctypes.c_int32, _P],
-    "geraf_mlp_chain_adjoint_bf16": [_P, _I64, _P, _I64, _P, ctypes.c_int32, _P, _I64, _I64,
-                                     _P, _I64, _I64, _P, _I64, _I64, _I64, _P],
+    "geraf_mlp_chain_adjoint_bf16": [_P, _I64, _P, _I64, _P, ctypes.c_int32, _P,
+                                     ctypes.c_int32, ctypes.c_int32, _P, _I64, _I64, _P, _I64,
+                                     _I64, _P, _I64, _I64, _P, _I64, _I64, _P],
     "geraf_mlp_sdf_chain_bf16": [_P, _I64, ctypes.c_int32, ctypes.c_float, _P, _P, _P,
                                  _P, _P, _I64, _I64, _P, _I64, _I64,
                                  ctypes.c_float, ctypes.c_float, _P],

@@ -39,6 +39,7 @@ import numpy as np
 import torch
 import torch.nn.functional as F
 
+from geraf.models.networks.fused_reflectance import fused_reflectance
 from geraf.models.networks.fused_sdf import fused_sdf, fused_sdf_value
 from geraf.models.networks.sdf_network import (ReflectivePowerNetwork, SDFNetwork,
                                                 SingleVarianceNetwork)
@@ -249,7 +250,8 @@ class StageStep:
             feat = out[:, 1:]
         grad = grad.reshape(R, S, 3)
         inv_s = self.var_net.inv_s().clip(1e-6, 1e6)
-        refl = self.sig_net.reflectance(self.pts_n, feat).reshape(R, S)
+        refl = (fused_reflectance(self.sig_net, self.pts_n, feat) if self.hand_sdf
+                else self.sig_net.reflectance(self.pts_n, feat)).reshape(R, S)
         power = self.sig_net.power()
         r = self.radar
         if self.form == "fused":

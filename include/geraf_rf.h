@@ -441,15 +441,20 @@ int geraf_softplus_grad2_bf16(const uint16_t* ggx, const uint16_t* g, const uint
  * write layer by layer.
  *
  * geraf_mlp_chain_adjoint_bf16: `steps` chained 256 x 256 layer products
- * (the network's reverse sweeps), each output kept as the next product's
+ * (the networks' reverse sweeps), each output kept as the next product's
  * operand: y_0 = (prologue(x) . W_0^T) * s(e0_0) [+ e1_0], y_s = (y_{s-1} .
  * W_s^T) * s(e0_s) [+ e1_s], as geraf_mlp_gemm_bf16 with epilogue 6 (e1
- * NULL) or 5 computes each, bit for bit.  wpacked: the steps' packed
- * weights back to back, each geraf_mlp_pack_bytes(256, 256) bytes (pad a
- * narrower input with zero weights).  x [M, ldx] (ldx 0 broadcasts a row),
- * x2 its slope codes for the prologue x * s(x2) or NULL (not with e1).
- * Step s reads e0 + s * step_e0 (and e1 + s * step_e1) and writes y +
- * s * step_y (strides in elements, may be negative); y may equal e1. */
+ * NULL) or 5 computes each, bit for bit; steps from mask_steps on take no
+ * slope (y_s = z [+ e1_s], e0 unread).  The slopes s are the softplus codes'
+ * (mask_kind 0) or the ReLU masks a > 0 of BF16 activations (mask_kind 1,
+ * which needs the prologue and no e1).  wpacked: the steps' packed weights
+ * back to back, each geraf_mlp_pack_bytes(256, 256) bytes (pad a narrower
+ * input with zero weights).  x [M, ldx] (ldx 0 broadcasts a row); x2 the
+ * slopes for the prologue x * s(x2) (times x_scale[p] when x_scale, FP32
+ * [M], is not NULL) or NULL (not with e1); x_out, when not NULL, receives
+ * the first operand as multiplied.  Step s reads e0 + s * step_e0 (and e1
+ * + s * step_e1) and writes y + s * step_y (strides in elements, may be
+ * negative); y may equal e1. */
 int64_t geraf_mlp_pack_bytes(int64_t N, int64_t K);
 int geraf_mlp_pack_weights(const float* w, int64_t ldw, int32_t transpose, float scale,
                            int64_t N, int64_t K, void* packed, geraf_stream_t stream);
@@ -477,10 +482,12 @@ int geraf_mlp_pe_gradient_adjoint(const float* x, int64_t M, int32_t L, float sc
                                   int32_t pad_to, geraf_stream_t stream);
 int64_t geraf_mlp_sdf_chain_pack_bytes(int32_t L);
 int geraf_mlp_chain_adjoint_bf16(const uint16_t* x, int64_t ldx, const uint16_t* x2,
-                                 int64_t ldx2, const void* wpacked, int32_t steps,
+                                 int64_t ldx2, const float* x_scale, int32_t mask_kind,
+                                 const void* wpacked, int32_t steps, int32_t mask_steps,
                                  const uint16_t* e0, int64_t lde0, int64_t step_e0,
                                  const uint16_t* e1, int64_t lde1, int64_t step_e1, uint16_t* y,
-                                 int64_t ldy, int64_t step_y, int64_t M, geraf_stream_t stream);
+                                 int64_t ldy, int64_t step_y, uint16_t* x_out, int64_t ldx_out,
+                                 int64_t M, geraf_stream_t stream);
 int geraf_mlp_sdf_chain_bf16(const float* x, int64_t M, int32_t L, float scale,
                              const void* wpacked, const float* bias, const float* w_sdf,
                              float* sdf, uint16_t* h, int64_t ldh,

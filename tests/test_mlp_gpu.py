@@ -332,3 +332,68 @@ def test_chained_sweep_matches_layers(cuda, M, mode):
     if t is not None:
         fs._sweep(x, rev, q, t, t)
         assert torch.equal(t.view(torch.int16), y.view(torch.int16))
+
+
+def test_fused_reflectance_vs_fp64(cuda):
+    """The reflectance network (259 -> 256 -> 256 -> 256 -> 1, ReLU, sigmoid)
+    on the matrix-core kernels: r and the gradients of a loss through it
+    with respect to the features and every layer's weight and bias, against
+    the FP64 network; each error within 1.5x that of torch's BF16 network.
+    Features read in place from [features | sdf | padding] rows (the SDF
+    network's output buffer) give the same results, bit for bit, as a
+    contiguous copy."""
+    import torch.nn.utils.parametrize as parametrize
+
+    from geraf.models.networks.fused_reflectance import fused_reflectance
+    from geraf.models.networks.sdf_network import ReflectivePowerNetwork
+
+    torch.manual_seed(2)
+    ref = ReflectivePowerNetwork(256).to(cuda)
+    with torch.no_grad():
+        for p in ref.parameters():
+            p.add_(0.02 * torch.randn_like(p))
+    bfn = ReflectivePowerNetwork(256, compute_dtype=torch.bfloat16).to(cuda)
+    bfn.load_state_dict(ref.state_dict())
+    M = 30001
+    g = torch.Generator(device="cpu").manual_seed(3)
+    pts = (torch.rand(M, 3, generator=g) * 1.6 - 0.8).to(cuda)
+    rows = torch.zeros(M, 264, dtype=torch.bfloat16, device=cuda)
+    rows[:, :256] = (torch.randn(M, 256, generator=g) * 0.3).to(cuda)
+    rows[:, 256] = 7.0  # the sdf column: multiplied by a zero weight
+    feat_view = rows[:, :256]
+    feat = feat_view.contiguous()
+    w = torch.randn(M, generator=g).to(cuda)
+
+    def run(kind, f):
+        net = ref if kind == "f64" else bfn
+        if kind == "f64":
+            net.double()
+        lins = [net.refelective_predictor[i] for i in (0, 2, 4, 6)]
+        with parametrize.cached():
+            params = [t for l in lins for t in (l.weight, l.bias)]
+            fe = f.detach().double().requires_grad_(True) if kind == "f64" else \
+                f.detach().clone().requires_grad_(True) if kind == "torch_bf16" else \
+                f.detach().requires_grad_(True)
+            if kind == "fused":
+                r = fused_reflectance(net, pts, fe)
+            else:
+                r = net.reflectance(pts.double() if kind == "f64" else pts, fe)
+            L = (w.double() * r.double()).sum()
+            grads = torch.autograd.grad(L, [fe] + params)
+        if kind == "f64":
+            net.float()
+        return r.detach().double(), [t.double() for t in grads]
+
+    r64, g64 = run("f64", feat)
+    rt, gt = run("torch_bf16", feat)
+    rf, gf = run("fused", feat)
+    rv, gv = run("fused", feat_view)
+    assert torch.equal(rf, rv) and all(torch.equal(a, b) for a, b in zip(gf, gv))
+    et, ef = _rel(rt, r64), _rel(rf, r64)
+    print(f"reflectance: fused {ef:.2e}, torch BF16 {et:.2e}", flush=True)
+    assert ef <= 1.5 * et + 1e-4
+    names = ["features"] + [f"lin{i}.{k}" for i in range(4) for k in ("weight", "bias")]
+    for name, a64, at, af in zip(names, g64, gt, gf):
+        et, ef = _rel(at, a64), _rel(af, a64)
+        print(f"grad {name}: fused {ef:.2e}, torch BF16 {et:.2e}", flush=True)
+        assert ef <= 1.5 * et + 1e-4, name
