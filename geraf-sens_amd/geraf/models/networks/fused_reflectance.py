@@ -104,9 +104,11 @@ class _FusedReflectance(Function):
             fs.pack(w, True, out=rev[s * n:(s + 1) * n])
         w_d = ws[3].to(BF).expand(M, 256)
         g_c = torch.empty((M, 256), dtype=BF, device=dev)
-        g = torch.empty((3, M, 256), dtype=BF, device=dev)  # g_B, g_A, d features
+        # g_B, g_A, d features in rows of 264: the SDF network's backward takes
+        # d features as the first columns of its output-layer gradient rows
+        g = torch.empty((3, M, ROW), dtype=BF, device=dev)[:, :, :256]
         if M > 0:
-            fs.sweep(w_d, rev, 3, g[0], M * 256, e0=a[1], e0_step=-M * 256, x2=a[2],
+            fs.sweep(w_d, rev, 3, g[0], M * ROW, e0=a[1], e0_step=-M * 256, x2=a[2],
                      x_scale=g_d, relu=True, mask_steps=2, x_out=g_c)
         dws = [torch.zeros_like(w) for w in ws]
         dbs = [torch.zeros(w.shape[0], dtype=torch.float32, device=dev) for w in ws]

@@ -308,6 +308,13 @@ def _normal_pass(x, ws, q, rev):
     return g, us, n
 
 
+def _rows_of_264(t: torch.Tensor, M: int) -> bool:
+    """Whether t [M,256] BF16 is the first 256 columns of rows of 264."""
+    return (t.dtype == BF and t.dim() == 2 and tuple(t.shape) == (M, 256) and
+            t.stride() == (264, 1) and
+            (t.storage_offset() + M * 264) * 2 <= t.untyped_storage().nbytes())
+
+
 class _FusedSDF(Function):
     @staticmethod
     def forward(ctx, x, with_normals: bool, *params):
@@ -381,12 +388,17 @@ class _FusedSDF(Function):
             wgrad(ones, vb, 1, 256, row)
             dws[8][0] += row[0]
             del vb
-        # primal adjoint; the output layer's gradient [d features | d sdf]
-        zb = torch.empty((M, 264), dtype=BF, device=dev)
-        if d_feat is not None:
-            zb[:, :256].copy_(d_feat)
+        # primal adjoint; the output layer's gradient [d features | d sdf],
+        # in place when d features already are the first columns of such rows
+        # (the fused reflectance network's backward writes them so)
+        if d_feat is not None and _rows_of_264(d_feat, M):
+            zb = torch.as_strided(d_feat, (M, 264), (264, 1))
         else:
-            zb[:, :256].zero_()
+            zb = torch.empty((M, 264), dtype=BF, device=dev)
+            if d_feat is not None:
+                zb[:, :256].copy_(d_feat)
+            else:
+                zb[:, :256].zero_()
         if d_sdf is not None:
             zb[:, 256].copy_(d_sdf)
         else:
