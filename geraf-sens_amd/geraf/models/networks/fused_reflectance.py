@@ -7,8 +7,8 @@ b_C), a_B = relu(W_B a_A + b_B), a_A = relu(W_A x_0 + b_A), x_0 = [features
 (256), points (3)].
 
 Forward: three layer products with the ReLU epilogue and the one-row output
-product (its sigmoid in FP32).  The input is read in place from a [M, 264]
-row buffer: the features in columns 0..255 and the points in 257..259,
+product (its sigmoid in FP32).  The input is read in place from rows of at
+least 264 elements: the features in columns 0..255 and the points in 257..259,
 with column 256 multiplied by a zero weight.  The SDF network's output
 buffer has exactly that layout ([features | sdf | padding]), so features
 coming from it need no concatenation: the points are written into its
@@ -32,8 +32,8 @@ from geraf.models.networks import fused_sdf as fs
 from geraf.ops import _native
 
 BF = torch.bfloat16
-ROW = 264   # the input row: features, one unused column, the points, padding
-PTS = 257   # first column of the points
+PTS = 257   # first column of the points (the input rows: features, one
+            # unused column, the points, padding to 264)
 
 
 def _layers(net):
@@ -49,16 +49,13 @@ def _layers(net):
 
 def _input_rows(points: torch.Tensor, features: torch.Tensor) -> torch.Tensor:
     """[M, 264] BF16 rows [features | - | points | -]: the features' own
-    buffer when they are the first 256 columns of such rows, else a copy."""
+    buffer when they are the first 256 columns of wider rows, else a copy."""
     M = points.shape[0]
-    st = features.untyped_storage()
-    in_place = (features.dtype == BF and features.dim() == 2 and features.shape[1] == 256 and
-                features.stride() == (ROW, 1) and
-                (features.storage_offset() + M * ROW) * 2 <= st.nbytes())
-    if in_place:
-        x0 = torch.as_strided(features.detach(), (M, ROW), (ROW, 1))
+    L = fs.wide_rows(features, M)
+    if L:
+        x0 = torch.as_strided(features.detach(), (M, 264), (L, 1))
     else:
-        x0 = torch.zeros((M, ROW), dtype=BF, device=points.device)
+        x0 = torch.zeros((M, fs.OUT_ROW), dtype=BF, device=points.device)[:, :264]
         x0[:, :256] = features.detach()
     x0[:, PTS:PTS + 3] = points.detach().to(BF)
     return x0
@@ -104,11 +101,11 @@ class _FusedReflectance(Function):
             fs.pack(w, True, out=rev[s * n:(s + 1) * n])
         w_d = ws[3].to(BF).expand(M, 256)
         g_c = torch.empty((M, 256), dtype=BF, device=dev)
-        # g_B, g_A, d features in rows of 264: the SDF network's backward takes
-        # d features as the first columns of its output-layer gradient rows
-        g = torch.empty((3, M, ROW), dtype=BF, device=dev)[:, :, :256]
+        # g_B, g_A, d features in rows of OUT_ROW: the SDF network's backward
+        # takes d features as the first columns of its output-layer gradient
+        g = torch.empty((3, M, fs.OUT_ROW), dtype=BF, device=dev)[:, :, :256]
         if M > 0:
-            fs.sweep(w_d, rev, 3, g[0], M * ROW, e0=a[1], e0_step=-M * 256, x2=a[2],
+            fs.sweep(w_d, rev, 3, g[0], M * fs.OUT_ROW, e0=a[1], e0_step=-M * 256, x2=a[2],
                      x_scale=g_d, relu=True, mask_steps=2, x_out=g_c)
         dws = [torch.zeros_like(w) for w in ws]
         dbs = [torch.zeros(w.shape[0], dtype=torch.float32, device=dev) for w in ws]

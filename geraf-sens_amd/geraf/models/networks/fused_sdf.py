@@ -120,6 +120,10 @@ IN = [39, 256, 256, 256, 256, 256, 256, 256, 256]
 OUT = [256, 256, 256, 217, 256, 256, 256, 256, 257]
 SKIP = 4
 NL = 9
+# row pitch of the output layer's [features | sdf | padding] rows and of its
+# gradient: 640 bytes, a multiple of 128, so that the products' 64-byte row
+# segments stay aligned (rows of 264, 528 bytes, made them straddle lines)
+OUT_ROW = 320
 L_BANDS = 6
 SQRT2 = math.sqrt(2.0)
 
@@ -237,7 +241,7 @@ def _chain(x, hidden, table, w_sdf=None, sdf=None, h=None, q=None) -> None:
 def _forward(x, P, hidden, table, b8):
     """The training forward: the hidden layers chained in one kernel (h_l and
     the slope codes q_l stored), then the output layer.  Returns (gam, the
-    slope codes [8, M, 256], h list, out8 [M, 264] = [features | sdf |
+    slope codes [8, M, 256], h list, out8 [M, OUT_ROW] = [features | sdf |
     padding])."""
     M = x.shape[0]
     dev = x.device
@@ -247,7 +251,7 @@ def _forward(x, P, hidden, table, b8):
     h = torch.empty((NL - 1, M, 256), dtype=BF, device=dev)
     q = torch.empty((NL - 1, M, 256), dtype=CODE, device=dev)
     _chain(x, hidden, table, h=h, q=q)
-    out8 = torch.empty((M, 264), dtype=BF, device=dev)
+    out8 = torch.empty((M, OUT_ROW), dtype=BF, device=dev)
     gemm(h[7], P[8], OUT[8], IN[8], out8, bias=b8)
     return gam, q, list(h.unbind(0)), out8
 
@@ -280,7 +284,7 @@ def _forward_layers(x, ws, bs, P, train: bool):
         qs.append(z)
         hs.append(h)
         a = h
-    out8 = torch.empty((M, 264), dtype=BF, device=dev)
+    out8 = torch.empty((M, OUT_ROW), dtype=BF, device=dev)
     gemm(a, P[8], OUT[8], IN[8], out8, bias=bs[8])
     return gam, qs, hs, out8
 
@@ -308,11 +312,15 @@ def _normal_pass(x, ws, q, rev):
     return g, us, n
 
 
-def _rows_of_264(t: torch.Tensor, M: int) -> bool:
-    """Whether t [M,256] BF16 is the first 256 columns of rows of 264."""
-    return (t.dtype == BF and t.dim() == 2 and tuple(t.shape) == (M, 256) and
-            t.stride() == (264, 1) and
-            (t.storage_offset() + M * 264) * 2 <= t.untyped_storage().nbytes())
+def wide_rows(t: torch.Tensor, M: int) -> int:
+    """The row pitch L when t [M,256] BF16 is the first 256 columns of rows
+    of L >= 264 elements (a multiple of 8), else 0."""
+    if t.dtype != BF or t.dim() != 2 or tuple(t.shape) != (M, 256) or t.stride(1) != 1:
+        return 0
+    L = t.stride(0)
+    ok = L >= 264 and L % 8 == 0 and \
+        (t.storage_offset() + M * L) * 2 <= t.untyped_storage().nbytes()
+    return L if ok else 0
 
 
 class _FusedSDF(Function):
@@ -391,10 +399,11 @@ class _FusedSDF(Function):
         # primal adjoint; the output layer's gradient [d features | d sdf],
         # in place when d features already are the first columns of such rows
         # (the fused reflectance network's backward writes them so)
-        if d_feat is not None and _rows_of_264(d_feat, M):
-            zb = torch.as_strided(d_feat, (M, 264), (264, 1))
+        L = wide_rows(d_feat, M) if d_feat is not None else 0
+        if L:
+            zb = torch.as_strided(d_feat, (M, 264), (L, 1))
         else:
-            zb = torch.empty((M, 264), dtype=BF, device=dev)
+            zb = torch.empty((M, OUT_ROW), dtype=BF, device=dev)[:, :264]
             if d_feat is not None:
                 zb[:, :256].copy_(d_feat)
             else:
