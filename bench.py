@@ -549,6 +549,31 @@ def bench_sdf_network(dev, M: int = 8388608, reps: int = 3):
     log(f"SDF network pass on {M} points: hand-written {out['fused_ms']:.1f} ms, torch BF16 "
         f"{out['torch_bf16_ms']:.1f} ms")
 
+    # the reflectance network on the same points (features from the SDF
+    # network's output rows, as the stage steps feed it), forward + backward
+    from geraf.models.networks.fused_reflectance import fused_reflectance
+    from geraf.models.networks.sdf_network import ReflectivePowerNetwork
+    rnet = ReflectivePowerNetwork(256, compute_dtype=BF).to(dev)
+    with torch.no_grad():
+        feat_rows = fs.fused_sdf(net, x, with_normals=False)[1]
+    g_r = torch.randn(M, device=dev)
+
+    def refl_pass(fused):
+        f = feat_rows.detach().requires_grad_(True)
+        r = fused_reflectance(rnet, x, f) if fused else rnet.reflectance(x, f)
+        torch.autograd.backward([r], [g_r.to(r.dtype)])
+
+    out["reflectance_network"] = {
+        "workload": f"reflectance network forward + backward on {M} points",
+        "fused_ms": timed(lambda: refl_pass(True)),
+        "torch_bf16_ms": timed(lambda: refl_pass(False))}
+    rn = out["reflectance_network"]
+    rn["speedup_vs_torch_bf16"] = rn["torch_bf16_ms"] / rn["fused_ms"]
+    log(f"reflectance network on {M} points: hand-written {rn['fused_ms']:.1f} ms, torch BF16 "
+        f"{rn['torch_bf16_ms']:.1f} ms")
+    del rnet, feat_rows
+    torch.cuda.empty_cache()
+
     # the kernel forms the pass runs, per launch, on [M,256] operands
     params = [t for l in range(9)
               for t in (getattr(net, f"lin{l}").weight, getattr(net, f"lin{l}").bias)]
