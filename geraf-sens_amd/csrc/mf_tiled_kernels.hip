@@ -16,9 +16,9 @@
 // and truncating after N_T terms leaves at most sum_k |s_k| Z^N_T/N_T! e^Z
 // (the host picks N_T from the largest tile).
 //
-// Kernels: per antenna chunk, mf_tile_moments_kernel (wave = antenna, lane =
-// tile: the echo row and the xi^n table are wave-uniform SGPR operands; the
-// phasor e^{-j2pi kappa d tau_ac} is a rotation re-seeded every 32 samples
+// Kernels: per antenna chunk, mf_tile_moments_mma_kernel (the sums over k as
+// matrix products: the xi^n table times the rotated echo of 16 tiles; the
+// phasor e^{-j2pi kappa d tau_ac} is a rotation re-seeded every 8 samples
 // from the FP64-reduced phase), then mf_tile_eval_kernel (block = tile, two
 // voxels per lane, the tile's coefficients of one antenna are wave-uniform
 // SGPR operands; per pair: FP64 distances and phase reduction, one phasor
@@ -67,90 +67,172 @@ __device__ __forceinline__ TileGeom tile_geom(const float4* tile_info, int t, do
   return g;
 }
 
-// xi_pow[k * NT + n] = xi_k^n, xi_k = (k - kc) / kc (0 when kc == 0).
-__global__ void xi_pow_kernel(float* __restrict__ xi_pow, int K, int NT, double kc) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  const double xi = kc > 0.0 ? ((double)k - kc) / kc : 0.0;
-  double p = 1.0;
-  for (int n = 0; n < NT; ++n) {
-    xi_pow[(size_t)k * NT + n] = (float)p;
-    p *= xi;
-  }
-}
-
 __global__ void fill_zero_kernel(double* __restrict__ p, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
 }
 
+// The forward moments on the matrix cores: S_n(t) = sum_k xi_k^n b_k(t),
+// b_k(t) = s_k e^{-j 2 pi (k - kc) d tau_c(t)}, for 16 tiles per wave and
+// one antenna, on v_mfma_f32_16x16x32_f16 with FP16 hi/lo operands: A = the
+// powers xi_k^n (16 moments x 32 frequencies, from a table in operand
+// order), B = b (32 frequencies x 16 tiles; lane (j, g) forms b_k of tile j
+// for k = 32 ks + 8 g .. + 7 from one phasor seed and seven rotations), the
+// real and imaginary parts as two real products of three instructions: per
+// pair (frequency, tile) ~15 vector instructions instead of the 8 + 2 N_T
+// FMAs of a per-lane sum.  The echo row is scaled by a power of two (max |s|
+// in [0.5, 1)) so FP16 keeps its small entries; the moments are scaled back.
+constexpr int kMomTiles = 16;
+constexpr int kMomChunk = 1024;  // echo samples staged in LDS at a time (8 KB)
+
+__device__ __forceinline__ int moment_blocks(int nt) { return (nt + 15) / 16; }
+
+// xi_ops[((ks * NB + blk) * 2 + h) * 64 + lane], NB = ceil(NT / 16): the A
+// operand of frequency step ks and moment block blk, hi (h = 0) or lo;
+// element e of lane (i, g) is xi_k^n for n = 16 blk + i, k = 32 ks + 8 g + e
+// (0 past K or NT).
+__global__ void xi_ops_kernel(h8* __restrict__ ops, int K, int NT, double kc) {
+  const int NB = moment_blocks(NT);
+  const int steps = (K + 31) / 32;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= steps * NB * kWave) return;
+  const int lane = idx & (kWave - 1), blk = (idx / kWave) % NB, ks = idx / (kWave * NB);
+  const int n = 16 * blk + (lane & 15);
+  h8 hi, lo;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int k = 32 * ks + 8 * (lane >> 4) + e;
+    double x = 0.0;
+    if (k < K && n < NT) {
+      const double xi = kc > 0.0 ? ((double)k - kc) / kc : 0.0;
+      x = 1.0;
+      for (int i = 0; i < n; ++i) x *= xi;
+    }
+    const _Float16 h = (_Float16)x;
+    hi[e] = h;
+    lo[e] = (_Float16)(x - (double)h);
+  }
+  ops[((size_t)(ks * NB + blk) * 2 + 0) * kWave + lane] = hi;
+  ops[((size_t)(ks * NB + blk) * 2 + 1) * kWave + lane] = lo;
+}
+
 template <int NT>
-__global__ __launch_bounds__(64 * kMomentWaves) void mf_tile_moments_kernel(
+__global__ __launch_bounds__(64 * kMomentWaves) void mf_tile_moments_mma_kernel(
     const float* __restrict__ real, const float* __restrict__ imag,
     const float* __restrict__ tx_pos, const float* __restrict__ rx_pos,
-    const float4* __restrict__ tile_info, const float* __restrict__ xi_pow, int a0, int na,
-    int K, int ntiles, PhaseConsts pc, double kc, float2* __restrict__ coef,
+    const float4* __restrict__ tile_info, const h8* __restrict__ xi_ops, int a0, int na, int K,
+    int ntiles, PhaseConsts pc, double kc, float2* __restrict__ coef,
     double* __restrict__ tau_c) {
+  constexpr int NB = (NT + 15) / 16;
+  __shared__ float4 echo[kMomChunk / 2];  // (re, im) pairs of the chunk, zero past K
+  __shared__ float wmax[kMomentWaves];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ai = blockIdx.y * kMomentWaves + wave;  // antenna within the chunk
-  if (ai >= na) return;                             // wave-uniform
+  const int tg = blockIdx.x * kMomentWaves + wave;  // group of 16 tiles
+  const int ai = blockIdx.y;                        // antenna within the chunk (< na)
   const int a = a0 + ai;
-  const int t = blockIdx.x * kWave + lane;
-  const bool tile_ok = t < ntiles;
-  const TileGeom g = tile_geom(tile_info, tile_ok ? t : ntiles - 1, pc.inv_c);
+  const int j = lane & 15, g = lane >> 4;
+  const int t = tg * kMomTiles + j;
+  const bool tile_ok = t < ntiles;  // waves past the last tile still join the barriers
+  const TileGeom geo = tile_geom(tile_info, tile_ok ? t : ntiles - 1, pc.inv_c);
   const double tx = tx_pos[3 * a], ty = tx_pos[3 * a + 1], tz = tx_pos[3 * a + 2];
   const double rx = rx_pos[3 * a], ry = rx_pos[3 * a + 1], rz = rx_pos[3 * a + 2];
-  const double tac = (dist3_f64_fast(tx, ty, tz, g.cx, g.cy, g.cz) +
-                      dist3_f64_fast(rx, ry, rz, g.cx, g.cy, g.cz) + pc.bias) *
+  const double tac = (dist3_f64_fast(tx, ty, tz, geo.cx, geo.cy, geo.cz) +
+                      dist3_f64_fast(rx, ry, rz, geo.cx, geo.cy, geo.cz) + pc.bias) *
                      pc.inv_c;
   const double step = pc.dfs * tac;  // cycles per sample at the centre delay
-  // e^{-j 2 pi d tau_ac}: one sample forward
-  const float2 w = phasor_cycles((float)centered_frac(step), -1.0f);
+  const float2 w = cis_neg_reduced((float)centered_frac(step));  // one sample forward
 
-  float Sr[NT], Si[NT];
+  // the echo row's scale: max |s| of the row into [0.5, 1)
+  const float* __restrict__ er = real + (size_t)a * K;
+  const float* __restrict__ ei = imag + (size_t)a * K;
+  float mx = 0.0f;
+  for (int k = threadIdx.x; k < K; k += 64 * kMomentWaves)
+    mx = fmaxf(mx, fmaxf(fabsf(er[k]), fabsf(ei[k])));
+  mx = wave_max_dpp(mx);
+  if (lane == 0) wmax[wave] = mx;
+  __syncthreads();
 #pragma unroll
-  for (int n = 0; n < NT; ++n) Sr[n] = Si[n] = 0.0f;
+  for (int w2 = 0; w2 < kMomentWaves; ++w2) mx = fmaxf(mx, wmax[w2]);
+  int ex = 0;
+  if (mx > 0.0f && isfinite(mx)) (void)frexpf(mx, &ex);
+  ex = min(max(ex, -120), 120);  // both scales stay finite normal numbers
+  const float down = ldexpf(1.0f, -ex);
 
-  const int row = __builtin_amdgcn_readfirstlane(a * K);
-  const float* __restrict__ er = real + row;
-  const float* __restrict__ ei = imag + row;
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    // e^{-j 2 pi (k0 - kc) d tau_ac} from the FP64-reduced phase
-    float2 ph = phasor_cycles((float)centered_frac((kc - (double)k0) * step), 1.0f);
-    const int k1 = min(K, k0 + 32);
-    for (int k = k0; k < k1; ++k) {
-      const float sr = er[k], si = ei[k];
-      const float br = fmaf(sr, ph.x, -si * ph.y);
-      const float bi = fmaf(sr, ph.y, si * ph.x);
-      const float* __restrict__ xp = xi_pow + (size_t)k * NT;
+  f32x4v Sr[NB], Si[NB];
 #pragma unroll
-      for (int n = 0; n < NT; ++n) {
-        Sr[n] = fmaf(br, xp[n], Sr[n]);
-        Si[n] = fmaf(bi, xp[n], Si[n]);
+  for (int blk = 0; blk < NB; ++blk) Sr[blk] = Si[blk] = f32x4v{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int c0 = 0; c0 < K; c0 += kMomChunk) {
+    const int steps = (min(kMomChunk, K - c0) + 31) / 32;
+    if (c0 > 0) __syncthreads();  // the previous chunk's reads are done
+    for (int i = threadIdx.x; i < 16 * steps; i += 64 * kMomentWaves) {
+      const int k = c0 + 2 * i;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (k < K) v.x = er[k], v.y = ei[k];
+      if (k + 1 < K) v.z = er[k + 1], v.w = ei[k + 1];
+      echo[i] = v;
+    }
+    __syncthreads();
+    for (int ks = 0; ks < steps; ++ks) {
+      const int kl = 32 * ks + 8 * g;  // first sample of the lane, within the chunk
+      // e^{-j 2 pi (k - kc) d tau_c} at k = c0 + kl from the FP64-reduced
+      // phase, times the row's scale
+      float2 ph = phasor_cycles_hw((float)centered_frac((kc - (double)(c0 + kl)) * step));
+      ph = make_float2(ph.x * down, ph.y * down);
+      float2 b[8];
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        const float4 v = echo[kl / 2 + e / 2];
+        b[e] = make_float2(fmaf(v.x, ph.x, -v.y * ph.y), fmaf(v.x, ph.y, v.y * ph.x));
+        ph = cmul(ph, w);
+        b[e + 1] = make_float2(fmaf(v.z, ph.x, -v.w * ph.y), fmaf(v.z, ph.y, v.w * ph.x));
+        ph = cmul(ph, w);
       }
-      ph = cmul(ph, w);
+      h8 brh, brl, bih, bil;
+      split8(b, brh, brl, bih, bil);
+      const int kg = c0 / 32 + ks;  // frequency step of the whole row
+#pragma unroll
+      for (int blk = 0; blk < NB; ++blk) {
+        const h8 ah = xi_ops[((size_t)(kg * NB + blk) * 2 + 0) * kWave + lane];
+        const h8 al = xi_ops[((size_t)(kg * NB + blk) * 2 + 1) * kWave + lane];
+        Sr[blk] = mma3(ah, al, brh, brl, Sr[blk]);
+        Si[blk] = mma3(ah, al, bih, bil, Si[blk]);
+      }
     }
   }
   if (!tile_ok) return;
-  // C_n = (-j)^n Z^n / n! S_n
-  const float Z = (float)(3.14159265358979323846 * (double)(K - 1) * pc.dfs * g.H);
-  float scale = 1.0f;
+  // lane (j, g) holds S_n of tile j for n = 16 blk + 4 g + r.
+  // C_n = (-j)^n Z^n / n! S_n, times the row's scale back
+  const float Z = (float)(3.14159265358979323846 * (double)(K - 1) * pc.dfs * geo.H);
+  const float up = ldexpf(1.0f, ex);
+  float sc[NB][4];
+  {
+    float cur = up;  // up Z^n / n!, n = 0, 1, ...
+#pragma unroll
+    for (int n = 0; n < 16 * NB; ++n) {
+      if (((n >> 2) & 3) == g) sc[n >> 4][n & 3] = cur;
+      cur *= Z / (float)(n + 1);
+    }
+  }
   float2* __restrict__ out = coef + ((size_t)ai * ntiles + t) * NT;
 #pragma unroll
-  for (int n = 0; n < NT; ++n) {
-    const float xr = Sr[n] * scale, xi = Si[n] * scale;
-    float2 c;
-    switch (n & 3) {
-      case 0: c = make_float2(xr, xi); break;
-      case 1: c = make_float2(xi, -xr); break;
-      case 2: c = make_float2(-xr, -xi); break;
-      default: c = make_float2(-xi, xr); break;
+  for (int blk = 0; blk < NB; ++blk)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * blk + 4 * g + r;
+      if (n < NT) {
+        const float xr = Sr[blk][r] * sc[blk][r], xi = Si[blk][r] * sc[blk][r];
+        float2 c;
+        switch (r) {  // n & 3 == r
+          case 0: c = make_float2(xr, xi); break;
+          case 1: c = make_float2(xi, -xr); break;
+          case 2: c = make_float2(-xr, -xi); break;
+          default: c = make_float2(-xi, xr); break;
+        }
+        out[n] = c;
+      }
     }
-    out[n] = c;
-    scale *= Z / (float)(n + 1);
-  }
-  tau_c[(size_t)ai * ntiles + t] = tac;
+  if (g == 0) tau_c[(size_t)ai * ntiles + t] = tac;
 }
 
 // TW waves per tile (tiles of 128 TW voxels); a block of kEvalWaves waves
@@ -494,7 +576,8 @@ TiledLayout tiled_layout(int64_t K, int64_t N, int NT, int tv) {
   // evaluation blocks: one per 512-voxel tile, one per four 128-voxel tiles
   l.groups = antenna_groups(tv == 512 ? ntiles : (ntiles + 3) / 4);
   l.acc_bytes = align256((size_t)N * sizeof(double)) * l.groups;
-  l.xi_bytes = align256((size_t)K * NT * sizeof(float));
+  // the xi^n table in matrix-operand order (hi and lo per step and block)
+  l.xi_bytes = align256((size_t)((K + 31) / 32) * ((NT + 15) / 16) * 2 * kWave * sizeof(h8));
   l.per_antenna = (size_t)ntiles * (NT * sizeof(float2) + sizeof(double));
   l.fixed = 2 * l.acc_bytes + l.xi_bytes + 2 * 256;
   return l;
@@ -502,15 +585,15 @@ TiledLayout tiled_layout(int64_t K, int64_t N, int NT, int tv) {
 
 template <int NT>
 void launch_chunk(const float* real, const float* imag, const float* tx, const float* rx,
-                  const float* voxel, const float4* tile_info, const float* xi_pow, int a0,
+                  const float* voxel, const float4* tile_info, const h8* xi_ops, int a0,
                   int na, int K, int N, int ntiles, int tv, const PhaseConsts& pc, double kc,
                   float2* coef, double* tau_c, double* acc_re, double* acc_im, int groups,
                   size_t acc_stride, hipStream_t stream) {
-  const dim3 mgrid((unsigned)((ntiles + kWave - 1) / kWave),
-                   (unsigned)((na + kMomentWaves - 1) / kMomentWaves));
-  hipLaunchKernelGGL((mf_tile_moments_kernel<NT>), mgrid, dim3(64 * kMomentWaves), 0, stream,
-                     real, imag, tx, rx, tile_info, xi_pow, a0, na, K, ntiles, pc, kc, coef,
-                     tau_c);
+  const int groups16 = (ntiles + kMomTiles - 1) / kMomTiles;
+  const dim3 mgrid((unsigned)((groups16 + kMomentWaves - 1) / kMomentWaves), (unsigned)na);
+  hipLaunchKernelGGL((mf_tile_moments_mma_kernel<NT>), mgrid, dim3(64 * kMomentWaves), 0,
+                     stream, real, imag, tx, rx, tile_info, xi_ops, a0, na, K, ntiles, pc, kc,
+                     coef, tau_c);
   const double fcen = pc.f0 + kc * pc.dfs;
   const int g = std::min(groups, na);
   const int per_group = (na + g - 1) / g;
@@ -536,7 +619,7 @@ namespace {
 struct TiledBuffers {
   double* acc_re;
   double* acc_im;
-  float* xi_pow;
+  h8* xi_ops;
   float2* coef;
   double* tau_c;
   int na_max;
@@ -551,7 +634,7 @@ TiledBuffers carve(void* workspace, int64_t workspace_bytes, const TiledLayout& 
   TiledBuffers b;
   b.acc_re = reinterpret_cast<double*>(ws);
   b.acc_im = reinterpret_cast<double*>(ws + l.acc_bytes);  // l.acc_bytes covers all groups
-  b.xi_pow = reinterpret_cast<float*>(ws + 2 * l.acc_bytes);
+  b.xi_ops = reinterpret_cast<h8*>(ws + 2 * l.acc_bytes);
   char* chunk = ws + 2 * l.acc_bytes + l.xi_bytes;
   const size_t avail = (size_t)workspace_bytes - base - 2 * l.acc_bytes - l.xi_bytes - 256;
   b.na_max = (int)std::min<int64_t>(A, (int64_t)(avail / l.per_antenna));
@@ -607,7 +690,7 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
   const TiledBuffers b = carve(workspace, workspace_bytes, l, A, ntiles, NT);
   double* acc_re = b.acc_re;
   double* acc_im = b.acc_im;
-  float* xi_pow = b.xi_pow;
+  h8* xi_ops = b.xi_ops;
   float2* coef = b.coef;
   double* tau_c = b.tau_c;
   const int na_max = b.na_max;
@@ -618,14 +701,17 @@ extern "C" int geraf_mf_forward_tiled_f32(const float* real, const float* imag,
   const size_t acc_stride = l.acc_bytes / l.groups / sizeof(double);  // doubles per slice
   hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((2 * l.acc_bytes / 8 + 255) / 256)),
                      dim3(256), 0, st, acc_re, (long long)(2 * l.acc_bytes / 8));
-  hipLaunchKernelGGL(xi_pow_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, xi_pow,
-                     (int)K, NT, kc);
+  {
+    const int ops = (int)((K + 31) / 32) * ((NT + 15) / 16) * kWave;
+    hipLaunchKernelGGL(xi_ops_kernel, dim3((unsigned)((ops + 255) / 256)), dim3(256), 0, st,
+                       xi_ops, (int)K, NT, kc);
+  }
   const float4* ti = reinterpret_cast<const float4*>(tile_info);
   for (int a0 = 0; a0 < A; a0 += na_max) {
     const int na = (int)std::min<int64_t>(na_max, A - a0);
 #define GERAF_TILED_CASE(T)                                                                    \
   case T:                                                                                      \
-    launch_chunk<T>(real, imag, tx_pos, rx_pos, voxel, ti, xi_pow, a0, na, (int)K, (int)N,     \
+    launch_chunk<T>(real, imag, tx_pos, rx_pos, voxel, ti, xi_ops, a0, na, (int)K, (int)N,     \
                     ntiles, tv, pc, kc, coef, tau_c, acc_re, acc_im, l.groups, acc_stride,     \
                     st);                                                                       \
     break;
