@@ -31,9 +31,10 @@
 //   g[a,k] = (1/norm) sum_t e^{+j2pi kappa d tau_ac} sum_n (j Z xi_k)^n / n! M_n(a,t),
 //   M_n(a,t) = sum_{v in t} gout_v e^{+j2pi(f0 + k_c d) tau_av} u_av^n,
 // computed by mf_tile_adj_moments_kernel (one wave per tile, eight voxels per
-// lane, a DPP wave sum per moment) and mf_tile_adj_eval_kernel (one wave per
-// antenna, consecutive frequencies per lane: a real Horner in Z xi_k per tile,
-// the phasor rotated along the lane's frequencies, Kahan over the tiles).
+// lane, wave sums of the moments by recursive halving) and
+// mf_tile_adj_eval_mma_kernel (one wave per antenna and 64 frequencies: the
+// sums over n as matrix products of the xi^n table and the tiles' Z^n M'_n,
+// then the phasor-weighted sum over the tiles on the vector units).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -446,61 +447,206 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
   }
 }
 
-template <int NT, int KPL>
-__global__ __launch_bounds__(256) void mf_tile_adj_eval_kernel(
+// The adjoint evaluation on the matrix cores.  Per antenna and tile t,
+//   Q[k, t] = sum_n xi_k^n Z_t^n M'_n(a, t)      (M'_n = j^n / n! M_n),
+// a real (frequencies x N_T) by complex (N_T x tiles) product on
+// v_mfma_f32_16x16x32_f16 with FP16 hi/lo operands (contraction over the
+// terms, zero-padded to 32): A = xi_k^n from a table in operand order, B =
+// the tile's scaled moments, built per 16 tiles (lane (j, g): tile j, terms
+// 8 g .. 8 g + 7), each tile's column scaled by a power of two (its largest
+// entry in [0.5, 1)) and scaled back through the phasor.  Then
+//   g[a, k] = sum_t e^{+j 2 pi (k - kc) d tau_c(t)} Q[k, t]
+// on the vector units: lane (j, q) holds k = k_w + 16 kb + 4 q + r of tile j
+// (kb, r = 0..3) and rotates one phasor seed per tile along those 16
+// frequencies; products are summed in FP32 over kAdjFlush tile blocks and
+// then in FP64, and the 16 lanes of a row (its tiles) are summed at the end.
+// One wave per (antenna, 64 frequencies).
+constexpr int kAdjKB = 4;      // 16-frequency blocks per wave (one per 4 lanes of a row)
+constexpr int kAdjFlush = 8;   // tile blocks summed in FP32 before each FP64 add
+
+// xi_adj_ops[(kb * 2 + h) * 64 + lane]: the A operand of frequency block kb,
+// hi (h = 0) or lo; element e of lane (i, g) is xi_k^n for k = 16 kb + i,
+// n = 8 g + e (0 past K or NT).
+__global__ void xi_adj_ops_kernel(h8* __restrict__ ops, int K, int NT, double kc) {
+  const int nkb = (K + 15) / 16;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nkb * kWave) return;
+  const int lane = idx & (kWave - 1), kb = idx / kWave;
+  const int k = 16 * kb + (lane & 15);
+  const double xi = kc > 0.0 ? ((double)k - kc) / kc : 0.0;
+  h8 hi, lo;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int n = 8 * (lane >> 4) + e;
+    double x = 0.0;
+    if (k < K && n < NT) {
+      x = 1.0;
+      for (int i = 0; i < n; ++i) x *= xi;
+    }
+    const _Float16 h = (_Float16)x;
+    hi[e] = h;
+    lo[e] = (_Float16)(x - (double)h);
+  }
+  ops[(size_t)(kb * 2 + 0) * kWave + lane] = hi;
+  ops[(size_t)(kb * 2 + 1) * kWave + lane] = lo;
+}
+
+// Maximum of the four lanes j, j + 16, j + 32, j + 48, in each of them.
+__device__ __forceinline__ float column_max4(float x) {
+  const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x),
+                                                    false, false);
+  x = fmaxf(__uint_as_float(s32[0]), __uint_as_float(s32[1]));
+  const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x),
+                                                    false, false);
+  return fmaxf(__uint_as_float(s16[0]), __uint_as_float(s16[1]));
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void mf_tile_adj_eval_mma_kernel(
     const float2* __restrict__ mom, const double* __restrict__ tau_c,
-    const float4* __restrict__ tile_info, int a0, int na, int K, int ntiles, PhaseConsts pc,
-    double kc, double inv_norm, float* __restrict__ grad_re, float* __restrict__ grad_im) {
+    const float4* __restrict__ tile_info, const h8* __restrict__ xi_ops, int a0, int na, int K,
+    int ntiles, PhaseConsts pc, double kc, double inv_norm, float* __restrict__ grad_re,
+    float* __restrict__ grad_im) {
+  static_assert(NT <= 32, "one contraction step of 32 terms");
+  static_assert(kAdjKB == 4, "the 16 lanes of a row write the 16 frequencies (kb, r)");
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ai = blockIdx.x * 4 + wave;
-  if (ai >= na) return;  // wave-uniform
+  const int nkw = (K + 16 * kAdjKB - 1) / (16 * kAdjKB);  // waves per antenna
+  const int wid = blockIdx.x * 4 + wave;
+  if (wid >= na * nkw) return;  // wave-uniform; no block barriers below
+  const int ai = wid / nkw;
+  const int kw = (wid - ai * nkw) * 16 * kAdjKB;  // first frequency of the wave
   const int a = a0 + ai;
-  const int kb = lane * KPL;
-  float xk[KPL];
+  const int j = lane & 15, q = lane >> 4;
+  const int nkb = (K + 15) / 16;
+  h8 ah[kAdjKB], al[kAdjKB];
 #pragma unroll
-  for (int j = 0; j < KPL; ++j) xk[j] = kc > 0.0 ? (float)(((double)(kb + j) - kc) / kc) : 0.0f;
-  KahanC acc[KPL];
-#pragma unroll
-  for (int j = 0; j < KPL; ++j) acc[j].init();
-  const double zfac = 3.14159265358979323846 * (double)(K - 1) * pc.dfs;
-  for (int t = 0; t < ntiles; ++t) {
-    const size_t ct = __builtin_amdgcn_readfirstlane((unsigned)ai) * (size_t)ntiles + t;
-    const double tac = tau_c[ct];
-    const float Z = (float)(zfac * 2.0 * (double)tile_info[t].w * pc.inv_c);
-    const float2* __restrict__ m = mom + ct * NT;
-    const double step = pc.dfs * tac;
-    float2 ph = phasor_cycles((float)centered_frac(((double)kb - kc) * step), 1.0f);
-    const float2 w = phasor_cycles((float)centered_frac(step), 1.0f);
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) {
-      const float x = Z * xk[j];
-      float Sr = m[NT - 1].x, Si = m[NT - 1].y;
-#pragma unroll
-      for (int n = NT - 2; n >= 0; --n) {
-        Sr = fmaf(Sr, x, m[n].x);
-        Si = fmaf(Si, x, m[n].y);
-      }
-      acc[j].add(cmul(ph, make_float2(Sr, Si)));
-      ph = cmul(ph, w);
+  for (int kb = 0; kb < kAdjKB; ++kb) {
+    const int b = kw / 16 + kb;  // uniform
+    if (b < nkb) {
+      ah[kb] = xi_ops[(size_t)(b * 2 + 0) * kWave + lane];
+      al[kb] = xi_ops[(size_t)(b * 2 + 1) * kWave + lane];
+    } else {
+      ah[kb] = al[kb] = h8{0, 0, 0, 0, 0, 0, 0, 0};
     }
   }
+  const double zfac = 3.14159265358979323846 * (double)(K - 1) * pc.dfs;
+  const double k0 = (double)(kw + 4 * q) - kc;  // the lane's first frequency, centred
+  float2 part[kAdjKB][4];
+  double accr[kAdjKB][4], acci[kAdjKB][4];
 #pragma unroll
-  for (int j = 0; j < KPL; ++j) {
-    const int k = kb + j;
-    if (k < K) {
-      grad_re[(size_t)a * K + k] = (float)(acc[j].re() * inv_norm);
-      grad_im[(size_t)a * K + k] = (float)(acc[j].im() * inv_norm);
+  for (int kb = 0; kb < kAdjKB; ++kb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      part[kb][r] = make_float2(0.0f, 0.0f);
+      accr[kb][r] = acci[kb][r] = 0.0;
     }
+  const int ntb = (ntiles + 15) / 16;
+  // the inputs of tile block tb: lane (j, q) loads tile j's moments 8 q ..
+  // 8 q + 7, its radius and its centre delay (clamped to the last tile)
+  auto load = [&](int tb, float2 (&m)[8], float& rad, double& tac) {
+    const int t = min(tb * 16 + j, ntiles - 1);
+    const size_t ct = (size_t)ai * ntiles + t;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) m[e] = mom[ct * NT + min(8 * q + e, NT - 1)];
+    rad = tile_info[t].w;
+    tac = tau_c[ct];
+  };
+  float2 nm[8];
+  float nrad;
+  double ntac;
+  load(0, nm, nrad, ntac);
+  for (int tb = 0; tb < ntb; ++tb) {
+    const bool ok = tb * 16 + j < ntiles;
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      v[e] = (ok && 8 * q + e < NT) ? nm[e] : make_float2(0.0f, 0.0f);
+    const float rad = nrad;
+    const double tac = ntac;
+    if (tb + 1 < ntb) load(tb + 1, nm, nrad, ntac);  // in flight during this block
+    // B: Z^n M'_n for n = 8 q + e, the tile's column scaled into [0.5, 1)
+    const float Z = (float)(zfac * 2.0 * (double)rad * pc.inv_c);
+    const float z2 = Z * Z, z4 = z2 * z2, z8 = z4 * z4;
+    float zp = q == 0 ? 1.0f : (q == 1 ? z8 : (q == 2 ? z8 * z8 : z8 * z8 * z8));
+    float mx = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = make_float2(v[e].x * zp, v[e].y * zp);
+      mx = fmaxf(mx, fmaxf(fabsf(v[e].x), fabsf(v[e].y)));
+      zp *= Z;
+    }
+    mx = column_max4(mx);
+    int ex = 0;
+    if (mx > 0.0f && isfinite(mx)) (void)frexpf(mx, &ex);
+    ex = min(max(ex, -120), 120);
+    const float down = ldexpf(1.0f, -ex);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = make_float2(v[e].x * down, v[e].y * down);
+    h8 brh, brl, bih, bil;
+    split8(v, brh, brl, bih, bil);
+    // the phasor of the lane's first frequency for tile j, with the scale back
+    // (polynomial sine and cosine: the three phasors set the accuracy of all
+    // 16 products of the lane)
+    const double step = pc.dfs * tac;
+    float2 ph = phasor_cycles((float)centered_frac(k0 * step), 1.0f);
+    const float up = ldexpf(1.0f, ex);
+    ph = make_float2(ph.x * up, ph.y * up);
+    const float2 w1 = phasor_cycles((float)centered_frac(step), 1.0f);
+    const float2 w16 = phasor_cycles((float)centered_frac(16.0 * step), 1.0f);
+#pragma unroll
+    for (int kb = 0; kb < kAdjKB; ++kb) {
+      const f32x4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
+      const f32x4v qr = mma3(ah[kb], al[kb], brh, brl, zero);
+      const f32x4v qi = mma3(ah[kb], al[kb], bih, bil, zero);
+      float2 p = ph;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        part[kb][r].x = fmaf(p.x, qr[r], fmaf(-p.y, qi[r], part[kb][r].x));
+        part[kb][r].y = fmaf(p.x, qi[r], fmaf(p.y, qr[r], part[kb][r].y));
+        if (r < 3) p = cmul(p, w1);
+      }
+      if (kb + 1 < kAdjKB) ph = cmul(ph, w16);
+    }
+    if ((tb + 1) % kAdjFlush == 0 || tb + 1 == ntb) {
+#pragma unroll
+      for (int kb = 0; kb < kAdjKB; ++kb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          accr[kb][r] += (double)part[kb][r].x;
+          acci[kb][r] += (double)part[kb][r].y;
+          part[kb][r] = make_float2(0.0f, 0.0f);
+        }
+    }
+  }
+  // sum the row's 16 lanes (tiles j); lane j of row q then writes
+  // k = k_w + 16 (j >> 2) + 4 q + (j & 3)
+  double outr = 0.0, outi = 0.0;
+#pragma unroll
+  for (int kb = 0; kb < kAdjKB; ++kb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double sr = accr[kb][r], si = acci[kb][r];
+#pragma unroll
+      for (int m = 1; m < 16; m <<= 1) {
+        sr += __shfl_xor(sr, m, kWave);
+        si += __shfl_xor(si, m, kWave);
+      }
+      if (j == 4 * kb + r) outr = sr, outi = si;
+    }
+  const int k = kw + 16 * (j >> 2) + 4 * q + (j & 3);
+  if (k < K) {
+    grad_re[(size_t)a * K + k] = (float)(outr * inv_norm);
+    grad_im[(size_t)a * K + k] = (float)(outi * inv_norm);
   }
 }
 
-template <int NT, int KPL>
+template <int NT>
 void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const float* rx,
-                      const float* voxel, const float4* tile_info, int a0, int na, int K, int N,
-                      int ntiles, int tv, const PhaseConsts& pc, double kc, double inv_norm,
-                      float2* mom, double* tau_c, float* grad_re, float* grad_im,
-                      hipStream_t stream) {
+                      const float* voxel, const float4* tile_info, const h8* xi_ops, int a0,
+                      int na, int K, int N, int ntiles, int tv, const PhaseConsts& pc, double kc,
+                      double inv_norm, float2* mom, double* tau_c, float* grad_re,
+                      float* grad_im, hipStream_t stream) {
   const double fcen = pc.f0 + kc * pc.dfs;
   // antenna ranges side by side when the tiles alone give fewer than ~32
   // waves per SIMD (one wave per tile)
@@ -516,32 +662,10 @@ void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const
     hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT, 2>), mgrid, dim3(256), 0, stream, gre,
                        gim, tx, rx, voxel, tile_info, a0, na, N, ntiles, pc, fcen, mom, tau_c,
                        per_group);
-  hipLaunchKernelGGL((mf_tile_adj_eval_kernel<NT, KPL>), dim3((unsigned)((na + 3) / 4)),
-                     dim3(256), 0, stream, mom, tau_c, tile_info, a0, na, K, ntiles, pc, kc,
-                     inv_norm, grad_re, grad_im);
-}
-
-template <int NT>
-void launch_adj_chunk_k(int kpl, const float* gre, const float* gim, const float* tx,
-                        const float* rx, const float* voxel, const float4* tile_info, int a0,
-                        int na, int K, int N, int ntiles, int tv, const PhaseConsts& pc,
-                        double kc, double inv_norm, float2* mom, double* tau_c, float* grad_re,
-                        float* grad_im, hipStream_t stream) {
-#define GERAF_ADJ_K(P)                                                                        \
-  case P:                                                                                     \
-    launch_adj_chunk<NT, P>(gre, gim, tx, rx, voxel, tile_info, a0, na, K, N, ntiles, tv, pc, \
-                            kc, inv_norm, mom, tau_c, grad_re, grad_im, stream);              \
-    break;
-  switch (kpl) {
-    GERAF_ADJ_K(1)
-    GERAF_ADJ_K(2)
-    GERAF_ADJ_K(4)
-    GERAF_ADJ_K(8)
-    GERAF_ADJ_K(16)
-    GERAF_ADJ_K(32)
-    default: throw ArgError("too many samples per antenna for the tiled adjoint");
-  }
-#undef GERAF_ADJ_K
+  const int64_t waves = (int64_t)na * ((K + 16 * kAdjKB - 1) / (16 * kAdjKB));
+  hipLaunchKernelGGL((mf_tile_adj_eval_mma_kernel<NT>), dim3((unsigned)((waves + 3) / 4)),
+                     dim3(256), 0, stream, mom, tau_c, tile_info, xi_ops, a0, na, K, ntiles, pc,
+                     kc, inv_norm, grad_re, grad_im);
 }
 
 // Instantiated term counts; a request is rounded up to the next one.
@@ -577,7 +701,9 @@ TiledLayout tiled_layout(int64_t K, int64_t N, int NT, int tv) {
   l.groups = antenna_groups(tv == 512 ? ntiles : (ntiles + 3) / 4);
   l.acc_bytes = align256((size_t)N * sizeof(double)) * l.groups;
   // the xi^n table in matrix-operand order (hi and lo per step and block)
-  l.xi_bytes = align256((size_t)((K + 31) / 32) * ((NT + 15) / 16) * 2 * kWave * sizeof(h8));
+  // (the forward's per 32 frequencies and 16 terms, the adjoint's per 16 frequencies)
+  l.xi_bytes = align256((size_t)std::max<int64_t>((K + 31) / 32 * ((NT + 15) / 16), (K + 15) / 16) *
+                        2 * kWave * sizeof(h8));
   l.per_antenna = (size_t)ntiles * (NT * sizeof(float2) + sizeof(double));
   l.fixed = 2 * l.acc_bytes + l.xi_bytes + 2 * 256;
   return l;
@@ -755,9 +881,6 @@ extern "C" int geraf_mf_backward_tiled_f32(const float* grad_out_real, const flo
     throw ArgError("A*K and N must be < 2^31");
   const int NT = round_terms(nterms);
   if (nterms < 1 || NT < 0) throw ArgError("nterms must be in [1, 32]");
-  int kpl = 1;
-  while (kpl * kWave < K) kpl *= 2;
-  if (kpl > 32) throw ArgError("the tiled adjoint takes at most 2048 samples per antenna");
   require_ptrs({grad_out_real, grad_out_imag, tx_pos, rx_pos, grad_real, grad_imag, voxel,
                 tile_info, workspace},
                true);
@@ -770,13 +893,18 @@ extern "C" int geraf_mf_backward_tiled_f32(const float* grad_out_real, const flo
   const double kc = 0.5 * (double)(K - 1);
   const float4* ti = reinterpret_cast<const float4*>(tile_info);
   const double inv_norm = 1.0 / (double)norm_count;
+  {
+    const int ops = (int)((K + 15) / 16) * kWave;
+    hipLaunchKernelGGL(xi_adj_ops_kernel, dim3((unsigned)((ops + 255) / 256)), dim3(256), 0, st,
+                       b.xi_ops, (int)K, NT, kc);
+  }
   for (int a0 = 0; a0 < A; a0 += b.na_max) {
     const int na = (int)std::min<int64_t>(b.na_max, A - a0);
 #define GERAF_TILED_ADJ(T)                                                                   \
   case T:                                                                                    \
-    launch_adj_chunk_k<T>(kpl, grad_out_real, grad_out_imag, tx_pos, rx_pos, voxel, ti, a0,  \
-                          na, (int)K, (int)N, ntiles, tv, pc, kc, inv_norm, b.coef, b.tau_c, \
-                          grad_real, grad_imag, st);                                         \
+    launch_adj_chunk<T>(grad_out_real, grad_out_imag, tx_pos, rx_pos, voxel, ti, b.xi_ops, a0, \
+                        na, (int)K, (int)N, ntiles, tv, pc, kc, inv_norm, b.coef, b.tau_c,     \
+                        grad_real, grad_imag, st);                                             \
     break;
     switch (NT) {
       GERAF_TILED_ADJ(4)
@@ -793,6 +921,6 @@ extern "C" int geraf_mf_backward_tiled_f32(const float* grad_out_real, const flo
     }
 #undef GERAF_TILED_ADJ
   }
-  return check_launch("mf_tile_adj_eval_kernel");
+  return check_launch("mf_tile_adj_eval_mma_kernel");
   GERAF_TRY_END
 }
