@@ -460,7 +460,10 @@ __global__ __launch_bounds__(256) void mf_tile_adj_moments_kernel(
 // (kb, r = 0..3) and rotates one phasor seed per tile along those 16
 // frequencies; products are summed in FP32 over kAdjFlush tile blocks and
 // then in FP64, and the 16 lanes of a row (its tiles) are summed at the end.
-// One wave per (antenna, 64 frequencies).
+// One wave per (antenna, 64 frequencies); the waves of a block (up to four)
+// share an antenna and each tile block's B fragments and phase steps, built
+// once per block (wave w of W builds tile blocks w, w + W, ...) in
+// double-buffered LDS.
 constexpr int kAdjKB = 4;      // 16-frequency blocks per wave (one per 4 lanes of a row)
 constexpr int kAdjFlush = 8;   // tile blocks summed in FP32 before each FP64 add
 
@@ -501,21 +504,37 @@ __device__ __forceinline__ float column_max4(float x) {
   return fmaxf(__uint_as_float(s16[0]), __uint_as_float(s16[1]));
 }
 
+// One tile block's operands, shared by the block's four waves: the B
+// fragments (hi and lo of the real and imaginary parts, per lane), and per
+// tile the phase step d tau_c, the one- and sixteen-sample rotations and the
+// power of two that undoes the column's scale.
+struct AdjTileOps {
+  h8 b[4][kWave];
+  double step[16];
+  float2 w1[16], w16[16];
+  float up[16];
+};
+constexpr int kAdjWaves = 4;  // waves per block: one antenna, 4 x 64 frequencies
+
 template <int NT>
-__global__ __launch_bounds__(256) void mf_tile_adj_eval_mma_kernel(
+__global__ __launch_bounds__(64 * kAdjWaves) void mf_tile_adj_eval_mma_kernel(
     const float2* __restrict__ mom, const double* __restrict__ tau_c,
     const float4* __restrict__ tile_info, const h8* __restrict__ xi_ops, int a0, int na, int K,
     int ntiles, PhaseConsts pc, double kc, double inv_norm, float* __restrict__ grad_re,
     float* __restrict__ grad_im) {
   static_assert(NT <= 32, "one contraction step of 32 terms");
   static_assert(kAdjKB == 4, "the 16 lanes of a row write the 16 frequencies (kb, r)");
+  // [stage][tile block of the group]: wave w builds entry w of the next
+  // stage while all four read the current one
+  __shared__ AdjTileOps ops[2][kAdjWaves];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nkw = (K + 16 * kAdjKB - 1) / (16 * kAdjKB);  // waves per antenna
-  const int wid = blockIdx.x * 4 + wave;
-  if (wid >= na * nkw) return;  // wave-uniform; no block barriers below
-  const int ai = wid / nkw;
-  const int kw = (wid - ai * nkw) * 16 * kAdjKB;  // first frequency of the wave
+  const int W = blockDim.x / kWave;  // waves of the block, 1..kAdjWaves
+  const int nkw = (K + 16 * kAdjKB - 1) / (16 * kAdjKB);  // 64-frequency ranges
+  const int nbk = (nkw + W - 1) / W;                     // blocks per antenna
+  const int ai = blockIdx.x / nbk;                       // < na (grid = na * nbk)
+  const int kw = ((blockIdx.x - ai * nbk) * W + wave) * 16 * kAdjKB;
+  const bool active = kw < K;  // wave-uniform; idle waves still build and join barriers
   const int a = a0 + ai;
   const int j = lane & 15, q = lane >> 4;
   const int nkb = (K + 15) / 16;
@@ -532,18 +551,11 @@ __global__ __launch_bounds__(256) void mf_tile_adj_eval_mma_kernel(
   }
   const double zfac = 3.14159265358979323846 * (double)(K - 1) * pc.dfs;
   const double k0 = (double)(kw + 4 * q) - kc;  // the lane's first frequency, centred
-  float2 part[kAdjKB][4];
-  double accr[kAdjKB][4], acci[kAdjKB][4];
-#pragma unroll
-  for (int kb = 0; kb < kAdjKB; ++kb)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      part[kb][r] = make_float2(0.0f, 0.0f);
-      accr[kb][r] = acci[kb][r] = 0.0;
-    }
   const int ntb = (ntiles + 15) / 16;
-  // the inputs of tile block tb: lane (j, q) loads tile j's moments 8 q ..
-  // 8 q + 7, its radius and its centre delay (clamped to the last tile)
+  const int ngroups = (ntb + W - 1) / W;  // W tile blocks per group
+
+  // the inputs of tile block tb (tiles past the last one clamped; their B
+  // columns are zeroed in build)
   auto load = [&](int tb, float2 (&m)[8], float& rad, double& tac) {
     const int t = min(tb * 16 + j, ntiles - 1);
     const size_t ct = (size_t)ai * ntiles + t;
@@ -552,20 +564,13 @@ __global__ __launch_bounds__(256) void mf_tile_adj_eval_mma_kernel(
     rad = tile_info[t].w;
     tac = tau_c[ct];
   };
-  float2 nm[8];
-  float nrad;
-  double ntac;
-  load(0, nm, nrad, ntac);
-  for (int tb = 0; tb < ntb; ++tb) {
+  // B = Z^n M'_n for n = 8 q + e (each tile's column scaled into [0.5, 1)),
+  // and the tile's phase steps
+  auto build = [&](int tb, const float2 (&m)[8], float rad, double tac, AdjTileOps& o) {
     const bool ok = tb * 16 + j < ntiles;
     float2 v[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-      v[e] = (ok && 8 * q + e < NT) ? nm[e] : make_float2(0.0f, 0.0f);
-    const float rad = nrad;
-    const double tac = ntac;
-    if (tb + 1 < ntb) load(tb + 1, nm, nrad, ntac);  // in flight during this block
-    // B: Z^n M'_n for n = 8 q + e, the tile's column scaled into [0.5, 1)
+    for (int e = 0; e < 8; ++e) v[e] = (ok && 8 * q + e < NT) ? m[e] : make_float2(0.0f, 0.0f);
     const float Z = (float)(zfac * 2.0 * (double)rad * pc.inv_c);
     const float z2 = Z * Z, z4 = z2 * z2, z8 = z4 * z4;
     float zp = q == 0 ? 1.0f : (q == 1 ? z8 : (q == 2 ? z8 * z8 : z8 * z8 * z8));
@@ -585,40 +590,86 @@ __global__ __launch_bounds__(256) void mf_tile_adj_eval_mma_kernel(
     for (int e = 0; e < 8; ++e) v[e] = make_float2(v[e].x * down, v[e].y * down);
     h8 brh, brl, bih, bil;
     split8(v, brh, brl, bih, bil);
-    // the phasor of the lane's first frequency for tile j, with the scale back
-    // (polynomial sine and cosine: the three phasors set the accuracy of all
-    // 16 products of the lane)
-    const double step = pc.dfs * tac;
-    float2 ph = phasor_cycles((float)centered_frac(k0 * step), 1.0f);
-    const float up = ldexpf(1.0f, ex);
-    ph = make_float2(ph.x * up, ph.y * up);
-    const float2 w1 = phasor_cycles((float)centered_frac(step), 1.0f);
-    const float2 w16 = phasor_cycles((float)centered_frac(16.0 * step), 1.0f);
-#pragma unroll
-    for (int kb = 0; kb < kAdjKB; ++kb) {
-      const f32x4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
-      const f32x4v qr = mma3(ah[kb], al[kb], brh, brl, zero);
-      const f32x4v qi = mma3(ah[kb], al[kb], bih, bil, zero);
-      float2 p = ph;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        part[kb][r].x = fmaf(p.x, qr[r], fmaf(-p.y, qi[r], part[kb][r].x));
-        part[kb][r].y = fmaf(p.x, qi[r], fmaf(p.y, qr[r], part[kb][r].y));
-        if (r < 3) p = cmul(p, w1);
-      }
-      if (kb + 1 < kAdjKB) ph = cmul(ph, w16);
+    o.b[0][lane] = brh;
+    o.b[1][lane] = brl;
+    o.b[2][lane] = bih;
+    o.b[3][lane] = bil;
+    if (q == 0) {
+      // polynomial sine and cosine: these phasors set the accuracy of all
+      // the products of the tile
+      const double step = pc.dfs * tac;
+      o.step[j] = step;
+      o.w1[j] = phasor_cycles((float)centered_frac(step), 1.0f);
+      o.w16[j] = phasor_cycles((float)centered_frac(16.0 * step), 1.0f);
+      o.up[j] = ldexpf(1.0f, ex);
     }
-    if ((tb + 1) % kAdjFlush == 0 || tb + 1 == ntb) {
+  };
+
+  float2 part[kAdjKB][4];
+  double accr[kAdjKB][4], acci[kAdjKB][4];
 #pragma unroll
-      for (int kb = 0; kb < kAdjKB; ++kb)
+  for (int kb = 0; kb < kAdjKB; ++kb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          accr[kb][r] += (double)part[kb][r].x;
-          acci[kb][r] += (double)part[kb][r].y;
-          part[kb][r] = make_float2(0.0f, 0.0f);
+    for (int r = 0; r < 4; ++r) {
+      part[kb][r] = make_float2(0.0f, 0.0f);
+      accr[kb][r] = acci[kb][r] = 0.0;
+    }
+  float2 nm[8];
+  float nrad;
+  double ntac;
+  load(wave, nm, nrad, ntac);  // group 0's block of this wave
+  build(wave, nm, nrad, ntac, ops[0][wave]);
+  if (W + wave < ntb) load(W + wave, nm, nrad, ntac);  // group 1's
+  __syncthreads();
+  for (int g = 0; g < ngroups; ++g) {
+    const int cur = g & 1;
+    if (g + 1 < ngroups) {
+      // stage cur ^ 1 was last read before the previous barrier
+      const int tbn = (g + 1) * W + wave;
+      build(tbn, nm, nrad, ntac, ops[cur ^ 1][wave]);
+      if (tbn + W < ntb) load(tbn + W, nm, nrad, ntac);  // in flight
+    }
+    if (active) {
+#pragma unroll 1
+      for (int i = 0; i < W; ++i) {
+        const int tb = g * W + i;  // uniform
+        if (tb >= ntb) break;
+        const AdjTileOps& o = ops[cur][i];
+        const h8 brh = o.b[0][lane], brl = o.b[1][lane], bih = o.b[2][lane],
+                 bil = o.b[3][lane];
+        const float2 w1 = o.w1[j], w16 = o.w16[j];
+        const float up = o.up[j];
+        float2 ph = phasor_cycles((float)centered_frac(k0 * o.step[j]), 1.0f);
+        ph = make_float2(ph.x * up, ph.y * up);
+#pragma unroll
+        for (int kb = 0; kb < kAdjKB; ++kb) {
+          const f32x4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
+          const f32x4v qr = mma3(ah[kb], al[kb], brh, brl, zero);
+          const f32x4v qi = mma3(ah[kb], al[kb], bih, bil, zero);
+          float2 p = ph;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            part[kb][r].x = fmaf(p.x, qr[r], fmaf(-p.y, qi[r], part[kb][r].x));
+            part[kb][r].y = fmaf(p.x, qi[r], fmaf(p.y, qr[r], part[kb][r].y));
+            if (r < 3) p = cmul(p, w1);
+          }
+          if (kb + 1 < kAdjKB) ph = cmul(ph, w16);
         }
+        if ((tb + 1) % kAdjFlush == 0 || tb + 1 == ntb) {
+#pragma unroll
+          for (int kb = 0; kb < kAdjKB; ++kb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              accr[kb][r] += (double)part[kb][r].x;
+              acci[kb][r] += (double)part[kb][r].y;
+              part[kb][r] = make_float2(0.0f, 0.0f);
+            }
+        }
+      }
     }
+    __syncthreads();  // stage cur is read; stage cur ^ 1 is built
   }
+  if (!active) return;  // after the last barrier
   // sum the row's 16 lanes (tiles j); lane j of row q then writes
   // k = k_w + 16 (j >> 2) + 4 q + (j & 3)
   double outr = 0.0, outi = 0.0;
@@ -662,10 +713,13 @@ void launch_adj_chunk(const float* gre, const float* gim, const float* tx, const
     hipLaunchKernelGGL((mf_tile_adj_moments_kernel<NT, 2>), mgrid, dim3(256), 0, stream, gre,
                        gim, tx, rx, voxel, tile_info, a0, na, N, ntiles, pc, fcen, mom, tau_c,
                        per_group);
-  const int64_t waves = (int64_t)na * ((K + 16 * kAdjKB - 1) / (16 * kAdjKB));
-  hipLaunchKernelGGL((mf_tile_adj_eval_mma_kernel<NT>), dim3((unsigned)((waves + 3) / 4)),
-                     dim3(256), 0, stream, mom, tau_c, tile_info, xi_ops, a0, na, K, ntiles, pc,
-                     kc, inv_norm, grad_re, grad_im);
+  // up to four waves per block, all on one antenna (fewer when K < 256)
+  const int nkw = (K + 16 * kAdjKB - 1) / (16 * kAdjKB);
+  const int W = std::min(kAdjWaves, nkw);
+  const int64_t blocks = (int64_t)na * ((nkw + W - 1) / W);
+  hipLaunchKernelGGL((mf_tile_adj_eval_mma_kernel<NT>), dim3((unsigned)blocks),
+                     dim3(64 * W), 0, stream, mom, tau_c, tile_info, xi_ops, a0, na, K,
+                     ntiles, pc, kc, inv_norm, grad_re, grad_im);
 }
 
 // Instantiated term counts; a request is rounded up to the next one.
