@@ -338,7 +338,7 @@ int geraf_mf_forward_tiled_f32(const float* real, const float* imag, const float
  * of geraf_mf_backward_f32, i.e. _C.MatchedFilter_backward_realimag,
  * vision.cpp:75-80; MatchedFilter_cuda_kernel.cu:171-220), with the same
  * truncation bound; no counterpart in the reference.  grad_out_real/imag [N]
- * in the tile order of voxel; grad [A, K] divided by norm_count; K <= 2048. */
+ * in the tile order of voxel; grad [A, K] divided by norm_count. */
 int geraf_mf_backward_tiled_f32(const float* grad_out_real, const float* grad_out_imag,
                                 const float* tx_pos, const float* rx_pos, float* grad_real,
                                 float* grad_imag, const float* voxel, const float* tile_info,
