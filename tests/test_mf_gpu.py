@@ -669,6 +669,44 @@ def test_tiled_adjoint_matches_direct_and_is_the_transpose(cuda, which, tile):
     assert fwd <= 1e-5, fwd
 
 
+@pytest.mark.parametrize("K,tile", [(2, 512), (37, 512), (100, 512), (300, 512), (1100, 512),
+                                    (1100, 128)])
+def test_tiled_pair_vs_fp64_oracle_over_k(cuda, K, tile):
+    """Tiled forward and adjoint against the FP64 oracle for K that is not a
+    multiple of the kernels' steps: partial 32-sample steps of the moments,
+    1, 2 and 4 waves of 64 frequencies per block in the adjoint evaluation,
+    a second 1024-sample echo chunk (K = 1100).  The sample rate grows with K
+    so that the bandwidth, and so the term count, stays that of the
+    config-1 chirp; 64 antennas, 2,048 voxels."""
+    from geraf.ops.RFMatchedFilter import _C as mfC
+    from geraf.ops.RFMatchedFilter.tiled_matched_filter import (TilePlan, tiled_adjoint,
+                                                                tiled_forward)
+    sc = mf_scenes.config1(0)
+    A, N = 64, 2048
+    fs = sc.fs * max(1.0, (K - 1) / (sc.K - 1))
+    prm = dict(sc.params(), fs=fs)
+    rng = np.random.default_rng(K)
+    echo = (rng.standard_normal((A, K)) + 1j * rng.standard_normal((A, K))).astype(np.complex64)
+    vox = np.ascontiguousarray(sc.voxels[5000:5000 + N])
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), device=cuda)  # noqa: E731
+    re, im, tx, rx, v = t(echo.real), t(echo.imag), t(sc.tx[:A]), t(sc.rx[:A]), t(vox)
+    plan = TilePlan(v, K, mfC._params(fs, sc.As_sci), 1e-6, tile=tile)
+    assert plan.usable and plan.tiled_fraction == 1.0, (K, tile, plan.tiled_fraction)
+    o_r, o_i, info = tiled_forward(re, im, tx, rx, v, K, fs, sc.As_sci, 1e-6, plan=plan)
+    assert info["path"] == "tiled"
+    ref = orc.mf_forward(echo.real, echo.imag, sc.tx[:A], sc.rx[:A], vox, **prm)
+    got = torch.complex(o_r.double(), o_i.double()).cpu().numpy()
+    frel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    g = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    a_r, a_i = tiled_adjoint(t(g.real), t(g.imag), tx, rx, plan, A, K, A, fs, sc.As_sci, v)
+    aref = orc.mf_backward(g, sc.tx[:A], sc.rx[:A], vox, K, norm=A, **prm)
+    agot = torch.complex(a_r.double(), a_i.double()).cpu().numpy()
+    arel = np.linalg.norm(agot - aref) / np.linalg.norm(aref)
+    print(f"K={K} tile {tile}: {plan.terms} terms; forward rel-L2 {frel:.2e}, adjoint {arel:.2e}",
+          flush=True)
+    assert frel <= 1e-5 and arel <= 1e-5, (frel, arel)
+
+
 @pytest.mark.parametrize("tile", [512, 128])
 def test_tiled_mixed_plan_forward_and_adjoint(cuda, tile):
     """A compact block plus scattered far points: the block's tiles run the
