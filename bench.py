@@ -325,9 +325,11 @@ def bench_adjoint(dev, sc, re, im, tx, rx, vox, g_r, g_i, prm, sel):
             "workload": "matched-filter adjoint, 64x64 MIMO, K=512, 128^3 voxels"}
 
 
-def bench_stage_step(dev, target_volume, axes, steps: int = 5, warmup: int = 2):
+def bench_stage_step(dev, target_volume, axes, steps: int = 5, warmup: int = 3):
     """Stage-1-style training step (8,192 rays x 64 samples, valid counts
-    16..64, K=64), fused form vs the all-FP32 autograd form."""
+    16..64, K=64), fused form vs the all-FP32 autograd form.  Three warm-up
+    steps: the third step of a fresh process carries a one-time ~70 ms (the
+    first two do not reach it), the steps after it are steady."""
     import torch
 
     from rf_workloads import stage_step as st
